@@ -1,0 +1,132 @@
+"""Benchmark of the fused hybrid-PIC time step on one MI355X.
+
+Workload: 256x128x128 cells x 64 ppc (268,435,456 protons), drifting
+Maxwellian (vsw = 400 km/s, vth = 30 km/s), uniform B0 along z,
+Omega_i dt = 0.05, 10 field subcycles per step, periodic boundaries, FP32
+particle state (split-precision positions), FP64 B advance.  A "step" is one
+full swap_step: the three particle passes, the counting sort when due, and
+all field subcycles.  Particles are generated on the device before timing.
+
+Prints one JSON line (the driver's contract) with the roofline of the
+dominant kernel and a CPU baseline (the FP64 oracle on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s incl. deposit, % HBM peak; s/step vs plain at equal error"
+
+# algorithmic bytes of each particle pass: per particle (SoA state read and
+# written) and per grid node (moments written, 6-component gather source read)
+PASS_BYTES = {"P1": (40, 16), "P2": (28, 12 + 24), "P3": (68, 16 + 24)}
+
+
+def cpu_baseline(steps=15):
+    """FP64 oracle (1 thread) on the 64x32x32 x 8 ppc workload, `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    from swap_amd import physics
+    p = physics.workload(64, 32, 32, 8)
+    n = p["npart"]
+    cell, frac, v = O.load_maxwellian(n, 20261203, (64, 32, 32), p["vsw"], p["vth"])
+    i, j, k = cell & 2047, (cell >> 11) & 1023, cell >> 21
+    xp = np.stack([(i + 1 + frac[0]) * p["dx"], (j + 1 + frac[1]) * p["dx"],
+                   (k + 1 + frac[2]) * p["dx"]])
+    o = O.Oracle(66, 34, 34, p["dx"], p["dx"], p["dx"], p["dt"], 10, p["alpha"], p["beta"])
+    st = {nm: o.grid() for nm in ("b1", "b12", "b1p2", "bt", "btc", "up", "aj", "E")}
+    for b in ("b1", "b12", "b1p2", "bt"):
+        st[b][2] = p["b0z"]
+    st.update(np=o.grid(1), nu=np.full((34, 34, 66), p["nu"]), xp=np.ascontiguousarray(xp),
+              vp=np.ascontiguousarray(v), vp1=v.copy(), vplus=o.parts(n), vminus=o.parts(n),
+              Ep=o.parts(n), wght=o.parts(n, 8), ijkp=np.zeros((3, n), np.int32), Ni_tot=n)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(st)
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle (oracle/swap_oracle.c, 1 thread), 64x32x32 cells x 8 ppc "
+                      "(524,288 protons), %d full steps incl. 10 field subcycles, %.1f s"
+                      % (steps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, nargs=3, default=[256, 128, 128])
+    ap.add_argument("--ppc", type=int, default=64)
+    ap.add_argument("--sort-every", type=int, default=2)
+    ap.add_argument("--precision", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from swap_amd import physics
+    ncx, ncy, ncz = args.cells
+    p = physics.workload(ncx, ncy, ncz, args.ppc)
+    npart = p["npart"]
+    nn = (ncx + 2) * (ncy + 2) * (ncz + 2)
+    s = physics.make_swap(p, precision=args.precision, sort_every=args.sort_every)
+    s.load_maxwellian(npart, p["vsw"], p["vth"])
+    print("bench: loaded %d particles on %dx%dx%d cells" % (npart, ncx, ncy, ncz),
+          file=sys.stderr, flush=True)
+    s.step(args.warmup)
+    s.sync()
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    prof = s.profile_read()
+    s.profile(False)
+    diag = s.diagnostics()
+    ms_step = elapsed * 1e3 / args.steps
+    value = npart * args.steps / elapsed
+
+    # dominant particle pass and its roofline
+    dom = max(PASS_BYTES, key=lambda k: prof[k][0])
+    ms_tot, cnt = prof[dom]
+    bp, bn = PASS_BYTES[dom]
+    alg = bp * npart + bn * nn
+    achieved = alg / (ms_tot / cnt * 1e-3) / 1e9
+    step_alg = 136 * npart + 116 * nn
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32" if args.precision == 32 else "fp64",
+        "data": "synthetic: seeded drifting Maxwellian (Philox), uniform B0",
+        "config": {"workload": "%dx%dx%d cells x %d ppc (%d protons), 10 B subcycles/step, "
+                               "periodic, split-precision FP32 particles, FP64 B advance"
+                               % (ncx, ncy, ncz, args.ppc, npart),
+                   "cells": [ncx, ncy, ncz], "ppc": args.ppc, "n_particles": npart,
+                   "sort_every": args.sort_every, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_pass<float,%s>" % dom[1],
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg,
+                     "avg_launch_ms": ms_tot / cnt},
+        "step_roofline": {"alg_bytes": step_alg, "achieved_GBs": step_alg / (ms_step * 1e-3) /
+                          1e9, "frac": step_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "stages_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+        "window_misses": diag["window_misses"],
+    }
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

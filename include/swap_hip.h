@@ -1,0 +1,233 @@
+/*
+ * swap_hip.h -- C-ABI of the MI355X (gfx950) hybrid-PIC library.
+ *
+ * Each entry point replaces the body of one routine of the reference's
+ * Fortran time step; the citation on each declaration names the reference
+ * routine it stands in for.  The Fortran side keeps the reference's
+ * subroutine names and argument lists and reaches these functions through
+ * ISO_C_BINDING (swap_amd/fortran/swap_f.f90).
+ *
+ * Conventions shared by every function:
+ *  - return 0 on success, a negative SWAP_E* code on failure; the text of
+ *    the last failure is swap_last_error(ctx).  No C++ exception crosses
+ *    this boundary.
+ *  - work is enqueued on the context's own HIP stream; functions that hand
+ *    data back to host memory (swap_get_*, swap_check_time_step, diagnostics)
+ *    synchronise that stream before returning.
+ *  - state is device resident.  Host arrays are mirrors moved only by the
+ *    explicit swap_get_* / swap_set_* calls.
+ *  - grid arrays use the reference's Fortran layouts: scalars (nx,ny,nz),
+ *    vectors (nx,ny,nz,3), with nx = ncx+2 etc. (two ghost layers,
+ *    dimensions.f:5).  Particle arrays are (n,3) column-major, i.e. the
+ *    reference's (Ni_max,3) with Ni_max = n.
+ *  - a context is not thread safe; separate contexts are independent.
+ */
+#ifndef SWAP_HIP_H
+#define SWAP_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct swap_ctx swap_ctx;
+
+enum {
+  SWAP_OK = 0,
+  SWAP_EINVAL = -1,   /* bad argument */
+  SWAP_ENOMEM = -2,   /* device allocation failed */
+  SWAP_EHIP = -3,     /* HIP runtime error */
+  SWAP_ECAP = -4,     /* particle capacity exceeded */
+  SWAP_ESTATE = -5    /* call out of order (e.g. no particles loaded) */
+};
+
+/* precision of particle state, moments and E (B is always advanced in FP64) */
+enum { SWAP_FP32 = 32, SWAP_FP64 = 64 };
+
+/* config flags */
+enum {
+  SWAP_SPECIES = 1,        /* per-particle beta_p and mrat arrays exist */
+  SWAP_OUTFLOW_X = 2,      /* x is inflow/outflow instead of periodic */
+  SWAP_TRACK_IDS = 4       /* keep a particle id through sorts (tests) */
+};
+
+typedef struct {
+  int ncx, ncy, ncz;       /* physical cells per axis (reference nx-2 ...) */
+  double dx, dy, dz;       /* uniform cell size, km (inputs.f:136-138) */
+  double dt;               /* particle step, s (inputs.f:144) */
+  int ntsub;               /* field subcycles per step (inputs.f:27) */
+  double alpha;            /* (mu0/1e3) q^2/m_ion (inputs.f:170) */
+  double beta;             /* particles per physical density unit */
+  double nu;               /* collision frequency (grd6_setup, initial.f:79) */
+  int64_t capacity;        /* particle slots (reference Ni_max) */
+  int precision;           /* SWAP_FP32 or SWAP_FP64 */
+  int flags;               /* SWAP_SPECIES | SWAP_OUTFLOW_X | ... */
+  int sort_every;          /* fused step: counting sort every k steps (0: never) */
+  uint64_t seed;           /* counter-based RNG key */
+} swap_config;
+
+/* field ids for swap_get_field / swap_set_field / swap_periodic / curls */
+enum {
+  SWAP_F_B1 = 0,    /* b1 == bt (f_update_tlev keeps them equal) */
+  SWAP_F_B12 = 1,
+  SWAP_F_B1P2 = 2,
+  SWAP_F_BTC = 3,   /* node B, edge_to_center(bt) */
+  SWAP_F_NP = 4,    /* scalar */
+  SWAP_F_UP = 5,
+  SWAP_F_AJ = 6,
+  SWAP_F_E = 7,
+  SWAP_F_NU = 8,    /* scalar */
+  SWAP_F_AJC = 9,   /* node aj */
+  SWAP_F_UPC = 10,  /* node up */
+  SWAP_F_TMP0 = 11, /* general vector scratch fields for the routine-level */
+  SWAP_F_TMP1 = 12, /* API (curlE output, face/edge interpolation results) */
+  SWAP_F_TMP2 = 13,
+  SWAP_F_COUNT = 14
+};
+
+/* particle arrays for swap_get_particle_array */
+enum {
+  SWAP_P_XP = 0,     /* absolute position (n,3), km, FP64 on the host side */
+  SWAP_P_VP = 1,     /* (n,3) */
+  SWAP_P_VP1 = 2,    /* (n,3) */
+  SWAP_P_EP = 3,     /* (n,3)  get_Ep output */
+  SWAP_P_VPLUS = 4,  /* (n,3) */
+  SWAP_P_VMINUS = 5, /* (n,3) */
+  SWAP_P_IJKP = 6,   /* (n,3) int32, 1-based cell indices */
+  SWAP_P_WGHT = 7,   /* (n,8) */
+  SWAP_P_CELL = 8,   /* (n) uint32 packed i | j<<11 | k<<21, 0-based cells */
+  SWAP_P_FRAC = 9,   /* (n,3) in-cell fraction, (0,1] */
+  SWAP_P_BETA = 10,  /* (n) beta_p */
+  SWAP_P_MRAT = 11,  /* (n) mrat */
+  SWAP_P_ID = 12     /* (n) uint32 particle id (SWAP_TRACK_IDS) */
+};
+
+/* update_up velocity sources */
+enum { SWAP_V_CUR = 0, SWAP_V_EXTRAP = 1, SWAP_V_VP1 = 2 };
+
+typedef struct {
+  double charge;        /* sum 1/(beta beta_p) */
+  double momentum[3];   /* sum m v /(beta beta_p), kg km/s (m = m_p/mrat) */
+  double kinetic;       /* sum 1/2 m |v|^2 /(beta beta_p), J */
+  double divb_l2;       /* sqrt(sum_cells (div b1)^2) */
+  double b_energy;      /* sum_cells |b1|^2 (q/m_p units) */
+  int64_t nparticles;
+  int64_t escaped;      /* cumulative outflow removals */
+  int64_t injected;     /* cumulative injections */
+  int64_t window_misses;/* particles that took the global-atomic deposit path */
+} swap_diag;
+
+/* ---- lifecycle (replaces dimensions.f/global.f static state) ---- */
+int swap_create(const swap_config *cfg, swap_ctx **out);
+int swap_destroy(swap_ctx *ctx);
+const char *swap_last_error(const swap_ctx *ctx);
+int swap_sync(swap_ctx *ctx);
+int64_t swap_num_particles(const swap_ctx *ctx);
+/* geometry: qx(i) etc. as the device uses them (grd7, initial.f:147-301) */
+int swap_get_coords(swap_ctx *ctx, int axis, double *q /* n_axis+2 */);
+
+/* ---- particles ---- */
+/* reference layout: xp(n,3) absolute positions in (q(1), q(n-1)],
+   vp(n,3), vp1(n,3); beta_p, mrat may be NULL (then 1).  Cell search follows
+   get_interp_weights (gutsp.f:1391-1412): q(i) < x <= q(i+1). */
+int swap_load_particles(swap_ctx *ctx, int64_t n, const double *xp, const double *vp,
+                        const double *vp1, const double *beta_p, const double *mrat);
+/* split form: packed cells and fractions as stored on the device */
+int swap_load_particles_split(swap_ctx *ctx, int64_t n, const uint32_t *cell,
+                              const void *frac, const void *vp, const void *vp1,
+                              const float *beta_p, const float *mrat);
+/* seeded drifting-Maxwellian load, the sampler of load_Maxwellian
+   (part_init.f:176-206) driven by Philox-4x32-10 keyed by (seed, id) */
+int swap_load_maxwellian(swap_ctx *ctx, int64_t n, double vsw, double vth,
+                         double beta_p, double mrat);
+/* host copy of one particle array (see SWAP_P_*); dtype: FP64 for XP, the
+   context precision for real arrays, uint32/int32 for integer arrays */
+int swap_get_particle_array(swap_ctx *ctx, int which, void *host);
+
+/* ---- fields ---- */
+int swap_set_field(swap_ctx *ctx, int field, const double *host);
+int swap_get_field(swap_ctx *ctx, int field, double *host);
+/* the same for REAL*4 host arrays, as the reference declares them */
+int swap_set_field_f32(swap_ctx *ctx, int field, const float *host);
+int swap_get_field_f32(swap_ctx *ctx, int field, float *host);
+/* periodic / periodic_scalar (boundary.f:9-52, 167-206) */
+int swap_periodic(swap_ctx *ctx, int field);
+
+/* ---- routine-level API (one call per reference routine) ---- */
+int swap_get_interp_weights(swap_ctx *ctx);        /* gutsp.f:1378 */
+int swap_update_np(swap_ctx *ctx);                 /* gutsp.f:1441 */
+int swap_update_up(swap_ctx *ctx, int vsrc);       /* gutsp.f:1649 */
+int swap_extrapol_up(swap_ctx *ctx);               /* gutsp.f:127 */
+int swap_get_Ep(swap_ctx *ctx);                    /* gutsp.f:155 */
+int swap_get_vplus_vminus(swap_ctx *ctx);          /* gutsp.f:281 */
+int swap_improve_up(swap_ctx *ctx);                /* gutsp.f:407 */
+int swap_get_vp_final(swap_ctx *ctx);              /* gutsp.f:433 */
+int swap_move_ion_half(swap_ctx *ctx);             /* gutsp.f:454 */
+int swap_curlB(swap_ctx *ctx, int b_in, int aj_out);      /* gutsf.f:178 */
+int swap_curlE(swap_ctx *ctx, int e_in, int out);         /* gutsf.f:249 */
+int swap_crossf2(swap_ctx *ctx, int aa, int btc, int out);/* gutsf.f:40 */
+int swap_get_E(swap_ctx *ctx);                     /* gutsf.f:994 */
+int swap_predict_B(swap_ctx *ctx);                 /* gutsf.f:1115 */
+int swap_get_Ep1(swap_ctx *ctx);                   /* gutsf.f:1174 */
+int swap_correct_B(swap_ctx *ctx);                 /* gutsf.f:1309 */
+int swap_f_update_tlev(swap_ctx *ctx);             /* gutsf.f:10 */
+/* gutsf.f:1776; resets dtsub=dt/ntsub, ntf=ntsub first as maind.f:420-421
+   does, returns the adjusted values */
+int swap_check_time_step(swap_ctx *ctx, int *ntf, double *dtsub);
+int swap_edge_to_center(swap_ctx *ctx, int in, int out);  /* grid_interp.f:10 */
+int swap_edge_to_face(swap_ctx *ctx, int in, int out);    /* grid_interp.f:67 */
+int swap_face_to_center(swap_ctx *ctx, int in, int out);  /* grid_interp.f:136 */
+
+/* ---- fused time step (maind.f:385-479 as three particle passes) ---- */
+/* curlB .. f_update_tlev for ntf subcycles (maind.f:425-476) */
+int swap_field_subcycles(swap_ctx *ctx, int ntf, double dtsub);
+/* counting sort of particles by tile/cell (build-only) */
+int swap_sort(swap_ctx *ctx);
+/* moments at the current positions: np and extrapolated up (the first
+   get_interp_weights/update_np/extrapol_up of maind.f:385-399) */
+int swap_prime(swap_ctx *ctx);
+/* nsteps full steps; swap_prime must have run after the last load */
+int swap_step(swap_ctx *ctx, int nsteps);
+
+/* ---- sources (chem_rates.f) ---- */
+/* Ionize_sw_mp, chem_rates.f:508-646: ring-distribution pickup protons,
+   uniform in the domain, beta_p = beta_pu; returns the count injected */
+int swap_ionize_sw(swap_ctx *ctx, int step, double ndot, double vsw, double beta_pu,
+                   int64_t *injected);
+/* spherical neutral cloud (chem_rates.f:290-503 form with the exp(-r/H)
+   profile), photoionization + charge exchange (res_chex chem_rates.f:805) */
+typedef struct {
+  double cx, cy, cz;     /* cloud centre, km */
+  double N0, r0, H;      /* n_n(r) = N0 (r0/r)^2 exp(-r/H), km^-3, km, km */
+  double tau_photo;      /* s */
+  double sigma_chex;     /* km^2 (chem_rates.f:817: 8e-26) */
+  double mrat_pu;        /* 1/28 */
+  double beta_min;       /* smallest macro-weight beta_p */
+  int max_per_cell;      /* cap of new macro ions per cell per step */
+  double rmin;           /* cloud core cut-off radius, km */
+} swap_cloud;
+int swap_ionize_cloud(swap_ctx *ctx, int step, const swap_cloud *cloud, int64_t *injected,
+                      int64_t *chex);
+
+/* ---- diagnostics (FP64 reductions; Energy_diag part_init.f:14-139) ---- */
+int swap_diagnostics(swap_ctx *ctx, swap_diag *out);
+
+/* ---- build-only instrumentation: HIP events on the context stream around
+   each stage of swap_step; no reference counterpart (the reference times
+   only the whole run, maind.f:648-654) ---- */
+enum {
+  SWAP_K_P1 = 0,      /* particle pass 1 */
+  SWAP_K_P2 = 1,      /* particle pass 2 */
+  SWAP_K_P3 = 2,      /* particle pass 3 */
+  SWAP_K_SORT = 3,    /* counting sort */
+  SWAP_K_FIELD = 4,   /* all field subcycles of a step */
+  SWAP_K_GLUE = 5,    /* moment finalisation and node sources */
+  SWAP_K_COUNT = 6
+};
+int swap_profile(swap_ctx *ctx, int enable);  /* enabling clears the totals */
+/* total milliseconds and launch counts per stage since enabling */
+int swap_profile_read(swap_ctx *ctx, double *ms, int64_t *count, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""ctypes binding of the CPU oracle (oracle/swap_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package swap_amd.
+
+Arrays follow the reference's Fortran layouts, held as C-ordered numpy arrays
+with the axes reversed: a vector grid (nx,ny,nz,3) is np.empty((3,nz,ny,nx)),
+a particle array (Ni_max,3) is np.empty((3,Ni_max)), ijkp is int32 (3,Ni_max)
+with 1-based values, wght is (8,Ni_max).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_BUILD = os.path.join(_HERE, "_build")
+
+
+def build():
+    """Compile both oracle widths (gcc; a few seconds)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load(width):
+    path = os.path.join(_BUILD, "liboracle_f%d.so" % width)
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    real = C.c_double if width == 64 else C.c_float
+    vp = C.c_void_p
+    i32, i64, dbl = C.c_int, C.c_int64, C.c_double
+    sig = {
+        "or_create": ([i32, i32, i32, dbl, dbl, dbl, dbl, i32, dbl, dbl], vp),
+        "or_destroy": ([vp], None),
+        "or_real_size": ([], i32),
+        "or_get_coord": ([vp, i32, vp], None),
+        "or_set_dtsub": ([vp, dbl, dbl], None),
+        "or_get_dtsub": ([vp, vp, vp], None),
+        "or_initparameters": ([dbl, i32, dbl, dbl, dbl, dbl, i32, vp], None),
+        "or_grd6_b0": ([vp, dbl, dbl, vp], None),
+        "or_periodic": ([vp, vp], None),
+        "or_periodic_scalar": ([vp, vp], None),
+        "or_edge_to_center": ([vp, vp, vp], None),
+        "or_edge_to_face": ([vp, vp, vp], None),
+        "or_face_to_center": ([vp, vp, vp], None),
+        "or_crossf2": ([vp, vp, vp, vp], None),
+        "or_curlB": ([vp, vp, vp, vp], None),
+        "or_curlE": ([vp, vp, vp], None),
+        "or_get_E": ([vp, vp, vp, vp, vp, vp, vp], None),
+        "or_predict_B": ([vp] + [vp] * 9, None),
+        "or_get_Ep1": ([vp] + [vp] * 7, None),
+        "or_correct_B": ([vp] + [vp] * 7, None),
+        "or_f_update_tlev": ([vp, vp, vp, vp, vp], None),
+        "or_check_time_step": ([vp, vp, vp], None),
+        "or_get_interp_weights": ([vp, i32, i32, vp, vp, vp], None),
+        "or_update_np": ([vp, i32, i32, vp, vp, vp, vp], None),
+        "or_update_up": ([vp, i32, i32, vp, vp, vp, vp, vp, vp], None),
+        "or_extrapol_up": ([vp, i32, i32, vp, vp, vp, vp, vp, vp, vp], None),
+        "or_get_Ep": ([vp, i32, i32, vp, vp, vp, vp, vp, vp, vp], None),
+        "or_get_vplus_vminus": ([vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp], None),
+        "or_improve_up": ([vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp], None),
+        "or_get_vp_final": ([vp, i32, i32, vp, vp, vp, vp], None),
+        "or_move_ion_half": ([vp, i32, i32, vp, vp], None),
+        "or_step": ([vp, vp], None),
+        "or_philox4x32_10": ([vp, vp, vp], None),
+        "or_split_move": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+        "or_stable_sort": ([i64, vp, C.c_uint32, vp], None),
+        "or_load_maxwellian": ([i64, C.c_uint64, C.c_uint32, vp, dbl, dbl, vp, vp, vp], None),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib, real
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OrState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "b1", "b12", "b1p2", "bt", "btc", "np", "up", "aj", "E", "nu",
+        "xp", "vp", "vp1", "vplus", "vminus", "Ep", "wght", "beta_p", "mrat",
+        "ijkp")] + [("Ni_tot", C.c_int), ("Ni_max", C.c_int)]
+
+
+class Oracle:
+    """One oracle grid context.  `width` is 64 (FP64 judge) or 32."""
+
+    COORDS = ("qx", "qy", "qz", "dx_grid", "dy_grid", "dz_grid",
+              "dx_cell", "dy_cell", "dz_cell", "xrat", "yrat", "zrat")
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, dt, ntsub, alpha, beta, width=64):
+        self.lib, self._real = _load(width)
+        self.dtype = np.float64 if width == 64 else np.float32
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.h = self.lib.or_create(nx, ny, nz, dx, dy, dz, dt, ntsub, alpha, beta)
+        self.dt = dt
+
+    def __del__(self):
+        try:
+            self.lib.or_destroy(self.h)
+        except Exception:
+            pass
+
+    # -- allocation helpers --
+    def grid(self, ncomp=3):
+        shape = (self.nz, self.ny, self.nx) if ncomp == 1 else (ncomp, self.nz, self.ny, self.nx)
+        return np.zeros(shape, self.dtype)
+
+    def parts(self, n, ncomp=3):
+        return np.zeros((ncomp, n), self.dtype)
+
+    def coord(self, name):
+        n = {"x": self.nx, "y": self.ny, "z": self.nz}[name[1] if name[0] == "q" else name[1]]
+        out = np.zeros(n, self.dtype)
+        self.lib.or_get_coord(self.h, self.COORDS.index(name), _p(out))
+        return out
+
+    def _chk(self, *arrs):
+        for a in arrs:
+            if a is not None:
+                assert a.flags.c_contiguous, "oracle arrays must be C-contiguous"
+                assert a.dtype in (self.dtype, np.int32, np.uint32), a.dtype
+
+    def dtsub(self):
+        d, n = C.c_double(), C.c_double()
+        self.lib.or_get_dtsub(self.h, C.byref(d), C.byref(n))
+        return d.value, n.value
+
+    def set_dtsub(self, dtsub, ntf):
+        self.lib.or_set_dtsub(self.h, dtsub, ntf)
+
+    # -- reference routines --
+    def __getattr__(self, name):
+        # or_<name> with numpy arrays converted to pointers
+        if name.startswith("_") or name in ("lib", "h"):
+            raise AttributeError(name)
+        fn = getattr(self.lib, "or_" + name)
+
+        def call(*args):
+            conv = []
+            for a in args:
+                if isinstance(a, np.ndarray):
+                    self._chk(a)
+                    conv.append(a.ctypes.data)
+                else:
+                    conv.append(a)
+            return fn(self.h, *conv)
+        return call
+
+    def step(self, st):
+        """st: dict of numpy arrays (see OrState); advances one step in place."""
+        s = OrState()
+        for n, _ in OrState._fields_[:-2]:
+            setattr(s, n, _p(st.get(n)))
+        s.Ni_tot = st["Ni_tot"]
+        s.Ni_max = st["xp"].shape[1]
+        self.lib.or_step(self.h, C.byref(s))
+
+
+def initparameters(b0_init=0.2e-9, ion_amu=1, nf_init=0.1e15, dt_frac=0.05,
+                   dx_frac=0.5, nu_init_frac=0.002, ntsub=10, width=64):
+    lib, _ = _load(width)
+    out = np.zeros(8)
+    lib.or_initparameters(b0_init, ion_amu, nf_init, dt_frac, dx_frac, nu_init_frac,
+                          ntsub, out.ctypes.data)
+    keys = ("mion", "omega_p", "lambda_i", "dx", "dt", "dtsub_init", "nu_init", "alpha")
+    return dict(zip(keys, out.tolist()))
+
+
+def philox4x32_10(ctr, key):
+    lib, _ = _load(64)
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib.or_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
+
+
+def split_move(cell, fx, fy, fz, vx, vy, vz, s, nc):
+    lib, _ = _load(64)
+    s = np.asarray(s, np.float32)
+    nc = np.asarray(nc, np.int32)
+    lib.or_split_move(cell.size, cell.ctypes.data, fx.ctypes.data, fy.ctypes.data,
+                      fz.ctypes.data, vx.ctypes.data, vy.ctypes.data, vz.ctypes.data,
+                      s.ctypes.data, nc.ctypes.data)
+
+
+def stable_sort(key, nkeys):
+    lib, _ = _load(64)
+    key = np.ascontiguousarray(key, np.uint32)
+    perm = np.zeros(key.size, np.uint32)
+    lib.or_stable_sort(key.size, key.ctypes.data, nkeys, perm.ctypes.data)
+    return perm
+
+
+def load_maxwellian(n, seed, nc, vsw, vth, stream=1):
+    """(cell uint32[n], frac float64[3,n], v float64[3,n]) of the seeded load."""
+    lib, _ = _load(64)
+    nc = np.asarray(nc, np.int32)
+    cell = np.zeros(n, np.uint32)
+    frac = np.zeros((3, n))
+    v = np.zeros((3, n))
+    lib.or_load_maxwellian(n, seed, stream, nc.ctypes.data, vsw, vth, cell.ctypes.data,
+                           frac.ctypes.data, v.ctypes.data)
+    return cell, frac, v

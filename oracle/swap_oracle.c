@@ -1,0 +1,807 @@
+/*
+ * swap_oracle.c -- CPU restatement of SWAP's hybrid-PIC time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never the
+ * product.  Each routine cites the reference routine (file:line) whose
+ * algorithm it restates.  Loops are written with the reference's 1-based
+ * Fortran indices through the F3/F4 macros so every index can be checked
+ * against the Fortran text.
+ *
+ * Compiled twice: OR_REAL=double (FP64 judge) and OR_REAL=float (the
+ * reference's own "-real-size 32" width, makefile:1).  Every literal goes
+ * through R() so the float build really computes in float.
+ */
+#include "swap_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define R(x) ((real)(x))
+
+struct orc {
+  int nx, ny, nz;
+  real dx, dy, delz, dt, dtsub, ntf, alpha, beta;
+  int ntsub;
+  real *qx, *qy, *qz, *dx_grid, *dy_grid, *dz_grid;
+  real *dx_cell, *dy_cell, *dz_cell, *xrat, *yrat, *zrat;
+  /* module scratch of global.f:59 (a, c, ct) */
+  real *a, *c, *ct;
+};
+
+/* Fortran (i,j,k) / (i,j,k,m), 1-based, column-major */
+#define F3(i, j, k) ((size_t)((i)-1) + (size_t)h->nx * ((size_t)((j)-1) + (size_t)h->ny * (size_t)((k)-1)))
+#define NN ((size_t)h->nx * h->ny * h->nz)
+#define F4(i, j, k, m) (F3(i, j, k) + NN * (size_t)((m)-1))
+/* 1-D coordinate arrays, 1-based */
+#define QX(i) h->qx[(i)-1]
+#define QY(j) h->qy[(j)-1]
+#define QZ(k) h->qz[(k)-1]
+
+int or_real_size(void) { return (int)sizeof(real); }
+
+/* grd7, initial.f:147-301, with stretch factors xsf = zsf = 0 (initial.f:173,
+   209), i.e. a uniform grid built as an FP cumulative sum (initial.f:193-196,
+   233-237).  The reference's stretch loops also write dx_grid outside 1..nx
+   when nx < 41 (initial.f:175-191); with xsf = 0 every value they write is dx,
+   so the in-bounds result is the same as filling dx_grid with dx. */
+static void grd7(orc *h) {
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  for (int j = 1; j <= ny; j++) {
+    QY(j) = (real)j * h->dy; /* initial.f:159-162 */
+    h->dy_grid[j - 1] = h->dy;
+  }
+  for (int i = 1; i <= nx; i++) h->dx_grid[i - 1] = h->dx;
+  QX(1) = h->dx;
+  for (int i = 2; i <= nx; i++) QX(i) = QX(i - 1) + h->dx_grid[i - 1];
+  for (int i = 1; i <= nx - 1; i++) h->dx_grid[i - 1] = QX(i + 1) - QX(i);
+  h->dx_grid[nx - 1] = h->dx_grid[nx - 2];
+  for (int k = 1; k <= nz; k++) h->dz_grid[k - 1] = h->delz;
+  QZ(1) = h->delz;
+  for (int k = 2; k <= nz; k++) QZ(k) = QZ(k - 1) + h->dz_grid[k - 1];
+  for (int k = 1; k <= nz - 1; k++) h->dz_grid[k - 1] = QZ(k + 1) - QZ(k);
+  h->dz_grid[nz - 1] = h->dz_grid[nz - 2];
+  /* cell sizes and interpolation ratios, initial.f:240-282 */
+  h->dz_cell[0] = h->dz_grid[0];
+  h->dz_cell[nz - 1] = h->dz_grid[nz - 1];
+  h->zrat[0] = R(0.5);
+  h->zrat[nz - 1] = R(0.5);
+  for (int k = 2; k <= nz - 1; k++) {
+    real zp = (QZ(k + 1) + QZ(k)) / R(2.0), zm = (QZ(k) + QZ(k - 1)) / R(2.0);
+    h->dz_cell[k - 1] = zp - zm;
+    h->zrat[k - 1] = (QZ(k) - zm) / (zp - zm);
+  }
+  h->dx_cell[0] = h->dx_grid[0];
+  h->dx_cell[nx - 1] = h->dx_grid[nx - 1];
+  h->xrat[0] = R(0.5);
+  h->xrat[nx - 1] = R(0.5);
+  for (int i = 2; i <= nx - 1; i++) {
+    real xp = (QX(i + 1) + QX(i)) / R(2.0), xm = (QX(i) + QX(i - 1)) / R(2.0);
+    h->dx_cell[i - 1] = xp - xm;
+    h->xrat[i - 1] = (QX(i) - xm) / (xp - xm);
+  }
+  h->dy_cell[0] = h->dy_grid[0];
+  h->dy_cell[ny - 1] = h->dy_grid[ny - 1];
+  h->yrat[0] = R(0.5);
+  h->yrat[ny - 1] = R(0.5);
+  for (int j = 2; j <= ny - 1; j++) {
+    real yp = (QY(j + 1) + QY(j)) / R(2.0), ym = (QY(j) + QY(j - 1)) / R(2.0);
+    h->dy_cell[j - 1] = yp - ym;
+    h->yrat[j - 1] = (QY(j) - ym) / (yp - ym);
+  }
+}
+
+orc *or_create(int nx, int ny, int nz, double dx, double dy, double delz,
+               double dt, int ntsub, double alpha, double beta) {
+  orc *h = (orc *)calloc(1, sizeof(orc));
+  h->nx = nx; h->ny = ny; h->nz = nz;
+  h->dx = (real)dx; h->dy = (real)dy; h->delz = (real)delz;
+  h->dt = (real)dt; h->ntsub = ntsub;
+  h->dtsub = h->dt / (real)ntsub; /* inputs.f:145 */
+  h->ntf = (real)ntsub;
+  h->alpha = (real)alpha; h->beta = (real)beta;
+  real **c1[] = {&h->qx, &h->qy, &h->qz, &h->dx_grid, &h->dy_grid, &h->dz_grid,
+                 &h->dx_cell, &h->dy_cell, &h->dz_cell, &h->xrat, &h->yrat, &h->zrat};
+  int n1[] = {nx, ny, nz, nx, ny, nz, nx, ny, nz, nx, ny, nz};
+  for (int q = 0; q < 12; q++) *c1[q] = (real *)calloc((size_t)n1[q], sizeof(real));
+  size_t n4 = (size_t)nx * ny * nz * 3;
+  h->a = (real *)calloc(n4, sizeof(real));
+  h->c = (real *)calloc(n4, sizeof(real));
+  h->ct = (real *)calloc(n4, sizeof(real));
+  grd7(h);
+  return h;
+}
+
+void or_destroy(orc *h) {
+  if (!h) return;
+  real *p[] = {h->qx, h->qy, h->qz, h->dx_grid, h->dy_grid, h->dz_grid, h->dx_cell,
+               h->dy_cell, h->dz_cell, h->xrat, h->yrat, h->zrat, h->a, h->c, h->ct};
+  for (int q = 0; q < 15; q++) free(p[q]);
+  free(h);
+}
+
+void or_get_coord(const orc *h, int which, real *out) {
+  const real *src[] = {h->qx, h->qy, h->qz, h->dx_grid, h->dy_grid, h->dz_grid,
+                       h->dx_cell, h->dy_cell, h->dz_cell, h->xrat, h->yrat, h->zrat};
+  int n1[] = {h->nx, h->ny, h->nz, h->nx, h->ny, h->nz, h->nx, h->ny, h->nz, h->nx, h->ny, h->nz};
+  if (which < 0 || which > 11) return;
+  memcpy(out, src[which], sizeof(real) * (size_t)n1[which]);
+}
+
+void or_set_dtsub(orc *h, double dtsub, double ntf) { h->dtsub = (real)dtsub; h->ntf = (real)ntf; }
+void or_get_dtsub(const orc *h, double *dtsub, double *ntf) { *dtsub = h->dtsub; *ntf = h->ntf; }
+
+/* initparameters, inputs.f:125-172 (constants inputs.f:19-67) */
+void or_initparameters(double b0_init, int ion_amu, double nf_init, double dt_frac,
+                       double dx_frac, double nu_init_frac, int ntsub, double *out) {
+  const real q = R(1.6e-19), pi = R(3.14159);
+  const real mu0 = pi * R(4.0e-7);
+  real mion = (real)ion_amu * R(1.67e-27);
+  real omega_p = q * (real)b0_init / mion;
+  real lambda_i = (R(3e8) / (real)sqrt((double)(((real)nf_init / R(1e9)) * q * q /
+                                                (R(8.85e-12) * mion)))) / R(1e3);
+  real dx = lambda_i * (real)dx_frac;
+  real dt = (real)dt_frac * mion / (q * (real)b0_init);
+  real dtsub = dt / (real)ntsub;
+  real nu_init = (real)nu_init_frac * omega_p;
+  real alpha = (mu0 / R(1e3)) * q * (q / mion);
+  double o[8] = {mion, omega_p, lambda_i, dx, dt, dtsub, nu_init, alpha};
+  memcpy(out, o, sizeof(o));
+}
+
+/* grd6_setup, initial.f:36-63: uniform b0 = (0, 0, b0_init*q/mO) */
+void or_grd6_b0(const orc *h, double b0_init, double mion, real *b0) {
+  real eoverm = R(1.6e-19) / (real)mion;
+  real bz = (real)b0_init * eoverm;
+  for (size_t n = 0; n < NN; n++) {
+    b0[n] = 0; b0[n + NN] = 0; b0[n + 2 * NN] = bz;
+  }
+}
+
+/* periodic, boundary.f:9-52: x, then y, then z ghost copies */
+void or_periodic(const orc *h, real *b) {
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  for (int m = 1; m <= 3; m++) {
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++) {
+        b[F4(1, j, k, m)] = b[F4(nx - 1, j, k, m)];
+        b[F4(nx, j, k, m)] = b[F4(2, j, k, m)];
+      }
+  }
+  for (int m = 1; m <= 3; m++)
+    for (int k = 1; k <= nz; k++)
+      for (int i = 1; i <= nx; i++) {
+        b[F4(i, 1, k, m)] = b[F4(i, ny - 1, k, m)];
+        b[F4(i, ny, k, m)] = b[F4(i, 2, k, m)];
+      }
+  for (int m = 1; m <= 3; m++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        b[F4(i, j, 1, m)] = b[F4(i, j, nz - 1, m)];
+        b[F4(i, j, nz, m)] = b[F4(i, j, 2, m)];
+      }
+}
+
+/* periodic_scalar, boundary.f:167-206 */
+void or_periodic_scalar(const orc *h, real *b) {
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      b[F3(1, j, k)] = b[F3(nx - 1, j, k)];
+      b[F3(nx, j, k)] = b[F3(2, j, k)];
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++) {
+      b[F3(i, 1, k)] = b[F3(i, ny - 1, k)];
+      b[F3(i, ny, k)] = b[F3(i, 2, k)];
+    }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      b[F3(i, j, 1)] = b[F3(i, j, nz - 1)];
+      b[F3(i, j, nz)] = b[F3(i, j, 2)];
+    }
+}
+
+/* face-centred B component -> node, the bilinear stencil shared by
+   edge_to_center (grid_interp.f:22-55) and edge_to_face (:80-113) */
+static void e2c_point(const orc *h, const real *bt, int i, int j, int k, real *o3) {
+  int im = i - 1, jm = j - 1, km = k - 1;
+  real b2, b1;
+  b2 = bt[F4(i, jm, k, 1)] + h->yrat[j - 1] * (bt[F4(i, j, k, 1)] - bt[F4(i, jm, k, 1)]);
+  b1 = bt[F4(i, jm, km, 1)] + h->yrat[j - 1] * (bt[F4(i, j, km, 1)] - bt[F4(i, jm, km, 1)]);
+  o3[0] = b1 + h->zrat[k - 1] * (b2 - b1);
+  b2 = bt[F4(im, j, k, 2)] + h->xrat[i - 1] * (bt[F4(i, j, k, 2)] - bt[F4(im, j, k, 2)]);
+  b1 = bt[F4(im, j, km, 2)] + h->xrat[i - 1] * (bt[F4(i, j, km, 2)] - bt[F4(im, j, km, 2)]);
+  o3[1] = b1 + h->zrat[k - 1] * (b2 - b1);
+  b2 = bt[F4(i, jm, k, 3)] + h->yrat[j - 1] * (bt[F4(i, j, k, 3)] - bt[F4(i, jm, k, 3)]);
+  b1 = bt[F4(im, jm, k, 3)] + h->yrat[j - 1] * (bt[F4(im, j, k, 3)] - bt[F4(im, jm, k, 3)]);
+  o3[2] = b1 + h->xrat[i - 1] * (b2 - b1);
+}
+
+/* edge_to_center, grid_interp.f:10-62 */
+void or_edge_to_center(const orc *h, real *bt, real *btc) {
+  or_periodic(h, bt);
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++) {
+        real o[3];
+        e2c_point(h, bt, i, j, k, o);
+        for (int m = 1; m <= 3; m++) btc[F4(i, j, k, m)] = o[m - 1];
+      }
+  or_periodic(h, btc);
+}
+
+/* edge_to_face, grid_interp.f:67-130.  The reference loops i=2..nx and then
+   reads btc(i+1) at i=nx, one past the array (grid_interp.f:119-127); those
+   ghost-layer outputs are overwritten by the periodic() its only caller
+   (face_to_center, via get_Ep1 gutsf.f:1267-1269) applies first, so the
+   restatement computes the interior 2..n-1 and fills ghosts periodically. */
+void or_edge_to_face(const orc *h, real *bt, real *btmf) {
+  size_t n4 = NN * 3;
+  real *btc = (real *)calloc(n4, sizeof(real));
+  or_periodic(h, bt);
+  for (int k = 2; k <= h->nz; k++)
+    for (int j = 2; j <= h->ny; j++)
+      for (int i = 2; i <= h->nx; i++) {
+        real o[3];
+        e2c_point(h, bt, i, j, k, o);
+        for (int m = 1; m <= 3; m++) btc[F4(i, j, k, m)] = o[m - 1];
+      }
+  or_periodic(h, btc);
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++) {
+        btmf[F4(i, j, k, 1)] = R(0.5) * (btc[F4(i, j, k, 1)] + btc[F4(i + 1, j, k, 1)]);
+        btmf[F4(i, j, k, 2)] = R(0.5) * (btc[F4(i, j, k, 2)] + btc[F4(i, j + 1, k, 2)]);
+        btmf[F4(i, j, k, 3)] = R(0.5) * (btc[F4(i, j, k, 3)] + btc[F4(i, j, k + 1, 3)]);
+      }
+  or_periodic(h, btmf);
+  free(btc);
+}
+
+/* face_to_center, grid_interp.f:136-176 */
+void or_face_to_center(const orc *h, real *v, real *vc) {
+  or_periodic(h, v);
+  for (int k = 2; k <= h->nz; k++)
+    for (int j = 2; j <= h->ny; j++)
+      for (int i = 2; i <= h->nx; i++) {
+        vc[F4(i, j, k, 1)] = h->xrat[i - 1] * (v[F4(i, j, k, 1)] - v[F4(i - 1, j, k, 1)]) + v[F4(i - 1, j, k, 1)];
+        vc[F4(i, j, k, 2)] = h->yrat[j - 1] * (v[F4(i, j, k, 2)] - v[F4(i, j - 1, k, 2)]) + v[F4(i, j - 1, k, 2)];
+        vc[F4(i, j, k, 3)] = h->zrat[k - 1] * (v[F4(i, j, k, 3)] - v[F4(i, j, k - 1, 3)]) + v[F4(i, j, k - 1, 3)];
+      }
+  or_periodic(h, vc);
+}
+
+/* crossf2, gutsf.f:40-104 (node cross product into module scratch ct, then
+   the edge average) */
+void or_crossf2(orc *h, real *aa, real *btc, real *cc) {
+  real *ct = h->ct;
+  or_periodic(h, aa);
+  or_periodic(h, btc);
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++) {
+        real ax = aa[F4(i, j, k, 1)], bx = btc[F4(i, j, k, 1)];
+        real ay = aa[F4(i, j, k, 2)], by = btc[F4(i, j, k, 2)];
+        real az = aa[F4(i, j, k, 3)], bz = btc[F4(i, j, k, 3)];
+        ct[F4(i, j, k, 1)] = ay * bz - az * by;
+        ct[F4(i, j, k, 2)] = az * bx - ax * bz;
+        ct[F4(i, j, k, 3)] = ax * by - ay * bx;
+      }
+  or_periodic(h, ct);
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++) {
+        cc[F4(i, j, k, 1)] = R(0.5) * (ct[F4(i, j, k, 1)] + ct[F4(i + 1, j, k, 1)]);
+        cc[F4(i, j, k, 2)] = R(0.5) * (ct[F4(i, j, k, 2)] + ct[F4(i, j + 1, k, 2)]);
+        cc[F4(i, j, k, 3)] = R(0.5) * (ct[F4(i, j, k, 3)] + ct[F4(i, j, k + 1, 3)]);
+      }
+  or_periodic(h, cc);
+}
+
+/* curlB, gutsf.f:178-244.  periodic(b1) mutates the input (gutsf.f:199);
+   aj ghosts are not written (gutsf.f:241). */
+void or_curlB(const orc *h, real *b1, const real *np, real *aj) {
+  or_periodic(h, b1);
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++) {
+        real nt1 = R(0.5) * (np[F3(i, j, k)] + np[F3(i + 1, j, k)]);
+        real nt2 = R(0.5) * (np[F3(i, j, k)] + np[F3(i, j + 1, k)]);
+        real nt3 = R(0.5) * (np[F3(i, j, k)] + np[F3(i, j, k + 1)]);
+        real dxc = h->dx_cell[i - 1], dyc = h->dy_cell[j - 1], dzc = h->dz_cell[k - 1];
+        real c1 = (b1[F4(i, j, k, 3)] - b1[F4(i, j - 1, k, 3)]) / dyc +
+                  (b1[F4(i, j, k - 1, 2)] - b1[F4(i, j, k, 2)]) / dzc;
+        real c2 = (b1[F4(i, j, k, 1)] - b1[F4(i, j, k - 1, 1)]) / dzc +
+                  (b1[F4(i - 1, j, k, 3)] - b1[F4(i, j, k, 3)]) / dxc;
+        real c3 = (b1[F4(i, j, k, 2)] - b1[F4(i - 1, j, k, 2)]) / dxc +
+                  (b1[F4(i, j - 1, k, 1)] - b1[F4(i, j, k, 1)]) / dyc;
+        aj[F4(i, j, k, 1)] = c1 / (nt1 * h->alpha);
+        aj[F4(i, j, k, 2)] = c2 / (nt2 * h->alpha);
+        aj[F4(i, j, k, 3)] = c3 / (nt3 * h->alpha);
+      }
+}
+
+/* curlE, gutsf.f:249-308 (interior only; callers never read its ghosts) */
+void or_curlE(const orc *h, const real *E, real *cE) {
+  for (int i = 2; i <= h->nx - 1; i++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int k = 2; k <= h->nz - 1; k++) {
+        real dxg = h->dx_grid[i - 1], dyg = h->dy_grid[j - 1], dzg = h->dz_grid[k - 1];
+        cE[F4(i, j, k, 1)] = (E[F4(i, j + 1, k, 3)] - E[F4(i, j, k, 3)]) / dyg +
+                             (E[F4(i, j, k, 2)] - E[F4(i, j, k + 1, 2)]) / dzg;
+        cE[F4(i, j, k, 2)] = (E[F4(i, j, k, 3)] - E[F4(i + 1, j, k, 3)]) / dxg +
+                             (E[F4(i, j, k + 1, 1)] - E[F4(i, j, k, 1)]) / dzg;
+        cE[F4(i, j, k, 3)] = (E[F4(i, j, k, 1)] - E[F4(i, j + 1, k, 1)]) / dyg +
+                             (E[F4(i + 1, j, k, 2)] - E[F4(i, j, k, 2)]) / dxg;
+      }
+}
+
+/* E = c + nu*aj on the interior, then periodic (gutsf.f:1078-1089,
+   :1278-1288) */
+static void e_from_c(orc *h, real *E, const real *aj, const real *nu) {
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++)
+        for (int m = 1; m <= 3; m++)
+          E[F4(i, j, k, m)] = h->c[F4(i, j, k, m)] + nu[F3(i, j, k)] * aj[F4(i, j, k, m)];
+  or_periodic(h, E);
+}
+
+/* a = aj - up on the interior, into module scratch a (gutsf.f:1058-1064) */
+static void a_from_aj_up(orc *h, const real *aj, const real *up) {
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++)
+        for (int m = 1; m <= 3; m++)
+          h->a[F4(i, j, k, m)] = aj[F4(i, j, k, m)] - up[F4(i, j, k, m)];
+}
+
+/* get_E, gutsf.f:994-1110 */
+void or_get_E(orc *h, real *E, real *bt, real *aj, real *up, const real *np,
+              const real *nu) {
+  (void)np;
+  size_t n4 = NN * 3;
+  real *aa = (real *)calloc(n4, sizeof(real));
+  real *btc = (real *)calloc(n4, sizeof(real));
+  a_from_aj_up(h, aj, up);
+  or_face_to_center(h, h->a, aa);
+  or_edge_to_center(h, bt, btc);
+  or_crossf2(h, aa, btc, h->c);
+  e_from_c(h, E, aj, nu);
+  free(aa);
+  free(btc);
+}
+
+/* predict_B, gutsf.f:1115-1169: b1p2 = b12 - 2 dtsub curlE */
+void or_predict_B(orc *h, real *b1, real *b12, real *b1p2, real *bt, real *E,
+                  real *aj, real *up, const real *np, const real *nu) {
+  (void)b1;
+  size_t n4 = NN * 3;
+  real *cE = (real *)calloc(n4, sizeof(real));
+  or_get_E(h, E, bt, aj, up, np, nu);
+  or_curlE(h, E, cE);
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++)
+        for (int m = 1; m <= 3; m++)
+          b1p2[F4(i, j, k, m)] = b12[F4(i, j, k, m)] - R(2.0) * h->dtsub * cE[F4(i, j, k, m)];
+  or_periodic(h, b1p2);
+  free(cE);
+}
+
+/* get_Ep1, gutsf.f:1174-1304: E at m+1/2 from btp1 = (b1p2+b1)/2 with the
+   smoothed node field edge_to_face -> face_to_center */
+void or_get_Ep1(orc *h, real *E, real *b1, real *b1p2, real *aj, real *up,
+                const real *np, const real *nu) {
+  size_t n4 = NN * 3;
+  real *btp1 = (real *)calloc(n4, sizeof(real));
+  real *btp1mf = (real *)calloc(n4, sizeof(real));
+  real *btc = (real *)calloc(n4, sizeof(real));
+  real *aa = (real *)calloc(n4, sizeof(real));
+  for (size_t n = 0; n < n4; n++) btp1[n] = R(0.5) * (b1p2[n] + b1[n]);
+  or_curlB(h, btp1, np, aj);
+  a_from_aj_up(h, aj, up);
+  or_face_to_center(h, h->a, aa);
+  or_edge_to_face(h, btp1, btp1mf);
+  or_face_to_center(h, btp1mf, btc);
+  or_crossf2(h, aa, btc, h->c);
+  e_from_c(h, E, aj, nu);
+  free(btp1); free(btp1mf); free(btc); free(aa);
+}
+
+/* correct_B, gutsf.f:1309-1359: b1p2 = b1 - dtsub curlE */
+void or_correct_B(orc *h, real *b1, real *b1p2, real *E, real *aj, real *up,
+                  const real *np, const real *nu) {
+  size_t n4 = NN * 3;
+  real *cE = (real *)calloc(n4, sizeof(real));
+  or_get_Ep1(h, E, b1, b1p2, aj, up, np, nu);
+  or_curlE(h, E, cE);
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++)
+        for (int m = 1; m <= 3; m++)
+          b1p2[F4(i, j, k, m)] = b1[F4(i, j, k, m)] - h->dtsub * cE[F4(i, j, k, m)];
+  or_periodic(h, b1p2);
+  free(cE);
+}
+
+/* f_update_tlev, gutsf.f:10-35 (the B0 add is commented out there) */
+void or_f_update_tlev(const orc *h, real *b1, real *b12, const real *b1p2, real *bt) {
+  size_t n4 = NN * 3;
+  for (size_t n = 0; n < n4; n++) {
+    bt[n] = b1p2[n];
+    b12[n] = b1[n];
+    b1[n] = b1p2[n];
+  }
+}
+
+/* check_time_step, gutsf.f:1776-1822.  Node order is the reference's loop
+   nest (i outer, j, k inner); every failing node halves dtsub once against
+   the dtsub current at that node. */
+void or_check_time_step(orc *h, const real *bt, const real *np) {
+  for (int i = 1; i <= h->nx; i++)
+    for (int j = 1; j <= h->ny; j++)
+      for (int k = 1; k <= h->nz; k++) {
+        real ak = R(2.) / h->dx;
+        real b1 = bt[F4(i, j, k, 1)], b2 = bt[F4(i, j, k, 2)], b3 = bt[F4(i, j, k, 3)];
+        real btot = (real)sqrt((double)(b1 * b1 + b2 * b2 + b3 * b3));
+        real a1 = ak * ak * btot / (h->alpha * np[F3(i, j, k)]);
+        real a2 = (ak * btot) * (ak * btot) / (h->alpha * np[F3(i, j, k)]);
+        real womega = R(0.5) * (a1 + (real)sqrt((double)(a1 * a1 + R(4) * a2)));
+        real phi = womega / ak;
+        real deltat = h->dx / phi;
+        if (deltat <= R(2.0) * h->dtsub) {
+          h->dtsub = h->dtsub / R(2.0);
+          h->ntf = h->ntf * R(2.0);
+        }
+      }
+}
+
+/* ---------------------------------------------------------------- */
+/* particles (gutsp.f)                                               */
+/* ---------------------------------------------------------------- */
+#define P(arr, l, m) arr[(size_t)(Ni_max) * ((m)-1) + (size_t)((l)-1)]
+
+/* smallest idx in 1..n with x <= q(idx), minus 1: the result of the
+   reference's upward linear search (gutsp.f:1394-1401, 1407-1412), found by
+   bisection (same answer for the monotone q arrays grd7 builds) */
+static int search_cell(const real *q, int n, real x) {
+  int lo = 1, hi = n; /* answer idx in [1, n] */
+  if (x <= q[0]) return 0;
+  while (lo < hi) {
+    int mid = (lo + hi) / 2;
+    if (x <= q[mid - 1]) hi = mid; else lo = mid + 1;
+  }
+  return lo - 1;
+}
+
+/* get_interp_weights, gutsp.f:1378-1435 */
+void or_get_interp_weights(const orc *h, int Ni_tot, int Ni_max, const real *xp,
+                           int32_t *ijkp, real *wght) {
+  for (int l = 1; l <= Ni_tot; l++) {
+    real x = P(xp, l, 1), y = P(xp, l, 2), z = P(xp, l, 3);
+    int i = search_cell(h->qx, h->nx, x);
+    int j = (int)floor((double)(y / h->dy));
+    int k = search_cell(h->qz, h->nz, z);
+    P(ijkp, l, 1) = i; P(ijkp, l, 2) = j; P(ijkp, l, 3) = k;
+    real vol = R(1.0) / ((QX(i + 1) - QX(i)) * (QY(j + 1) - QY(j)) * (QZ(k + 1) - QZ(k)));
+    real x1 = (real)fabs((double)(x - QX(i))), x2 = (real)fabs((double)(x - QX(i + 1)));
+    real y1 = (real)fabs((double)(y - QY(j))), y2 = (real)fabs((double)(y - QY(j + 1)));
+    real z1 = (real)fabs((double)(z - QZ(k))), z2 = (real)fabs((double)(z - QZ(k + 1)));
+    P(wght, l, 1) = x2 * y2 * z2 * vol;
+    P(wght, l, 2) = x1 * y2 * z2 * vol;
+    P(wght, l, 3) = x2 * y2 * z1 * vol;
+    P(wght, l, 4) = x1 * y2 * z1 * vol;
+    P(wght, l, 5) = x2 * y1 * z2 * vol;
+    P(wght, l, 6) = x1 * y1 * z2 * vol;
+    P(wght, l, 7) = x2 * y1 * z1 * vol;
+    P(wght, l, 8) = x1 * y1 * z1 * vol;
+  }
+}
+
+/* corner c (1..8) of the reference's weight order (gutsp.f:1422-1429) */
+static inline void corner(int c, int i, int j, int k, int *ci, int *cj, int *ck) {
+  static const int di[8] = {0, 1, 0, 1, 0, 1, 0, 1};
+  static const int dj[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  static const int dk[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+  *ci = i + di[c - 1]; *cj = j + dj[c - 1]; *ck = k + dk[c - 1];
+}
+
+/* fold ghost layer 1 into n-1 per axis (gutsp.f:1496-1498, 1803-1805) */
+static void fold(const orc *h, real *g, int ncomp) {
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  for (int m = 1; m <= ncomp; m++) {
+    real *s = g + NN * (size_t)(m - 1);
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++) s[F3(nx - 1, j, k)] += s[F3(1, j, k)];
+    for (int k = 1; k <= nz; k++)
+      for (int i = 1; i <= nx; i++) s[F3(i, ny - 1, k)] += s[F3(i, 1, k)];
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) s[F3(i, j, nz - 1)] += s[F3(i, j, 1)];
+  }
+}
+
+/* update_np, gutsp.f:1441-1512 (one rank: the allreduce is the identity) */
+void or_update_np(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
+                  const real *wght, const real *beta_p, real *np) {
+  memset(np, 0, sizeof(real) * NN);
+  for (int l = 1; l <= Ni_tot; l++) {
+    int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
+    real bp = beta_p ? beta_p[l - 1] : R(1.0);
+    real volb = R(1.0) / (h->dx_grid[i - 1] * h->dy_grid[j - 1] * h->dz_grid[k - 1] * h->beta * bp);
+    for (int c = 1; c <= 8; c++) {
+      int ci, cj, ck;
+      corner(c, i, j, k, &ci, &cj, &ck);
+      np[F3(ci, cj, ck)] += P(wght, l, c) * volb;
+    }
+  }
+  fold(h, np, 1);
+  or_periodic_scalar(h, np);
+}
+
+/* update_up, gutsp.f:1649-1842 */
+void or_update_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
+                  const real *wght, const real *beta_p, const real *vp, real *np,
+                  real *up) {
+  real *ct = h->ct;
+  (void)np;
+  memset(up, 0, sizeof(real) * NN * 3);
+  memset(ct, 0, sizeof(real) * NN * 3);
+  for (int l = 1; l <= Ni_tot; l++) {
+    int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
+    real bp = beta_p ? beta_p[l - 1] : R(1.0);
+    real volb = (QX(i + 1) - QX(i)) * (QY(j + 1) - QY(j)) * (QZ(k + 1) - QZ(k)) * h->beta * bp;
+    for (int c = 1; c <= 8; c++) {
+      int ci, cj, ck;
+      corner(c, i, j, k, &ci, &cj, &ck);
+      real nvolb = R(1.0) / (np[F3(ci, cj, ck)] * volb);
+      for (int m = 1; m <= 3; m++)
+        ct[F4(ci, cj, ck, m)] += P(vp, l, m) * P(wght, l, c) * nvolb;
+    }
+  }
+  fold(h, ct, 3);
+  or_periodic(h, ct);
+  for (int k = 1; k <= h->nz - 1; k++)
+    for (int j = 1; j <= h->ny - 1; j++)
+      for (int i = 1; i <= h->nx - 1; i++) {
+        up[F4(i, j, k, 1)] = R(0.5) * (ct[F4(i, j, k, 1)] + ct[F4(i + 1, j, k, 1)]);
+        up[F4(i, j, k, 2)] = R(0.5) * (ct[F4(i, j, k, 2)] + ct[F4(i, j + 1, k, 2)]);
+        up[F4(i, j, k, 3)] = R(0.5) * (ct[F4(i, j, k, 3)] + ct[F4(i, j, k + 1, 3)]);
+      }
+  or_periodic(h, up);
+}
+
+/* extrapol_up, gutsp.f:127-150: deposit v = 1.5 vp - 0.5 vp1 */
+void or_extrapol_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
+                    const real *wght, const real *beta_p, const real *vp,
+                    const real *vp1, real *np, real *up) {
+  real *v = (real *)malloc(sizeof(real) * (size_t)Ni_max * 3);
+  for (int m = 1; m <= 3; m++)
+    for (int l = 1; l <= Ni_tot; l++) P(v, l, m) = R(1.5) * P(vp, l, m) - R(0.5) * P(vp1, l, m);
+  or_update_up(h, Ni_tot, Ni_max, ijkp, wght, beta_p, v, np, up);
+  free(v);
+}
+
+/* trilinear gather of a vector grid at particle l (gutsp.f:217-251) */
+static inline real gather(const orc *h, const real *g, int m, int i, int j, int k,
+                          const real *w8) {
+  int ip = i + 1, jp = j + 1, kp = k + 1;
+  return g[F4(i, j, k, m)] * w8[0] + g[F4(ip, j, k, m)] * w8[1] + g[F4(i, j, kp, m)] * w8[2] +
+         g[F4(ip, j, kp, m)] * w8[3] + g[F4(i, jp, k, m)] * w8[4] + g[F4(ip, jp, k, m)] * w8[5] +
+         g[F4(i, jp, kp, m)] * w8[6] + g[F4(ip, jp, kp, m)] * w8[7];
+}
+
+/* get_Ep, gutsp.f:155-276 */
+void or_get_Ep(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp, const real *wght,
+               const real *mrat, real *aj, real *up, const real *btc, real *Ep) {
+  size_t n4 = NN * 3;
+  real *ajc = (real *)calloc(n4, sizeof(real));
+  real *upc = (real *)calloc(n4, sizeof(real));
+  or_face_to_center(h, aj, ajc);
+  or_face_to_center(h, up, upc);
+  for (int l = 1; l <= Ni_tot; l++) {
+    int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
+    real w8[8];
+    for (int c = 0; c < 8; c++) w8[c] = P(wght, l, c + 1);
+    real aa[3], bb[3];
+    for (int m = 1; m <= 3; m++) {
+      real aj3 = gather(h, ajc, m, i, j, k, w8);
+      real up3 = gather(h, upc, m, i, j, k, w8);
+      aa[m - 1] = aj3 - up3;
+      bb[m - 1] = gather(h, btc, m, i, j, k, w8);
+    }
+    real mr = mrat ? mrat[l - 1] : R(1.0);
+    P(Ep, l, 1) = (aa[1] * bb[2] - aa[2] * bb[1]) * mr;
+    P(Ep, l, 2) = (aa[2] * bb[0] - aa[0] * bb[2]) * mr;
+    P(Ep, l, 3) = (aa[0] * bb[1] - aa[1] * bb[0]) * mr;
+  }
+  free(ajc);
+  free(upc);
+}
+
+/* get_vplus_vminus, gutsp.f:281-402 (the Crank-Nicolson rotation) */
+void or_get_vplus_vminus(const orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
+                         const real *wght, const real *mrat, const real *Ep,
+                         const real *btc, const real *vp, real *vplus, real *vminus) {
+  real dt = h->dt;
+  for (int m = 1; m <= 3; m++)
+    for (int l = 1; l <= Ni_tot; l++) P(vminus, l, m) = P(vp, l, m) + R(0.5) * dt * P(Ep, l, m);
+  for (int l = 1; l <= Ni_tot; l++) {
+    int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
+    real w8[8];
+    for (int c = 0; c < 8; c++) w8[c] = P(wght, l, c + 1);
+    real b3[3];
+    for (int m = 1; m <= 3; m++) b3[m - 1] = gather(h, btc, m, i, j, k, w8);
+    real mr = mrat ? mrat[l - 1] : R(1.0);
+    real vm1 = P(vminus, l, 1), vm2 = P(vminus, l, 2), vm3 = P(vminus, l, 3);
+    real vxb[3];
+    vxb[0] = vm2 * b3[2] * mr - vm3 * b3[1] * mr;
+    vxb[1] = vm3 * b3[0] * mr - vm1 * b3[2] * mr;
+    vxb[2] = vm1 * b3[1] * mr - vm2 * b3[0] * mr;
+    real vdb = vm1 * b3[0] * mr + vm2 * b3[1] * mr + vm3 * b3[2] * mr;
+    real Bx = b3[0] * mr, By = b3[1] * mr, Bz = b3[2] * mr;
+    real B2 = Bx * Bx + By * By + Bz * Bz;
+    real dt2 = dt * dt;
+    real a_d = R(1.0) / (R(1.0) + (R(0.25) * B2 * dt2));
+    real a1 = (R(1) - (R(0.25) * B2 * dt2)) * a_d;
+    real a2 = dt * a_d;
+    real a3 = R(0.5) * dt2 * a_d;
+    for (int m = 1; m <= 3; m++)
+      P(vplus, l, m) = a1 * P(vminus, l, m) + a2 * vxb[m - 1] + a3 * vdb * b3[m - 1] * mr;
+  }
+}
+
+/* improve_up, gutsp.f:407-428 */
+void or_improve_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp, const real *wght,
+                   const real *beta_p, real *vp1, const real *vplus, const real *vminus,
+                   real *np, real *up) {
+  for (int m = 1; m <= 3; m++)
+    for (int l = 1; l <= Ni_tot; l++)
+      P(vp1, l, m) = R(0.5) * (P(vplus, l, m) + P(vminus, l, m));
+  or_update_up(h, Ni_tot, Ni_max, ijkp, wght, beta_p, vp1, np, up);
+}
+
+/* get_vp_final, gutsp.f:433-449 */
+void or_get_vp_final(const orc *h, int Ni_tot, int Ni_max, const real *Ep, real *vp,
+                     real *vp1, const real *vplus) {
+  for (int m = 1; m <= 3; m++)
+    for (int l = 1; l <= Ni_tot; l++) {
+      P(vp1, l, m) = P(vp, l, m);
+      P(vp, l, m) = P(vplus, l, m) + R(0.5) * h->dt * P(Ep, l, m);
+    }
+}
+
+/* move_ion_half, gutsp.f:454-570: x += dt/2 v, one periodic wrap per axis
+   into (q(1), q(n-1)] */
+void or_move_ion_half(const orc *h, int Ni_tot, int Ni_max, real *xp, const real *vp) {
+  real dth = h->dt / R(2.);
+  const real *q[3] = {h->qx, h->qy, h->qz};
+  int n[3] = {h->nx, h->ny, h->nz};
+  for (int l = 1; l <= Ni_tot; l++)
+    for (int m = 1; m <= 3; m++) {
+      real x = P(xp, l, m) + dth * P(vp, l, m);
+      real q1 = q[m - 1][0], qn = q[m - 1][n[m - 1] - 2];
+      if (x > qn) x = q1 + (x - qn);
+      else if (x <= q1) x = qn - (q1 - x);
+      P(xp, l, m) = x;
+    }
+}
+
+/* one step in the order of maind.f:385-479 (ionization and the energy
+   diagnostic of maind.f:379-393 left out) */
+void or_step(orc *h, or_state *s) {
+  int N = s->Ni_tot, M = s->Ni_max;
+  or_get_interp_weights(h, N, M, s->xp, s->ijkp, s->wght);
+  or_update_np(h, N, M, s->ijkp, s->wght, s->beta_p, s->np);
+  or_update_up(h, N, M, s->ijkp, s->wght, s->beta_p, s->vp, s->np, s->up);
+  or_curlB(h, s->bt, s->np, s->aj);
+  or_edge_to_center(h, s->bt, s->btc);
+  or_extrapol_up(h, N, M, s->ijkp, s->wght, s->beta_p, s->vp, s->vp1, s->np, s->up);
+  or_get_Ep(h, N, M, s->ijkp, s->wght, s->mrat, s->aj, s->up, s->btc, s->Ep);
+  or_get_vplus_vminus(h, N, M, s->ijkp, s->wght, s->mrat, s->Ep, s->btc, s->vp, s->vplus, s->vminus);
+  or_improve_up(h, N, M, s->ijkp, s->wght, s->beta_p, s->vp1, s->vplus, s->vminus, s->np, s->up);
+  or_get_Ep(h, N, M, s->ijkp, s->wght, s->mrat, s->aj, s->up, s->btc, s->Ep);
+  or_get_vplus_vminus(h, N, M, s->ijkp, s->wght, s->mrat, s->Ep, s->btc, s->vp, s->vplus, s->vminus);
+  or_get_vp_final(h, N, M, s->Ep, s->vp, s->vp1, s->vplus);
+  or_move_ion_half(h, N, M, s->xp, s->vp);
+  or_get_interp_weights(h, N, M, s->xp, s->ijkp, s->wght);
+  or_update_np(h, N, M, s->ijkp, s->wght, s->beta_p, s->np);
+  or_update_up(h, N, M, s->ijkp, s->wght, s->beta_p, s->vp, s->np, s->up);
+  h->dtsub = h->dt / (real)h->ntsub; /* maind.f:420-421 */
+  h->ntf = (real)h->ntsub;
+  or_check_time_step(h, s->bt, s->np);
+  for (int n = 1; n <= (int)h->ntf; n++) {
+    or_curlB(h, s->bt, s->np, s->aj);
+    or_predict_B(h, s->b1, s->b12, s->b1p2, s->bt, s->E, s->aj, s->up, s->np, s->nu);
+    or_correct_B(h, s->b1, s->b1p2, s->E, s->aj, s->up, s->np, s->nu);
+    or_f_update_tlev(h, s->b1, s->b12, s->b1p2, s->bt);
+  }
+  or_move_ion_half(h, N, M, s->xp, s->vp);
+}
+
+/* ---------------------------------------------------------------- */
+/* build-only pieces                                                 */
+/* ---------------------------------------------------------------- */
+
+/* Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+   easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with the
+   published multipliers and Weyl key increments. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* split-precision half move; the device kernel's arithmetic, written with
+   explicit fmaf so both sides round identically */
+void or_split_move(int64_t n, uint32_t *cell, float *fx, float *fy, float *fz,
+                   const float *vx, const float *vy, const float *vz, const float s[3],
+                   const int nc[3]) {
+  for (int64_t p = 0; p < n; p++) {
+    uint32_t c = cell[p];
+    int ijk[3] = {(int)(c & 2047u), (int)((c >> 11) & 1023u), (int)(c >> 21)};
+    float *f[3] = {&fx[p], &fy[p], &fz[p]};
+    float v[3] = {vx[p], vy[p], vz[p]};
+    for (int a = 0; a < 3; a++) {
+      float t = fmaf(v[a], s[a], *f[a]);
+      float sh = ceilf(t) - 1.0f; /* right-closed cells: f in (0,1] */
+      *f[a] = t - sh;
+      int ci = ijk[a] + (int)sh;
+      ci %= nc[a];
+      if (ci < 0) ci += nc[a];
+      ijk[a] = ci;
+    }
+    cell[p] = (uint32_t)ijk[0] | ((uint32_t)ijk[1] << 11) | ((uint32_t)ijk[2] << 21);
+  }
+}
+
+void or_stable_sort(int64_t n, const uint32_t *key, uint32_t nkeys, uint32_t *perm) {
+  uint32_t *cnt = (uint32_t *)calloc((size_t)nkeys + 1, sizeof(uint32_t));
+  for (int64_t p = 0; p < n; p++) cnt[key[p] + 1]++;
+  for (uint32_t k = 0; k < nkeys; k++) cnt[k + 1] += cnt[k];
+  for (int64_t p = 0; p < n; p++) perm[cnt[key[p]]++] = (uint32_t)p;
+  free(cnt);
+}
+
+/* the drifting-Maxwellian sampler of load_Maxwellian (part_init.f:176-206)
+   as the device loader draws it: Philox counter (id lo, id hi, stream, draw
+   0..2) keyed by the 64-bit seed; position t = (1-u) n_cells in cell units
+   (exact in double), right-closed split t = i + f with f in (0,1]; velocity
+   vsw + vth sqrt(-ln u1) cos(PI u2) with PI = 3.14159 (inputs.f:52). */
+static double or_u24(uint32_t r) { return (double)(r >> 8) * (1.0 / 16777216.0); }
+void or_load_maxwellian(int64_t n, uint64_t seed, uint32_t stream, const int nc[3], double vsw,
+                        double vth, uint32_t *cell, double *frac, double *v) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t p = 0; p < n; p++) {
+    uint32_t r[3][4];
+    for (int d = 0; d < 3; d++) {
+      uint32_t ctr[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, (uint32_t)d};
+      or_philox4x32_10(ctr, key, r[d]);
+    }
+    int ijk[3];
+    for (int a = 0; a < 3; a++) {
+      double t = (1.0 - or_u24(r[0][a])) * (double)nc[a];
+      double i = ceil(t) - 1.0;
+      ijk[a] = (int)i;
+      frac[a * n + p] = t - i;
+    }
+    cell[p] = (uint32_t)ijk[0] | ((uint32_t)ijk[1] << 11) | ((uint32_t)ijk[2] << 21);
+    const uint32_t *ur[6] = {&r[0][3], &r[1][0], &r[1][1], &r[1][2], &r[1][3], &r[2][0]};
+    for (int a = 0; a < 3; a++) {
+      double u1 = 1.0 - or_u24(*ur[2 * a]);
+      double u2 = or_u24(*ur[2 * a + 1]);
+      v[a * n + p] = (a == 0 ? vsw : 0.0) + vth * sqrt(-log(u1)) * cos(3.14159 * u2);
+    }
+  }
+}

@@ -1,0 +1,525 @@
+// k_fields.hip -- staggered-grid field kernels of the hybrid step.
+//
+// Placement (the reference's Yee layout): n, ct and node averages at nodes;
+// aj, up, E component m on the m-directed edge; B component m on the
+// m-normal face.  Every kernel evaluates its stencil at the canonical node of
+// each storage node, so ghost layers come out equal to what boundary.f's
+// periodic() would copy (boundary.f:9-52) and no separate ghost pass runs.
+// B is held and advanced in FP64; everything else in the context precision R.
+#include "swap_internal.h"
+
+namespace {
+
+struct FC {  // uniform-grid constants
+  double dx, dy, dz;
+  double rx, ry, rz;  // xrat, yrat, zrat: 0.5 on a uniform grid (initial.f:276)
+  double alpha;
+};
+
+FC fc_of(swap_ctx* c) {
+  FC f;
+  f.dx = c->cfg.dx; f.dy = c->cfg.dy; f.dz = c->cfg.dz;
+  f.rx = f.ry = f.rz = 0.5;
+  f.alpha = c->cfg.alpha;
+  return f;
+}
+
+#define GRID_LOOP_HEAD                                                     \
+  int i = blockIdx.x * blockDim.x + threadIdx.x;                           \
+  int j = blockIdx.y * blockDim.y + threadIdx.y;                           \
+  int k = blockIdx.z;                                                      \
+  if (i >= g.nx || j >= g.ny) return;                                      \
+  int ci = canon(i, g.nx), cj = canon(j, g.ny), ck = canon(k, g.nz);       \
+  const int64_t o = gidx(g, i, j, k);                                      \
+  (void)o;                                                                 \
+  (void)ci; (void)cj; (void)ck;
+
+#define AT(arr, m, ii, jj, kk) (arr)[gidx(g, ii, jj, kk) + (int64_t)(m)*g.nn]
+
+dim3 gblock() { return dim3(64, 4, 1); }
+dim3 ggrid(const Geo& g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
+
+// B component m at face (i,j,k); with b2 given, the half-level average
+// btp1 = (b1p2 + b1)/2 of get_Ep1 (gutsf.f:1210-1219)
+__device__ __forceinline__ double bval(const Geo& g, const double* b, const double* b2, int m,
+                                       int i, int j, int k) {
+  int64_t q = gidx(g, i, j, k) + (int64_t)m * g.nn;
+  return b2 ? 0.5 * (b[q] + b2[q]) : b[q];
+}
+
+// edge_to_center stencil (grid_interp.f:22-55) on a face-centred field
+template <typename T>
+__device__ __forceinline__ void e2c3(const Geo& g, const FC& f, const T* b, const T* b2, int i,
+                                     int j, int k, double o3[3]) {
+  auto B = [&](int m, int ii, int jj, int kk) -> double {
+    int64_t q = gidx(g, ii, jj, kk) + (int64_t)m * g.nn;
+    return b2 ? 0.5 * ((double)b[q] + (double)b2[q]) : (double)b[q];
+  };
+  double hi, lo;
+  hi = B(0, i, j - 1, k) + f.ry * (B(0, i, j, k) - B(0, i, j - 1, k));
+  lo = B(0, i, j - 1, k - 1) + f.ry * (B(0, i, j, k - 1) - B(0, i, j - 1, k - 1));
+  o3[0] = lo + f.rz * (hi - lo);
+  hi = B(1, i - 1, j, k) + f.rx * (B(1, i, j, k) - B(1, i - 1, j, k));
+  lo = B(1, i - 1, j, k - 1) + f.rx * (B(1, i, j, k - 1) - B(1, i - 1, j, k - 1));
+  o3[1] = lo + f.rz * (hi - lo);
+  hi = B(2, i, j - 1, k) + f.ry * (B(2, i, j, k) - B(2, i, j - 1, k));
+  lo = B(2, i - 1, j - 1, k) + f.ry * (B(2, i - 1, j, k) - B(2, i - 1, j - 1, k));
+  o3[2] = lo + f.rx * (hi - lo);
+}
+
+// face_to_center stencil (grid_interp.f:150-170) on an edge-centred field
+template <typename R>
+__device__ __forceinline__ void f2c3(const Geo& g, const FC& f, const R* v, int i, int j, int k,
+                                     R o3[3]) {
+  o3[0] = (R)f.rx * (AT(v, 0, i, j, k) - AT(v, 0, i - 1, j, k)) + AT(v, 0, i - 1, j, k);
+  o3[1] = (R)f.ry * (AT(v, 1, i, j, k) - AT(v, 1, i, j - 1, k)) + AT(v, 1, i, j - 1, k);
+  o3[2] = (R)f.rz * (AT(v, 2, i, j, k) - AT(v, 2, i, j, k - 1)) + AT(v, 2, i, j, k - 1);
+}
+
+// curlB, gutsf.f:178-244: aj = curl B / (alpha n_edge); optional a = aj - up
+template <typename R>
+__global__ void k_curlB(Geo g, FC f, const double* b, const double* b2, const R* np, const R* up,
+                        R* aj, R* a) {
+  GRID_LOOP_HEAD
+  R n0 = np[gidx(g, ci, cj, ck)];
+  R nt1 = (R)0.5 * (n0 + np[gidx(g, ci + 1, cj, ck)]);
+  R nt2 = (R)0.5 * (n0 + np[gidx(g, ci, cj + 1, ck)]);
+  R nt3 = (R)0.5 * (n0 + np[gidx(g, ci, cj, ck + 1)]);
+  double c1 = (bval(g, b, b2, 2, ci, cj, ck) - bval(g, b, b2, 2, ci, cj - 1, ck)) / f.dy +
+              (bval(g, b, b2, 1, ci, cj, ck - 1) - bval(g, b, b2, 1, ci, cj, ck)) / f.dz;
+  double c2 = (bval(g, b, b2, 0, ci, cj, ck) - bval(g, b, b2, 0, ci, cj, ck - 1)) / f.dz +
+              (bval(g, b, b2, 2, ci - 1, cj, ck) - bval(g, b, b2, 2, ci, cj, ck)) / f.dx;
+  double c3 = (bval(g, b, b2, 1, ci, cj, ck) - bval(g, b, b2, 1, ci - 1, cj, ck)) / f.dx +
+              (bval(g, b, b2, 0, ci, cj - 1, ck) - bval(g, b, b2, 0, ci, cj, ck)) / f.dy;
+  R al = (R)f.alpha;
+  R j1 = (R)c1 / (nt1 * al), j2 = (R)c2 / (nt2 * al), j3 = (R)c3 / (nt3 * al);
+  aj[o] = j1; aj[o + g.nn] = j2; aj[o + 2 * g.nn] = j3;
+  if (a) {
+    int64_t oc = gidx(g, ci, cj, ck);
+    a[o] = j1 - up[oc];
+    a[o + g.nn] = j2 - up[oc + g.nn];
+    a[o + 2 * g.nn] = j3 - up[oc + 2 * g.nn];
+  }
+}
+
+// get_E's node part (gutsf.f:1067-1069): ct = face_to_center(a) x edge_to_center(B)
+template <typename R>
+__global__ void k_cross_nodes(Geo g, FC f, const R* a, const double* b, R* ct) {
+  GRID_LOOP_HEAD
+  R aa[3];
+  f2c3(g, f, a, ci, cj, ck, aa);
+  double bd[3];
+  e2c3<double>(g, f, b, nullptr, ci, cj, ck, bd);
+  R bx = (R)bd[0], by = (R)bd[1], bz = (R)bd[2];
+  ct[o] = aa[1] * bz - aa[2] * by;
+  ct[o + g.nn] = aa[2] * bx - aa[0] * bz;
+  ct[o + 2 * g.nn] = aa[0] * by - aa[1] * bx;
+}
+
+// edge_to_center of B (optionally of (b+b2)/2), into a real node field
+template <typename R>
+__global__ void k_e2c_B(Geo g, FC f, const double* b, const double* b2, R* out) {
+  GRID_LOOP_HEAD
+  double bd[3];
+  e2c3<double>(g, f, b, b2, ci, cj, ck, bd);
+  out[o] = (R)bd[0]; out[o + g.nn] = (R)bd[1]; out[o + 2 * g.nn] = (R)bd[2];
+}
+
+// edge_to_center of a real field (routine-level API)
+template <typename R>
+__global__ void k_e2c(Geo g, FC f, const R* in, R* out) {
+  GRID_LOOP_HEAD
+  double bd[3];
+  e2c3<R>(g, f, in, nullptr, ci, cj, ck, bd);
+  out[o] = (R)bd[0]; out[o + g.nn] = (R)bd[1]; out[o + 2 * g.nn] = (R)bd[2];
+}
+
+// get_Ep1's node part with the smoothed B (gutsf.f:1266-1273):
+// btmf = edge_to_face average of btc0 (grid_interp.f:119-127), then
+// face_to_center(btmf), crossed with face_to_center(a)
+template <typename R>
+__global__ void k_cross_smooth(Geo g, FC f, const R* a, const R* btc0, R* ct) {
+  GRID_LOOP_HEAD
+  R aa[3];
+  f2c3(g, f, a, ci, cj, ck, aa);
+  int im = canon(ci - 1, g.nx), jm = canon(cj - 1, g.ny), km = canon(ck - 1, g.nz);
+  R m1c = (R)0.5 * (AT(btc0, 0, ci, cj, ck) + AT(btc0, 0, ci + 1, cj, ck));
+  R m1m = (R)0.5 * (AT(btc0, 0, im, cj, ck) + AT(btc0, 0, im + 1, cj, ck));
+  R m2c = (R)0.5 * (AT(btc0, 1, ci, cj, ck) + AT(btc0, 1, ci, cj + 1, ck));
+  R m2m = (R)0.5 * (AT(btc0, 1, ci, jm, ck) + AT(btc0, 1, ci, jm + 1, ck));
+  R m3c = (R)0.5 * (AT(btc0, 2, ci, cj, ck) + AT(btc0, 2, ci, cj, ck + 1));
+  R m3m = (R)0.5 * (AT(btc0, 2, ci, cj, km) + AT(btc0, 2, ci, cj, km + 1));
+  R bx = (R)f.rx * (m1c - m1m) + m1m;
+  R by = (R)f.ry * (m2c - m2m) + m2m;
+  R bz = (R)f.rz * (m3c - m3m) + m3m;
+  ct[o] = aa[1] * bz - aa[2] * by;
+  ct[o + g.nn] = aa[2] * bx - aa[0] * bz;
+  ct[o + 2 * g.nn] = aa[0] * by - aa[1] * bx;
+}
+
+// crossf2's edge average plus E = c + nu aj (gutsf.f:84-98, 1078-1085)
+template <typename R>
+__global__ void k_edgeavg_E(Geo g, const R* ct, const R* aj, const R* nu, R* E) {
+  GRID_LOOP_HEAD
+  int64_t oc = gidx(g, ci, cj, ck);
+  R nuv = nu[oc];
+  E[o] = (R)0.5 * (AT(ct, 0, ci, cj, ck) + AT(ct, 0, ci + 1, cj, ck)) + nuv * aj[oc];
+  E[o + g.nn] = (R)0.5 * (AT(ct, 1, ci, cj, ck) + AT(ct, 1, ci, cj + 1, ck)) + nuv * aj[oc + g.nn];
+  E[o + 2 * g.nn] =
+      (R)0.5 * (AT(ct, 2, ci, cj, ck) + AT(ct, 2, ci, cj, ck + 1)) + nuv * aj[oc + 2 * g.nn];
+}
+
+// curlE (gutsf.f:262-303) in FP64
+template <typename R>
+__device__ __forceinline__ void curlE3(const Geo& g, const FC& f, const R* E, int i, int j, int k,
+                                       double c[3]) {
+  c[0] = ((double)AT(E, 2, i, j + 1, k) - (double)AT(E, 2, i, j, k)) / f.dy +
+         ((double)AT(E, 1, i, j, k) - (double)AT(E, 1, i, j, k + 1)) / f.dz;
+  c[1] = ((double)AT(E, 2, i, j, k) - (double)AT(E, 2, i + 1, j, k)) / f.dx +
+         ((double)AT(E, 0, i, j, k + 1) - (double)AT(E, 0, i, j, k)) / f.dz;
+  c[2] = ((double)AT(E, 0, i, j, k) - (double)AT(E, 0, i, j + 1, k)) / f.dy +
+         ((double)AT(E, 1, i + 1, j, k) - (double)AT(E, 1, i, j, k)) / f.dx;
+}
+
+// predict_B / correct_B update (gutsf.f:1158-1159, 1349-1350):
+// out = base - coef * curlE(E), coef = 2 dtsub or dtsub
+template <typename R>
+__global__ void k_curlE_update(Geo g, FC f, const R* E, const double* base, double coef,
+                               double* out) {
+  GRID_LOOP_HEAD
+  double cE[3];
+  curlE3(g, f, E, ci, cj, ck, cE);
+  int64_t oc = gidx(g, ci, cj, ck);
+#pragma unroll
+  for (int m = 0; m < 3; m++) out[o + m * g.nn] = base[oc + m * g.nn] - coef * cE[m];
+}
+
+template <typename R>
+__global__ void k_curlE(Geo g, FC f, const R* E, R* out) {
+  GRID_LOOP_HEAD
+  double cE[3];
+  curlE3(g, f, E, ci, cj, ck, cE);
+#pragma unroll
+  for (int m = 0; m < 3; m++) out[o + m * g.nn] = (R)cE[m];
+}
+
+template <typename R>
+__global__ void k_f2c(Geo g, FC f, const R* in, R* out) {
+  GRID_LOOP_HEAD
+  R v[3];
+  f2c3(g, f, in, ci, cj, ck, v);
+  out[o] = v[0]; out[o + g.nn] = v[1]; out[o + 2 * g.nn] = v[2];
+}
+
+// second half of edge_to_face (grid_interp.f:119-127) from node values btc0
+template <typename R>
+__global__ void k_face_avg(Geo g, const R* btc0, R* out) {
+  GRID_LOOP_HEAD
+  out[o] = (R)0.5 * (AT(btc0, 0, ci, cj, ck) + AT(btc0, 0, ci + 1, cj, ck));
+  out[o + g.nn] = (R)0.5 * (AT(btc0, 1, ci, cj, ck) + AT(btc0, 1, ci, cj + 1, ck));
+  out[o + 2 * g.nn] = (R)0.5 * (AT(btc0, 2, ci, cj, ck) + AT(btc0, 2, ci, cj, ck + 1));
+}
+
+// crossf2 (gutsf.f:40-104) for the routine-level API
+template <typename R>
+__global__ void k_crossf2(Geo g, const R* aa, const R* bb, R* out) {
+  GRID_LOOP_HEAD
+  auto cr = [&](int m, int ii, int jj, int kk) -> R {
+    R ax = AT(aa, 0, ii, jj, kk), ay = AT(aa, 1, ii, jj, kk), az = AT(aa, 2, ii, jj, kk);
+    R bx = AT(bb, 0, ii, jj, kk), by = AT(bb, 1, ii, jj, kk), bz = AT(bb, 2, ii, jj, kk);
+    return m == 0 ? ay * bz - az * by : (m == 1 ? az * bx - ax * bz : ax * by - ay * bx);
+  };
+  out[o] = (R)0.5 * (cr(0, ci, cj, ck) + cr(0, ci + 1, cj, ck));
+  out[o + g.nn] = (R)0.5 * (cr(1, ci, cj, ck) + cr(1, ci, cj + 1, ck));
+  out[o + 2 * g.nn] = (R)0.5 * (cr(2, ci, cj, ck) + cr(2, ci, cj, ck + 1));
+}
+
+template <typename T>
+__global__ void k_periodic(Geo g, T* f, int ncomp) {
+  GRID_LOOP_HEAD
+  if (ci == i && cj == j && ck == k) return;
+  int64_t oc = gidx(g, ci, cj, ck);
+  for (int m = 0; m < ncomp; m++) f[o + m * g.nn] = f[oc + m * g.nn];
+}
+
+template <typename R>
+__global__ void k_a_from(Geo g, const R* aj, const R* up, R* a) {
+  GRID_LOOP_HEAD
+  int64_t oc = gidx(g, ci, cj, ck);
+#pragma unroll
+  for (int m = 0; m < 3; m++) a[o + m * g.nn] = aj[oc + m * g.nn] - up[oc + m * g.nn];
+}
+
+// get_Ep's node fields (gutsp.f:183-184): ajc = face_to_center(aj),
+// upc = face_to_center(up); the particle gather source is (ajc-upc, btc)
+template <typename R>
+__global__ void k_build_src(Geo g, FC f, const R* aj, const R* up, const R* btc, R* ajc, R* upc,
+                            R* S) {
+  GRID_LOOP_HEAD
+  R jc[3], uc[3];
+  f2c3(g, f, aj, ci, cj, ck, jc);
+  f2c3(g, f, up, ci, cj, ck, uc);
+  int64_t oc = gidx(g, ci, cj, ck);
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    ajc[o + m * g.nn] = jc[m];
+    if (upc) upc[o + m * g.nn] = uc[m];
+    S[o + m * g.nn] = jc[m] - uc[m];
+    S[o + (3 + m) * g.nn] = btc[oc + m * g.nn];
+  }
+}
+
+template <typename R>
+__global__ void k_update_src(Geo g, FC f, const R* up, const R* ajc, R* upc, R* S) {
+  GRID_LOOP_HEAD
+  R uc[3];
+  f2c3(g, f, up, ci, cj, ck, uc);
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    if (upc) upc[o + m * g.nn] = uc[m];
+    S[o + m * g.nn] = ajc[o + m * g.nn] - uc[m];
+  }
+}
+
+// np = deposit sum / (vol beta) at every node (update_np's scaling,
+// gutsp.f:1480-1491; the fold of :1496-1498 happened at deposit time)
+template <typename R>
+__global__ void k_finalize_np(Geo g, const R* hi, const R* lo, R qn, R* np) {
+  GRID_LOOP_HEAD
+  np[o] = acc_read<R>(hi, lo, gidx(g, ci, cj, ck)) * qn;
+}
+
+// update_up's division by np and node->edge average (gutsp.f:1704-1756,
+// 1825-1836); hi/lo point at moment 1 (x velocity), moments are nn apart
+template <typename R>
+__global__ void k_finalize_up(Geo g, const R* hi, const R* lo, const R* np, R qn, R* up) {
+  GRID_LOOP_HEAD
+  int64_t q0 = gidx(g, ci, cj, ck);
+  int64_t qn1[3] = {gidx(g, canon(ci + 1, g.nx), cj, ck), gidx(g, ci, canon(cj + 1, g.ny), ck),
+                    gidx(g, ci, cj, canon(ck + 1, g.nz))};
+  R n0 = np[q0];
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    R n1 = np[qn1[m]];
+    R c0 = n0 != (R)0 ? acc_read<R>(hi, lo, q0 + m * g.nn) * qn / n0 : (R)0;
+    R c1 = n1 != (R)0 ? acc_read<R>(hi, lo, qn1[m] + m * g.nn) * qn / n1 : (R)0;
+    up[o + m * g.nn] = (R)0.5 * (c0 + c1);
+  }
+}
+
+// check_time_step's per-node whistler bound (gutsf.f:1782-1792)
+template <typename R>
+__global__ void k_deltat(Geo g, FC f, const double* b, const R* np, R* dtn, float* dmin) {
+  GRID_LOOP_HEAD
+  R b1 = (R)b[o], b2 = (R)b[o + g.nn], b3 = (R)b[o + 2 * g.nn];
+  R ak = (R)2. / (R)f.dx;
+  R btot = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+  R al = (R)f.alpha;
+  R a1 = ak * ak * btot / (al * np[o]);
+  R a2 = (ak * btot) * (ak * btot) / (al * np[o]);
+  R womega = (R)0.5 * (a1 + sqrt(a1 * a1 + (R)4 * a2));
+  R phi = womega / ak;
+  R dt = (R)f.dx / phi;
+  dtn[o] = dt;
+  float d = (float)dt;
+  if (!(d >= 0.f)) d = 0.f;  // NaN or negative: treat as failing
+  atomicMin((unsigned int*)dmin, __float_as_uint(d));
+}
+
+// the reference halves dtsub once per failing node, in its loop order
+// (i outer, j, k inner) and against the dtsub current at that node
+// (gutsf.f:1793-1796).  Rare path; one workgroup walks the nodes in order.
+template <typename R>
+__global__ void k_halving(Geo g, const R* dtn, R dtsub0, int* out_h) {
+  __shared__ int flag[1024];
+  R dtsub = dtsub0;
+  int h = 0;
+  int64_t total = g.nn;
+  for (int64_t base = 0; base < total; base += 1024) {
+    int64_t r = base + threadIdx.x;
+    int cand = 0;
+    if (r < total) {
+      int k = (int)(r % g.nz);
+      int j = (int)((r / g.nz) % g.ny);
+      int i = (int)(r / ((int64_t)g.nz * g.ny));
+      cand = dtn[gidx(g, i, j, k)] <= (R)2.0 * dtsub0;
+    }
+    flag[threadIdx.x] = cand;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int t = 0; t < 1024 && base + t < total; t++) {
+        if (!flag[t]) continue;
+        int64_t rr = base + t;
+        int k = (int)(rr % g.nz);
+        int j = (int)((rr / g.nz) % g.ny);
+        int i = (int)(rr / ((int64_t)g.nz * g.ny));
+        if (dtn[gidx(g, i, j, k)] <= (R)2.0 * dtsub) {
+          dtsub = dtsub / (R)2.0;
+          h++;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_h = h;
+}
+
+template <typename R>
+R* F(swap_ctx* c, int id) {
+  return (R*)c->fld[id];
+}
+
+}  // namespace
+
+#define DISPATCH(c, ...)                        \
+  do {                                          \
+    if ((c)->prec == 64) {                      \
+      using R = double;                         \
+      __VA_ARGS__;                              \
+    } else {                                    \
+      using R = float;                          \
+      __VA_ARGS__;                              \
+    }                                           \
+  } while (0)
+
+#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, ggrid(c->g), gblock(), 0, c->stream, __VA_ARGS__)
+
+static int post(swap_ctx* c) {
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int fl_curlB(swap_ctx* c, const double* b, const double* b2, void* aj, void* a) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_curlB<R>), c->g, f, b, b2, F<R>(c, SWAP_F_NP), F<R>(c, SWAP_F_UP), (R*)aj,
+                     (R*)a));
+  return post(c);
+}
+int fl_cross_nodes(swap_ctx* c, const void* a, const double* b, void* ct) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_cross_nodes<R>), c->g, f, (const R*)a, b, (R*)ct));
+  return post(c);
+}
+int fl_e2c_B(swap_ctx* c, const double* b, const double* b2, void* out) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_e2c_B<R>), c->g, f, b, b2, (R*)out));
+  return post(c);
+}
+int fl_cross_smooth(swap_ctx* c, const void* a, const void* btc0, void* ct) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_cross_smooth<R>), c->g, f, (const R*)a, (const R*)btc0, (R*)ct));
+  return post(c);
+}
+int fl_edgeavg_E(swap_ctx* c, const void* ct, const void* aj, void* E) {
+  DISPATCH(c, LAUNCH((k_edgeavg_E<R>), c->g, (const R*)ct, (const R*)aj, F<R>(c, SWAP_F_NU),
+                     (R*)E));
+  return post(c);
+}
+int fl_curlE_update(swap_ctx* c, const void* E, const double* base, double coef, double* out) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_curlE_update<R>), c->g, f, (const R*)E, base, coef, out));
+  return post(c);
+}
+int fl_curlE(swap_ctx* c, const void* E, void* out) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_curlE<R>), c->g, f, (const R*)E, (R*)out));
+  return post(c);
+}
+int fl_f2c(swap_ctx* c, const void* in, void* out) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_f2c<R>), c->g, f, (const R*)in, (R*)out));
+  return post(c);
+}
+int fl_e2c(swap_ctx* c, const void* in, void* out) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_e2c<R>), c->g, f, (const R*)in, (R*)out));
+  return post(c);
+}
+int fl_e2f(swap_ctx* c, const void* in, const double* bin, void* out) {
+  FC f = fc_of(c);
+  DISPATCH(c, if (bin) LAUNCH((k_e2c_B<R>), c->g, f, bin, (const double*)nullptr, (R*)c->btc0);
+           else LAUNCH((k_e2c<R>), c->g, f, (const R*)in, (R*)c->btc0);
+           LAUNCH((k_face_avg<R>), c->g, (const R*)c->btc0, (R*)out));
+  return post(c);
+}
+int fl_crossf2(swap_ctx* c, const void* aa, const void* btc, void* out) {
+  DISPATCH(c, LAUNCH((k_crossf2<R>), c->g, (const R*)aa, (const R*)btc, (R*)out));
+  return post(c);
+}
+int fl_periodic(swap_ctx* c, void* f, int ncomp, int is_double) {
+  if (is_double || c->prec == 64)
+    LAUNCH((k_periodic<double>), c->g, (double*)f, ncomp);
+  else
+    LAUNCH((k_periodic<float>), c->g, (float*)f, ncomp);
+  return post(c);
+}
+int fl_a_from(swap_ctx* c, const void* aj, const void* up, void* a) {
+  DISPATCH(c, LAUNCH((k_a_from<R>), c->g, (const R*)aj, (const R*)up, (R*)a));
+  return post(c);
+}
+int fl_build_src(swap_ctx* c, const void* aj, const void* up, const void* btc, void* ajc,
+                 void* S) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_build_src<R>), c->g, f, (const R*)aj, (const R*)up, (const R*)btc,
+                     (R*)ajc, F<R>(c, SWAP_F_UPC), (R*)S));
+  return post(c);
+}
+int fl_update_src(swap_ctx* c, const void* up, const void* ajc, void* S) {
+  FC f = fc_of(c);
+  DISPATCH(c, LAUNCH((k_update_src<R>), c->g, f, (const R*)up, (const R*)ajc,
+                     F<R>(c, SWAP_F_UPC), (R*)S));
+  return post(c);
+}
+int fl_finalize_np(swap_ctx* c) {
+  DISPATCH(c, LAUNCH((k_finalize_np<R>), c->g, (const R*)c->mhi, (const R*)c->mlo, (R)c->qn,
+                     F<R>(c, SWAP_F_NP)));
+  return post(c);
+}
+int fl_finalize_up(swap_ctx* c) {
+  DISPATCH(c, LAUNCH((k_finalize_up<R>), c->g, (const R*)c->mhi + c->g.nn,
+                     (const R*)c->mlo + c->g.nn, F<R>(c, SWAP_F_NP), (R)c->qn,
+                     F<R>(c, SWAP_F_UP)));
+  return post(c);
+}
+int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub) {
+  FC f = fc_of(c);
+  double dtsub0 = c->cfg.dt / (double)c->cfg.ntsub;
+  float big = 3.0e38f;
+  HIPCHK(c, hipMemcpyAsync(c->dmin, &big, sizeof(float), hipMemcpyHostToDevice, c->stream));
+  DISPATCH(c, LAUNCH((k_deltat<R>), c->g, f, c->b1, F<R>(c, SWAP_F_NP), (R*)c->scratch, c->dmin));
+  HIPCHK(c, hipGetLastError());
+  float mn = 0;
+  HIPCHK(c, hipMemcpyAsync(&mn, c->dmin, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int h = 0;
+  // every node passes unless some deltat <= 2 dtsub0 (compare in the
+  // context precision, as the per-node test does)
+  bool maybe = c->prec == 64 ? ((double)mn <= 2.0 * dtsub0 * (1 + 1e-6))
+                             : (mn <= 2.0f * (float)dtsub0 * (1 + 1e-6f));
+  if (maybe) {
+    int* dh = (int*)c->dred;
+    DISPATCH(c, hipLaunchKernelGGL((k_halving<R>), dim3(1), dim3(1024), 0, c->stream,
+                                   c->g, (const R*)c->scratch, (R)dtsub0, dh));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&h, dh, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  double ds = dtsub0;
+  int n = c->cfg.ntsub;
+  for (int q = 0; q < h; q++) {
+    if (c->prec == 64) ds = ds / 2.0; else ds = (double)((float)ds / 2.0f);
+    n *= 2;
+  }
+  *ntf = n;
+  *dtsub = c->prec == 64 ? ds : (double)(float)ds;
+  return 0;
+}
+int fl_copy_b(swap_ctx* c, const double* src, double* dst) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, sizeof(double) * 3 * c->g.nn, hipMemcpyDeviceToDevice,
+                           c->stream));
+  return 0;
+}
+int fl_zero(swap_ctx* c, void* p, size_t bytes) {
+  HIPCHK(c, hipMemsetAsync(p, 0, bytes, c->stream));
+  return 0;
+}

@@ -1,0 +1,328 @@
+// k_sources.hip -- particle loading, pickup-ion sources, FP64 diagnostics.
+#include "swap_internal.h"
+
+namespace {
+
+constexpr double PI_REF = 3.14159;  // the reference's PI (inputs.f:52)
+
+// absolute position x (km, uniform grid with q(i) = i d) -> cell + fraction,
+// right-closed as get_interp_weights' search (gutsp.f:1394-1412)
+__device__ __forceinline__ void split_pos(double x, double d, int nc, int& cell, double& f) {
+  double t = x / d;          // q(i) = i d
+  double i = ceil(t) - 1.0;  // Fortran cell index i: q(i) < x <= q(i+1)
+  f = t - i;
+  int c = (int)i - 1;        // 0-based cell
+  c %= nc;
+  if (c < 0) c += nc;
+  cell = c;
+}
+
+template <typename R>
+__global__ void k_load_abs(int64_t n, int64_t off, PDev<R> P, const double* xp, const double* vp,
+                           const double* vp1, double dx, double dy, double dz, int ncx, int ncy,
+                           int ncz) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int ci, cj, ck;
+  double f0, f1, f2;
+  split_pos(xp[p], dx, ncx, ci, f0);
+  split_pos(xp[p + n], dy, ncy, cj, f1);
+  split_pos(xp[p + 2 * n], dz, ncz, ck, f2);
+  int64_t q = off + p;
+  P.cell[q] = pack_cell(ci, cj, ck);
+  P.fx[q] = (R)f0; P.fy[q] = (R)f1; P.fz[q] = (R)f2;
+  P.vx[q] = (R)vp[p]; P.vy[q] = (R)vp[p + n]; P.vz[q] = (R)vp[p + 2 * n];
+  P.v1x[q] = (R)vp1[p]; P.v1y[q] = (R)vp1[p + n]; P.v1z[q] = (R)vp1[p + 2 * n];
+}
+
+template <typename R>
+__global__ void k_export_abs(int64_t n, PDev<R> P, double* xp, double dx, double dy, double dz) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t c = P.cell[p];
+  xp[p] = ((double)cell_i(c) + 1.0 + (double)P.fx[p]) * dx;
+  xp[p + n] = ((double)cell_j(c) + 1.0 + (double)P.fy[p]) * dy;
+  xp[p + 2 * n] = ((double)cell_k(c) + 1.0 + (double)P.fz[p]) * dz;
+}
+
+// load_Maxwellian's sampler (part_init.f:176-206): position uniform on
+// (q(1), q(n-1)] from 1-u, velocity vsw + vth sqrt(-ln u1) cos(PI u2) per
+// component, vp1 = vp; uniforms from Philox-4x32-10 keyed by the seed, with
+// counter (id lo, id hi, stream, draw)
+template <typename R>
+__global__ void k_maxwellian(int64_t n, int64_t off, PDev<R> P, uint32_t k0, uint32_t k1,
+                             uint32_t stream, double vsw, double vth, int ncx, int ncy, int ncz,
+                             float bpv, float mrv, float* bp, float* mr, uint32_t* id) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint64_t gid = (uint64_t)(off + p);
+  uint32_t r[3][4];
+  for (int d = 0; d < 3; d++) {
+    r[d][0] = (uint32_t)gid;
+    r[d][1] = (uint32_t)(gid >> 32);
+    r[d][2] = stream;
+    r[d][3] = (uint32_t)d;
+    philox4x32_10(r[d], k0, k1);
+  }
+  int nc[3] = {ncx, ncy, ncz};
+  int ijk[3];
+  double f[3];
+  for (int a = 0; a < 3; a++) {
+    double t = (1.0 - u24(r[0][a])) * (double)nc[a];  // exact in double
+    double i = ceil(t) - 1.0;
+    ijk[a] = (int)i;
+    f[a] = t - i;
+  }
+  double v[3];
+  const uint32_t* ur[6] = {&r[0][3], &r[1][0], &r[1][1], &r[1][2], &r[1][3], &r[2][0]};
+  for (int a = 0; a < 3; a++) {
+    double u1 = 1.0 - u24(*ur[2 * a]);  // (0,1]
+    double u2 = u24(*ur[2 * a + 1]);
+    v[a] = (a == 0 ? vsw : 0.0) + vth * sqrt(-log(u1)) * cos(PI_REF * u2);
+  }
+  int64_t q = off + p;
+  P.cell[q] = pack_cell(ijk[0], ijk[1], ijk[2]);
+  P.fx[q] = (R)f[0]; P.fy[q] = (R)f[1]; P.fz[q] = (R)f[2];
+  P.vx[q] = (R)v[0]; P.vy[q] = (R)v[1]; P.vz[q] = (R)v[2];
+  P.v1x[q] = (R)v[0]; P.v1y[q] = (R)v[1]; P.v1z[q] = (R)v[2];
+  if (bp) bp[q] = bpv;
+  if (mr) mr[q] = mrv;
+  if (id) id[q] = (uint32_t)gid;
+}
+
+// Ionize_sw_mp's new ions (chem_rates.f:567-608): ring velocity
+// (vsw + vsw cos th, vsw sin th, 0) with th = 2 PI u, uniform position,
+// mrat = 1, beta_p = beta_pu
+template <typename R>
+__global__ void k_ionize_sw(int64_t n, int64_t off, PDev<R> P, uint32_t k0, uint32_t k1,
+                            uint32_t stream, double vsw, int ncx, int ncy, int ncz, float beta_pu,
+                            float* bp, float* mr, uint32_t* id) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t r[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 7u};
+  philox4x32_10(r, k0, k1);
+  double th = u24(r[0]) * 2.0 * PI_REF;
+  int nc[3] = {ncx, ncy, ncz};
+  int ijk[3];
+  double f[3];
+  for (int a = 0; a < 3; a++) {
+    double t = (1.0 - u24(r[1 + a])) * (double)nc[a];
+    double i = ceil(t) - 1.0;
+    ijk[a] = (int)i;
+    f[a] = t - i;
+  }
+  double v[3] = {vsw + vsw * cos(th), vsw * sin(th), 0.0};
+  int64_t q = off + p;
+  P.cell[q] = pack_cell(ijk[0], ijk[1], ijk[2]);
+  P.fx[q] = (R)f[0]; P.fy[q] = (R)f[1]; P.fz[q] = (R)f[2];
+  P.vx[q] = (R)v[0]; P.vy[q] = (R)v[1]; P.vz[q] = (R)v[2];
+  P.v1x[q] = (R)v[0]; P.v1y[q] = (R)v[1]; P.v1z[q] = (R)v[2];
+  if (bp) bp[q] = beta_pu;
+  if (mr) mr[q] = 1.0f;
+  if (id) id[q] = (uint32_t)q;
+}
+
+// ---- FP64 reductions ----
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// particle sums: [0] charge, [1..3] momentum, [4] kinetic energy
+template <typename R>
+__global__ void k_diag_particles(int64_t n, PDev<R> P, double beta, double mion, double* out) {
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    double bp = P.bp ? (double)P.bp[p] : 1.0;
+    double mr = P.mr ? (double)P.mr[p] : 1.0;
+    double w = 1.0 / (beta * bp);
+    double m = mion / mr;
+    double vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
+    s[0] += w;
+    s[1] += w * m * vx;
+    s[2] += w * m * vy;
+    s[3] += w * m * vz;
+    s[4] += w * 0.5 * m * (vx * vx + vy * vy + vz * vz) * 1e6;  // (km/s)^2 -> J
+  }
+  for (int q = 0; q < 5; q++) {
+    double t = wave_sum(s[q]);
+    if (__lane_id() == 0) atomicAdd(out + q, t);
+  }
+}
+
+// grid sums over the unique dual cells: [5] sum (div b1)^2, [6] sum |b1|^2
+__global__ void k_diag_grid(Geo g, const double* b, double dx, double dy, double dz,
+                            double* out) {
+  double s0 = 0, s1 = 0;
+  int64_t ncell = (int64_t)g.ncx * g.ncy * g.ncz;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ncell;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int i = 1 + (int)(q % g.ncx), j = 1 + (int)((q / g.ncx) % g.ncy),
+        k = 1 + (int)(q / ((int64_t)g.ncx * g.ncy));
+    int64_t o = gidx(g, i, j, k);
+    double d = (b[gidx(g, i + 1, j, k)] - b[o]) / dx +
+               (b[gidx(g, i, j + 1, k) + g.nn] - b[o + g.nn]) / dy +
+               (b[gidx(g, i, j, k + 1) + 2 * g.nn] - b[o + 2 * g.nn]) / dz;
+    s0 += d * d;
+    s1 += b[o] * b[o] + b[o + g.nn] * b[o + g.nn] + b[o + 2 * g.nn] * b[o + 2 * g.nn];
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (__lane_id() == 0) {
+    atomicAdd(out + 5, s0);
+    atomicAdd(out + 6, s1);
+  }
+}
+
+int post(swap_ctx* c) {
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+#define SDISPATCH(c, ...)   \
+  do {                      \
+    if ((c)->prec == 64) {  \
+      using R = double;     \
+      __VA_ARGS__;          \
+    } else {                \
+      using R = float;      \
+      __VA_ARGS__;          \
+    }                       \
+  } while (0)
+
+int pl_load_abs(swap_ctx* c, int64_t n, const double* xp_d, const double* v_d,
+                const double* v1_d) {
+  if (n == 0) return 0;
+  SDISPATCH(c, hipLaunchKernelGGL((k_load_abs<R>), dim3(nblk(n, 256)), dim3(256), 0, c->stream, n,
+                                  (int64_t)0, pdev<R>(c), xp_d, v_d, v1_d, c->cfg.dx, c->cfg.dy,
+                                  c->cfg.dz, c->cfg.ncx, c->cfg.ncy, c->cfg.ncz));
+  return post(c);
+}
+
+int pl_export_abs(swap_ctx* c, double* xp_d) {
+  if (c->n == 0) return 0;
+  SDISPATCH(c, hipLaunchKernelGGL((k_export_abs<R>), dim3(nblk(c->n, 256)), dim3(256), 0,
+                                  c->stream, c->n, pdev<R>(c), xp_d, c->cfg.dx, c->cfg.dy,
+                                  c->cfg.dz));
+  return post(c);
+}
+
+int pl_maxwellian_full(swap_ctx* c, int64_t off, int64_t n, double vsw, double vth, float bpv,
+                       float mrv, uint32_t stream) {
+  if (n == 0) return 0;
+  uint32_t k0 = (uint32_t)c->cfg.seed, k1 = (uint32_t)(c->cfg.seed >> 32);
+  SDISPATCH(c, hipLaunchKernelGGL((k_maxwellian<R>), dim3(nblk(n, 256)), dim3(256), 0, c->stream,
+                                  n, off, pdev<R>(c), k0, k1, stream, vsw, vth, c->cfg.ncx,
+                                  c->cfg.ncy, c->cfg.ncz, bpv, mrv, c->bp, c->mr, c->id));
+  return post(c);
+}
+
+int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
+                 int64_t* injected) {
+  // dNi = ndot dt beta beta_pu dx dy delz nz (chem_rates.f:551), with the
+  // stochastic round-up of :553-558 and nint of :560
+  int nz_ref = c->cfg.ncz + 2;
+  double dNi = ndot * c->cfg.dt * c->cfg.beta * beta_pu * c->cfg.dx * c->cfg.dy * c->cfg.dz *
+               (double)nz_ref;
+  if (dNi < 1.0) {
+    uint32_t r[4] = {(uint32_t)step, 0u, 0x10u, 0u};
+    philox4x32_10(r, (uint32_t)c->cfg.seed, (uint32_t)(c->cfg.seed >> 32));
+    if (dNi > u24(r[0])) dNi = 1.0;
+  }
+  int64_t m = (int64_t)llround(dNi);
+  if (m < 0) m = 0;
+  if (c->n + m > c->cap) return set_err(c, SWAP_ECAP, "ionize: capacity %lld exceeded",
+                                        (long long)c->cap);
+  if (m > 0 && !(c->cfg.flags & SWAP_SPECIES) && beta_pu != 1.0)
+    return set_err(c, SWAP_EINVAL, "ionize: beta_pu != 1 needs SWAP_SPECIES");
+  uint32_t k0 = (uint32_t)c->cfg.seed, k1 = (uint32_t)(c->cfg.seed >> 32);
+  if (m > 0) {
+    SDISPATCH(c, hipLaunchKernelGGL((k_ionize_sw<R>), dim3(nblk(m, 256)), dim3(256), 0, c->stream,
+                                    m, c->n, pdev<R>(c), k0, k1, 0x20000u + (uint32_t)step, vsw,
+                                    c->cfg.ncx, c->cfg.ncy, c->cfg.ncz, (float)beta_pu, c->bp,
+                                    c->mr, c->id));
+    if (int e = post(c)) return e;
+  }
+  c->n += m;
+  c->injected += m;
+  *injected = m;
+  return 0;
+}
+
+int sr_diagnostics(swap_ctx* c, swap_diag* d) {
+  HIPCHK(c, hipMemsetAsync(c->dred, 0, 8 * sizeof(double), c->stream));
+  if (c->n > 0)
+    SDISPATCH(c, hipLaunchKernelGGL((k_diag_particles<R>), dim3(1024), dim3(256), 0, c->stream,
+                                    c->n, pdev<R>(c), c->cfg.beta, 1.67e-27, c->dred));
+  hipLaunchKernelGGL(k_diag_grid, dim3(1024), dim3(256), 0, c->stream, c->g, c->b1, c->cfg.dx,
+                     c->cfg.dy, c->cfg.dz, c->dred);
+  if (int e = post(c)) return e;
+  double h[8];
+  unsigned long long cnt[3];
+  HIPCHK(c, hipMemcpyAsync(h, c->dred, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cnt, c->dcnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d->charge = h[0];
+  d->momentum[0] = h[1]; d->momentum[1] = h[2]; d->momentum[2] = h[3];
+  d->kinetic = h[4];
+  d->divb_l2 = sqrt(h[5]);
+  d->b_energy = h[6];
+  d->nparticles = c->n;
+  d->escaped = c->escaped;
+  d->injected = c->injected;
+  d->window_misses = (int64_t)cnt[0];
+  return 0;
+}
+
+int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* injected,
+                    int64_t* chex) {
+  (void)step; (void)cl;
+  *injected = 0;
+  *chex = 0;
+  return set_err(c, SWAP_EINVAL, "ionize_cloud: not built yet");
+}
+
+namespace {
+__device__ __forceinline__ float wave_maxf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+template <typename R>
+__global__ void k_vmax(int64_t n, PDev<R> P, float* out) {
+  float m[4] = {0, 0, 0, 0};
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    float vx = fabsf((float)P.vx[p]), vy = fabsf((float)P.vy[p]), vz = fabsf((float)P.vz[p]);
+    m[0] = fmaxf(m[0], vx);
+    m[1] = fmaxf(m[1], vy);
+    m[2] = fmaxf(m[2], vz);
+    m[3] = fmaxf(m[3], sqrtf(vx * vx + vy * vy + vz * vz));
+  }
+  for (int q = 0; q < 4; q++) {
+    float t = wave_maxf(m[q]);
+    if (__lane_id() == 0) atomicMax((unsigned int*)out + q, __float_as_uint(t));
+  }
+}
+}  // namespace
+
+int sr_vmax(swap_ctx* c, double vmax[4]) {
+  float* d = (float*)c->dmin;
+  HIPCHK(c, hipMemsetAsync(d, 0, 4 * sizeof(float), c->stream));
+  if (c->n > 0)
+    SDISPATCH(c, hipLaunchKernelGGL((k_vmax<R>), dim3(512), dim3(256), 0, c->stream, c->n,
+                                    pdev<R>(c), d));
+  if (int e = post(c)) return e;
+  float h[4];
+  HIPCHK(c, hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < 4; q++) vmax[q] = h[q];
+  return 0;
+}
+
+int sr_outflow_compact(swap_ctx* c) {
+  (void)c;
+  return 0;
+}

@@ -1,0 +1,92 @@
+// swap_common.h -- shared device/host definitions of the gfx950 library.
+//
+// Grid storage is the reference's Fortran layout with two ghost layers:
+// index (i,j,k) 0-based storage, i fastest, component-major for vectors.
+// Storage index 0 and n-1 along an axis are the ghosts that boundary.f's
+// periodic() fills with copies of n-2 and 1 (boundary.f:18-46).  Every device
+// kernel writes ghosts as well as interior nodes by evaluating the interior
+// formula at the "canonical" node, which is exactly the value periodic()
+// would copy there, so device arrays are always periodic.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SWAP_HD __host__ __device__ __forceinline__
+
+struct Geo {
+  int nx, ny, nz;     // storage extents including ghosts
+  int ncx, ncy, ncz;  // physical cells = nx-2 ...
+  int64_t nn;         // nx*ny*nz
+};
+
+SWAP_HD int64_t gidx(const Geo& g, int i, int j, int k) {
+  return (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k);
+}
+// canonical storage index of a (possibly ghost) storage index s in [0,n)
+SWAP_HD int canon(int s, int n) { return s == 0 ? n - 2 : (s == n - 1 ? 1 : s); }
+// storage index of unwrapped node coordinate u (node u = lower corner of
+// cell u); folds ghost contributions into their periodic images
+// (update_np's fold, gutsp.f:1496-1498)
+SWAP_HD int node_of(int u, int nc) {
+  int r = (u - 1) % nc;
+  if (r < 0) r += nc;
+  return r + 1;
+}
+
+// packed cell: i | j<<11 | k<<21 (0-based cell indices)
+SWAP_HD uint32_t pack_cell(int i, int j, int k) {
+  return (uint32_t)i | ((uint32_t)j << 11) | ((uint32_t)k << 21);
+}
+SWAP_HD int cell_i(uint32_t c) { return (int)(c & 2047u); }
+SWAP_HD int cell_j(uint32_t c) { return (int)((c >> 11) & 1023u); }
+SWAP_HD int cell_k(uint32_t c) { return (int)(c >> 21); }
+
+// Philox-4x32-10 (Salmon et al., SC'11), counter-based: output depends only
+// on (counter, key), so every particle draws its own stream.
+SWAP_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+// uniform in [0,1) with 24 random bits (exact in float and double)
+SWAP_HD double u24(uint32_t r) { return (double)(r >> 8) * (1.0 / 16777216.0); }
+
+// ---- compensated accumulation ----
+// FP32 moments are accumulated as (hi, lo) pairs: the atomic add that
+// returns the old value lets the adding thread recover the exact rounding
+// error of its own addition (Knuth's TwoSum) and add it into lo.
+__device__ __forceinline__ float two_sum_err(float a, float b, float s) {
+  float bb = s - a;
+  return (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void acc_add(float* hi, float* lo, int64_t i, float v) {
+  if (v == 0.f) return;
+  float old = atomicAdd(hi + i, v);
+  float e = two_sum_err(old, v, old + v);
+  if (e != 0.f) atomicAdd(lo + i, e);
+}
+__device__ __forceinline__ void acc_add(double* hi, double* lo, int64_t i, double v) {
+  (void)lo;
+  if (v == 0.0) return;
+  atomicAdd(hi + i, v);
+}
+template <typename R>
+__device__ __forceinline__ R acc_read(const R* hi, const R* lo, int64_t i);
+template <>
+__device__ __forceinline__ float acc_read<float>(const float* hi, const float* lo, int64_t i) {
+  return hi[i] + lo[i];
+}
+template <>
+__device__ __forceinline__ double acc_read<double>(const double* hi, const double* lo, int64_t i) {
+  (void)lo;
+  return hi[i];
+}

@@ -1,0 +1,175 @@
+// swap_internal.h -- context layout and kernel launchers (not part of the ABI).
+#pragma once
+#include <string>
+#include <vector>
+#include "../../include/swap_hip.h"
+#include "swap_common.h"
+
+// particle-pass tiling: particles are counting-sorted by (tile, cell in
+// tile); one work item is a run of at most ITEM_MAX particles of one tile
+constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 4;
+constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
+constexpr int ITEM_MAX = 16384;
+constexpr int PASS_WG = 512;   // threads per particle-pass workgroup
+constexpr int PASS_KP = 8;     // consecutive particles per lane
+constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values
+
+struct Item {
+  int32_t start, end, tile, pad;
+};
+
+template <typename R>
+struct PDev {
+  uint32_t* cell;
+  R *fx, *fy, *fz, *vx, *vy, *vz, *v1x, *v1y, *v1z;
+  const float* bp;  // beta_p (null: 1)
+  const float* mr;  // mrat (null: 1)
+};
+
+template <typename R>
+struct PCoef {
+  R dt;        // particle step
+  R s[3];      // (dt/2)/d_axis, for the split-precision half move
+  int nc[3];   // cells per axis
+  int outflow; // x inflow/outflow instead of periodic
+};
+
+struct swap_ctx {
+  swap_config cfg;
+  Geo g;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  int prec = 32;
+  size_t rsz = 4;
+  double qn = 0;  // 1/(dx dy dz beta): deposit weight -> density
+  // grid state
+  double *b1 = nullptr, *b12 = nullptr, *b1p2 = nullptr;  // FP64 B advance
+  bool b1p2_alias = true;  // after f_update_tlev, b1p2 equals b1
+  void* fld[SWAP_F_COUNT] = {};  // real fields (B ids unused here)
+  void *a = nullptr, *ct = nullptr, *btc0 = nullptr, *cE = nullptr;  // scratch
+  void* S = nullptr;   // gather source: 6 components (aj-up at nodes, btc)
+  void *mhi = nullptr, *mlo = nullptr;  // moment accumulators [4][nn]
+  double dtsub = 0;
+  int ntf = 0;
+  // particles
+  int64_t n = 0, cap = 0;
+  uint32_t *cell = nullptr, *cell2 = nullptr;
+  void *p[9] = {}, *p2[9] = {};  // fx fy fz vx vy vz v1x v1y v1z
+  float *bp = nullptr, *mr = nullptr, *bp2 = nullptr, *mr2 = nullptr;
+  uint32_t *id = nullptr, *id2 = nullptr;
+  // routine-level particle scratch (allocated on first use)
+  void *Ep = nullptr, *vpl = nullptr, *vmi = nullptr, *wght = nullptr;
+  int32_t* ijkp = nullptr;
+  // sort and work items
+  int ntile[3] = {0, 0, 0};
+  int64_t ntiles = 0, nkeys = 0;
+  uint32_t *skey = nullptr, *scnt = nullptr, *sstart = nullptr, *scur = nullptr,
+           *sperm = nullptr, *sblk = nullptr;
+  Item* items = nullptr;
+  int* boxes = nullptr;       // [max_items][6]: cell box lo[3], hi[3]
+  int* boxes_next = nullptr;
+  int* d_nitems = nullptr;
+  int64_t max_items = 0;
+  int64_t n_sorted = -1;      // particles covered by the item list (-1: none)
+  int steps_since_sort = 0;
+  int margin[3] = {2, 1, 1};  // half-step displacement bound, cells
+  // counters
+  unsigned long long* dcnt = nullptr;  // [0] window misses [1] escapes [2] scratch
+  int64_t injected = 0, escaped = 0;
+  double* dred = nullptr;     // FP64 reduction scratch
+  float* dmin = nullptr;      // check_time_step minimum
+  void* scratch = nullptr;    // general device scratch
+  size_t scratch_bytes = 0;
+  int64_t step_count = 0;
+  // stage timing (swap_profile): event pairs recorded on `stream`
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev[SWAP_K_COUNT];  // start, stop, start, stop ...
+};
+
+// error helpers
+int set_err(swap_ctx* c, int code, const char* fmt, ...);
+#define HIPCHK(c, call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return set_err((c), SWAP_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- field launchers (k_fields.hip) ----
+int fl_curlB(swap_ctx* c, const double* b, const double* b2, void* aj, void* a);
+int fl_cross_nodes(swap_ctx* c, const void* a, const double* b, void* ct);
+int fl_e2c_B(swap_ctx* c, const double* b, const double* b2, void* out);
+int fl_cross_smooth(swap_ctx* c, const void* a, const void* btc0, void* ct);
+int fl_edgeavg_E(swap_ctx* c, const void* ct, const void* aj, void* E);
+int fl_curlE_update(swap_ctx* c, const void* E, const double* base, double coef, double* out);
+int fl_curlE(swap_ctx* c, const void* E, void* out);
+int fl_f2c(swap_ctx* c, const void* in, void* out);
+int fl_e2c(swap_ctx* c, const void* in, void* out);
+int fl_e2f(swap_ctx* c, const void* in, const double* bin, void* out);
+int fl_crossf2(swap_ctx* c, const void* aa, const void* btc, void* out);
+int fl_periodic(swap_ctx* c, void* f, int ncomp, int is_double);
+int fl_a_from(swap_ctx* c, const void* aj, const void* up, void* a);
+int fl_build_src(swap_ctx* c, const void* aj, const void* up, const void* btc, void* ajc,
+                 void* S);
+int fl_update_src(swap_ctx* c, const void* up, const void* ajc, void* S);
+int fl_finalize_np(swap_ctx* c);
+int fl_finalize_up(swap_ctx* c);
+int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub);
+int fl_copy_b(swap_ctx* c, const double* src, double* dst);
+int fl_zero(swap_ctx* c, void* p, size_t bytes);
+
+// ---- particle launchers (k_particles.hip) ----
+int pl_weights(swap_ctx* c);
+int pl_deposit_simple(swap_ctx* c, int mode);  // 0: np, 1: up(vp) 2: up(ext) 3: up(vp1)
+int pl_get_Ep(swap_ctx* c);
+int pl_vplus_vminus(swap_ctx* c);
+int pl_improve_vp1(swap_ctx* c);
+int pl_vp_final(swap_ctx* c);
+int pl_move_half(swap_ctx* c);
+int pl_pass(swap_ctx* c, int mode);  // fused passes 1, 2, 3
+int pl_load_abs(swap_ctx* c, int64_t n, const double* xp_d, const double* v_d,
+                const double* v1_d);
+int pl_export_abs(swap_ctx* c, double* xp_d);
+int pl_maxwellian_full(swap_ctx* c, int64_t off, int64_t n, double vsw, double vth, float bpv,
+                       float mrv, uint32_t stream);
+int pl_box_reset(swap_ctx* c);
+
+// ---- sort (k_sort.hip) ----
+int so_sort(swap_ctx* c);
+int so_trivial_items(swap_ctx* c);
+
+// ---- sources / diagnostics (k_sources.hip) ----
+int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
+                 int64_t* injected);
+int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* injected,
+                    int64_t* chex);
+int sr_diagnostics(swap_ctx* c, swap_diag* d);
+int sr_outflow_compact(swap_ctx* c);
+int sr_vmax(swap_ctx* c, double vmax[4]);
+
+// helpers
+template <typename R>
+PDev<R> pdev(swap_ctx* c) {
+  PDev<R> d;
+  d.cell = c->cell;
+  R** q = (R**)c->p;
+  d.fx = q[0]; d.fy = q[1]; d.fz = q[2];
+  d.vx = q[3]; d.vy = q[4]; d.vz = q[5];
+  d.v1x = q[6]; d.v1y = q[7]; d.v1z = q[8];
+  d.bp = c->bp;
+  d.mr = c->mr;
+  return d;
+}
+template <typename R>
+PCoef<R> pcoef(swap_ctx* c) {
+  PCoef<R> k;
+  k.dt = (R)c->cfg.dt;
+  k.s[0] = (R)(0.5 * c->cfg.dt / c->cfg.dx);
+  k.s[1] = (R)(0.5 * c->cfg.dt / c->cfg.dy);
+  k.s[2] = (R)(0.5 * c->cfg.dt / c->cfg.dz);
+  k.nc[0] = c->cfg.ncx; k.nc[1] = c->cfg.ncy; k.nc[2] = c->cfg.ncz;
+  k.outflow = (c->cfg.flags & SWAP_OUTFLOW_X) ? 1 : 0;
+  return k;
+}
+inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
