@@ -284,15 +284,15 @@ __global__ void k_update_src(Geo g, FC f, const R* up, const R* ajc, R* upc, R* 
 // np = deposit sum / (vol beta) at every node (update_np's scaling,
 // gutsp.f:1480-1491; the fold of :1496-1498 happened at deposit time)
 template <typename R>
-__global__ void k_finalize_np(Geo g, const R* hi, const R* lo, R qn, R* np) {
+__global__ void k_finalize_np(Geo g, const typename AccOf<R>::T* a, R qn, R* np) {
   GRID_LOOP_HEAD
-  np[o] = acc_read<R>(hi, lo, gidx(g, ci, cj, ck)) * qn;
+  np[o] = (R)acc_read(a, gidx(g, ci, cj, ck)) * qn;
 }
 
 // update_up's division by np and node->edge average (gutsp.f:1704-1756,
-// 1825-1836); hi/lo point at moment 1 (x velocity), moments are nn apart
+// 1825-1836); a points at moment 1 (x velocity), moments are nn apart
 template <typename R>
-__global__ void k_finalize_up(Geo g, const R* hi, const R* lo, const R* np, R qn, R* up) {
+__global__ void k_finalize_up(Geo g, const typename AccOf<R>::T* a, const R* np, R qn, R* up) {
   GRID_LOOP_HEAD
   int64_t q0 = gidx(g, ci, cj, ck);
   int64_t qn1[3] = {gidx(g, canon(ci + 1, g.nx), cj, ck), gidx(g, ci, canon(cj + 1, g.ny), ck),
@@ -301,8 +301,8 @@ __global__ void k_finalize_up(Geo g, const R* hi, const R* lo, const R* np, R qn
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     R n1 = np[qn1[m]];
-    R c0 = n0 != (R)0 ? acc_read<R>(hi, lo, q0 + m * g.nn) * qn / n0 : (R)0;
-    R c1 = n1 != (R)0 ? acc_read<R>(hi, lo, qn1[m] + m * g.nn) * qn / n1 : (R)0;
+    R c0 = n0 != (R)0 ? (R)acc_read(a, q0 + m * g.nn) * qn / n0 : (R)0;
+    R c1 = n1 != (R)0 ? (R)acc_read(a, qn1[m] + m * g.nn) * qn / n1 : (R)0;
     up[o + m * g.nn] = (R)0.5 * (c0 + c1);
   }
 }
@@ -471,14 +471,14 @@ int fl_update_src(swap_ctx* c, const void* up, const void* ajc, void* S) {
   return post(c);
 }
 int fl_finalize_np(swap_ctx* c) {
-  DISPATCH(c, LAUNCH((k_finalize_np<R>), c->g, (const R*)c->mhi, (const R*)c->mlo, (R)c->qn,
-                     F<R>(c, SWAP_F_NP)));
+  DISPATCH(c, using A = typename AccOf<R>::T;
+           LAUNCH((k_finalize_np<R>), c->g, (const A*)c->macc, (R)c->qn, F<R>(c, SWAP_F_NP)));
   return post(c);
 }
 int fl_finalize_up(swap_ctx* c) {
-  DISPATCH(c, LAUNCH((k_finalize_up<R>), c->g, (const R*)c->mhi + c->g.nn,
-                     (const R*)c->mlo + c->g.nn, F<R>(c, SWAP_F_NP), (R)c->qn,
-                     F<R>(c, SWAP_F_UP)));
+  DISPATCH(c, using A = typename AccOf<R>::T;
+           LAUNCH((k_finalize_up<R>), c->g, (const A*)c->macc + c->g.nn, F<R>(c, SWAP_F_NP),
+                  (R)c->qn, F<R>(c, SWAP_F_UP)));
   return post(c);
 }
 int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub) {

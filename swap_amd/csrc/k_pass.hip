@@ -1,0 +1,370 @@
+// k_pass.hip -- the three fused particle passes of swap_step.
+//
+//   P1: moments at x^n of r and r(1.5 v - 0.5 v1)             (maind.f:385-399)
+//   P2: Ep, v+-, deposit r (v+ + v-)/2 at x^n                   (maind.f:400-402)
+//   P3: Ep, v+-, v1 <- v, v <- v+ + dt/2 Ep, half move to x^{n+1/2}, deposit
+//       r and r v there, second half move to x^{n+1}            (maind.f:404-413,479)
+// with r = 1/beta_p.  The division of update_up by np(node)
+// (gutsp.f:1704-1756) is done once per node after the sums.
+//
+// One workgroup runs one work item: a run of one tile's particles after the
+// counting sort.  Lane l of a wavefront takes particle base + l (coalesced
+// loads).  Particles of one cell sit on neighbouring lanes, so the deposit is
+// reduced across the wavefront first: every one of the 8 corners x M moments
+// goes through one segmented inclusive scan over the lanes (DPP row shifts
+// and row broadcasts, segments = runs of lanes with the same cell), and only
+// the last lane of each run adds its run's sums into the LDS window.  The
+// window covers the item's deposit footprint and accumulates compensated
+// FP32 (hi, lo) pairs: the returning LDS add gives the exact rounding error
+// of each addition (TwoSum), which goes into lo.  The window is added to HBM
+// once per item, again compensated.  The window of P1/P2 is the item's cell
+// box at x^n; P3's is that box widened by the half-step displacement range
+// from signed per-axis velocity bounds (reduced by the previous P3).
+// Particles outside the window deposit straight to HBM (counted as misses).
+#include "particle_math.h"
+
+using namespace pm;
+
+namespace {
+
+template <int MODE>
+struct NMom {
+  static constexpr int value = MODE == 2 ? 3 : 4;
+};
+
+// nearest periodic image of cell c to a box centre
+__device__ __forceinline__ int unwrap(int c, int centre, int nc) {
+  int d = c - centre;
+  if (2 * d > nc) c -= nc;
+  else if (2 * d < -nc) c += nc;
+  return c;
+}
+
+// window accumulation: exact fixed point in the FP32 build, FP64 otherwise
+__device__ __forceinline__ void win_add(unsigned long long* a, int i, float v) {
+  atomicAdd(a + i, to_fix(v));
+}
+__device__ __forceinline__ void win_add(double* a, int i, double v) { atomicAdd(a + i, v); }
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// ---- segmented inclusive scan over the 64 lanes of a wavefront ----
+// Lane L belongs to the run starting at lane s(L).  Hillis-Steele inside each
+// row of 16 lanes (row_shr 1,2,4,8), then the row totals cross rows with
+// row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3); a contribution is
+// taken only from a lane of the same run.
+struct SegCond {
+  bool c1, c2, c4, c8, c15, c31;
+};
+__device__ __forceinline__ SegCond seg_cond(int L, int s) {
+  SegCond c;
+  c.c1 = L - 1 >= s;
+  c.c2 = L - 2 >= s;
+  c.c4 = L - 4 >= s;
+  c.c8 = L - 8 >= s;
+  c.c15 = ((L >> 4) & 1) && s <= (L & ~15) - 1;
+  c.c31 = (L >> 5) && s <= 31;
+  return c;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dppf(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xf, true));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dppf(double v) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xf, true);
+  int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ T segscan(T v, const SegCond& c) {
+  T t;
+  t = dppf<0x111, 0xf>(v); v = c.c1 ? v + t : v;   // row_shr:1
+  t = dppf<0x112, 0xf>(v); v = c.c2 ? v + t : v;   // row_shr:2
+  t = dppf<0x114, 0xf>(v); v = c.c4 ? v + t : v;   // row_shr:4
+  t = dppf<0x118, 0xf>(v); v = c.c8 ? v + t : v;   // row_shr:8
+  t = dppf<0x142, 0xa>(v); v = c.c15 ? v + t : v;  // row_bcast:15
+  t = dppf<0x143, 0xc>(v); v = c.c31 ? v + t : v;  // row_bcast:31
+  return v;
+}
+
+template <typename R, int MODE>
+__global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
+                                                  const R* __restrict__ S,
+                                                  typename AccOf<R>::T* __restrict__ gacc,
+                                                  const Item* __restrict__ items,
+                                                  const int* __restrict__ d_nitems,
+                                                  int64_t n_sorted, int64_t n_total,
+                                                  const int* __restrict__ boxes, int* boxes_next,
+                                                  const int* __restrict__ vb, int* vb_next,
+                                                  unsigned long long* dcnt) {
+  constexpr int M = NMom<MODE>::value;
+  using A = typename AccOf<R>::T;
+  __shared__ A lacc[WIN_VALUES];
+  __shared__ int sred[12];
+
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int nitems = *d_nitems;
+  const int nc[3] = {g.ncx, g.ncy, g.ncz};
+  int64_t i0, i1;
+  int bl[3] = {0, 0, 0}, bh[3] = {-1, -1, -1};
+  bool windowed = false;
+  const int w = blockIdx.x;
+  if (w < nitems) {
+    Item it = items[w];
+    i0 = it.start;
+    i1 = it.end;
+    const int* b = boxes + 6 * w;
+    for (int a = 0; a < 3; a++) {
+      bl[a] = b[a];
+      bh[a] = b[3 + a];
+      int cen = (bl[a] + bh[a]) >> 1;  // keep the box centre in [0, nc)
+      int sh = (cen >= 0 ? cen / nc[a] : -((-cen + nc[a] - 1) / nc[a])) * nc[a];
+      bl[a] -= sh;
+      bh[a] -= sh;
+    }
+    windowed = bh[0] >= bl[0] && bh[1] >= bl[1] && bh[2] >= bl[2];
+  } else {
+    int64_t t = w - nitems;
+    i0 = n_sorted + t * ITEM_MAX;
+    if (i0 >= n_total) return;
+    i1 = i0 + ITEM_MAX < n_total ? i0 + ITEM_MAX : n_total;
+  }
+  int lo_m[3] = {0, 0, 0}, hi_m[3] = {0, 0, 0};
+  if (MODE == 3 && windowed) {
+    // the cell shift of a half move by d in [dlo, dhi] cells lies in
+    // [floor(dlo), ceil(dhi)]; 10 % of the largest speed covers the change of
+    // v within one step (rotation by Omega dt and the E x B acceleration)
+    float vmax_abs = 0.f;
+    for (int a = 0; a < 3; a++)
+      vmax_abs = fmaxf(vmax_abs, fmaxf(-fval(vb[a]), fval(vb[3 + a])));
+    float sl = 0.1f * vmax_abs;
+    for (int a = 0; a < 3; a++) {
+      float dlo = (float)K.s[a] * (fval(vb[a]) - sl), dhi = (float)K.s[a] * (fval(vb[3 + a]) + sl);
+      if (!(dlo >= -64.f && dhi <= 64.f)) windowed = false;  // unset or absurd bounds
+      else {
+        lo_m[a] = (int)floorf(dlo);
+        hi_m[a] = (int)ceilf(dhi);
+      }
+    }
+  }
+  int u0[3], W[3], cen[3];
+  for (int a = 0; a < 3; a++) {
+    u0[a] = bl[a] + lo_m[a];
+    W[a] = bh[a] + hi_m[a] - u0[a] + 2;
+    cen[a] = (bl[a] + bh[a]) >> 1;
+  }
+  int64_t Wn64 = (int64_t)W[0] * W[1] * W[2];
+  if (windowed && (W[0] < 2 || W[1] < 2 || W[2] < 2 || Wn64 * M > WIN_VALUES)) windowed = false;
+  const int Wn = windowed ? (int)Wn64 : 0;
+  if (windowed)
+    for (int q = tid; q < Wn * M; q += PASS_WG) lacc[q] = (A)0;
+  if (MODE == 3 && tid < 12) sred[tid] = (tid % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;
+  __syncthreads();
+
+  const int coff[8] = {0, 1, W[0] * W[1], 1 + W[0] * W[1], W[0], 1 + W[0],
+                       W[0] + W[0] * W[1], 1 + W[0] + W[0] * W[1]};
+  int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int bmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
+  float vlo[3] = {3e38f, 3e38f, 3e38f}, vhi[3] = {-3e38f, -3e38f, -3e38f};
+  unsigned long long misses = 0;
+
+  // The trip count is uniform across the workgroup (the segmented scan needs
+  // every lane of a wavefront).
+  for (int64_t base = i0; base < i1; base += PASS_WG) {
+    const int64_t p = base + tid;
+    const bool valid = p < i1;
+    uint32_t c = valid ? P.cell[p] : 0u;
+    R w8[8], q[M];
+    int ux = 0, uy = 0, uz = 0;
+    R f0 = 0, f1 = 0, f2 = 0, vn[3] = {0, 0, 0};
+    if (valid) {
+      const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
+      f0 = P.fx[p]; f1 = P.fy[p]; f2 = P.fz[p];
+      const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
+      weights8(f0, f1, f2, w8);
+      ux = unwrap(ci, cen[0], g.ncx);
+      uy = unwrap(cj, cen[1], g.ncy);
+      uz = unwrap(ck, cen[2], g.ncz);
+      const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
+      if constexpr (MODE == 1) {
+        q[0] = r;
+        q[1] = r * ((R)1.5 * v[0] - (R)0.5 * P.v1x[p]);
+        q[2] = r * ((R)1.5 * v[1] - (R)0.5 * P.v1y[p]);
+        q[3] = r * ((R)1.5 * v[2] - (R)0.5 * P.v1z[p]);
+      } else {
+        R A[3], B[3], E[3], vm[3], vpl[3];
+        gather6(g, S, gidx(g, ci, cj, ck), w8, A, B);
+        const R mr = P.mr ? (R)P.mr[p] : (R)1;
+        ep_of(A, B, mr, E);
+        rotate(K.dt, v, E, B, mr, vm, vpl);
+        if constexpr (MODE == 2) {
+#pragma unroll
+          for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
+        } else {
+          const R hdt = (R)0.5 * K.dt;
+#pragma unroll
+          for (int m = 0; m < 3; m++) {
+            vn[m] = vpl[m] + hdt * E[m];
+            vlo[m] = fminf(vlo[m], (float)vn[m]);
+            vhi[m] = fmaxf(vhi[m], (float)vn[m]);
+          }
+          P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
+          P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
+          ux += half_move(f0, vn[0], K.s[0]);
+          uy += half_move(f1, vn[1], K.s[1]);
+          uz += half_move(f2, vn[2], K.s[2]);
+          weights8(f0, f1, f2, w8);
+          q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
+        }
+      }
+    }
+    // deposit key: the window cell, or a distinct negative number
+    const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
+    const bool inw = valid && windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
+                     ly < W[1] - 1 && lz < W[2] - 1;
+    const int key = inw ? lx + W[0] * (ly + W[1] * lz) : -1 - lane;
+    if (valid && !inw) {
+      misses++;
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++) {
+        int64_t node = gidx(g, node_of(ux + (cc & 1), g.ncx),
+                            node_of(uy + ((cc >> 2) & 1), g.ncy),
+                            node_of(uz + ((cc >> 1) & 1), g.ncz));
+#pragma unroll
+        for (int m = 0; m < M; m++) acc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
+      }
+    }
+    // runs of equal keys on neighbouring lanes; the last lane of a run adds
+    // the run's segmented sums into the window
+    const int prevkey = __shfl_up(key, 1);
+    const unsigned long long hm = __ballot(lane == 0 || prevkey != key);
+    const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+    const int s = 63 - __clzll(hm & le);
+    const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
+    if (__ballot(inw)) {
+      const SegCond sc = seg_cond(lane, s);
+      R sv[8][M];
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+        for (int m = 0; m < M; m++) sv[cc][m] = segscan(inw ? w8[cc] * q[m] : (R)0, sc);
+      if (tail && inw) {
+#pragma unroll
+        for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+          for (int m = 0; m < M; m++) win_add(lacc, key + coff[cc] + m * Wn, sv[cc][m]);
+      }
+    }
+    if constexpr (MODE == 3) {
+      if (valid) {
+        ux += half_move(f0, vn[0], K.s[0]);
+        uy += half_move(f1, vn[1], K.s[1]);
+        uz += half_move(f2, vn[2], K.s[2]);
+        P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
+        P.cell[p] = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
+        bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
+        bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
+        bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+      }
+    }
+  }
+  if (misses) atomicAdd(dcnt, misses);
+  if (MODE == 3) {
+    // the next step's windows: cell box of x^{n+1} and signed velocity bounds
+    for (int a = 0; a < 3; a++) {
+      int m0 = wave_min(bmin[a]), m1 = wave_max(bmax[a]);
+      int k0 = wave_min(fkey(vlo[a])), k1 = wave_max(fkey(vhi[a]));
+      if (lane == 0) {
+        atomicMin(&sred[a], m0);
+        atomicMax(&sred[3 + a], m1);
+        atomicMin(&sred[6 + a], k0);
+        atomicMax(&sred[9 + a], k1);
+      }
+    }
+  }
+  __syncthreads();
+  if (MODE == 3 && tid < 6) {
+    if (w < nitems) {
+      if (tid < 3) atomicMin(boxes_next + 6 * w + tid, sred[tid]);
+      else atomicMax(boxes_next + 6 * w + tid, sred[tid]);
+    }
+    if (tid < 3) atomicMin(vb_next + tid, sred[6 + tid]);
+    else atomicMax(vb_next + tid, sred[6 + tid]);
+  }
+  if (windowed) {
+    for (int qn = tid; qn < Wn; qn += PASS_WG) {
+      int lx = qn % W[0], ly = (qn / W[0]) % W[1], lz = qn / (W[0] * W[1]);
+      int64_t node = gidx(g, node_of(u0[0] + lx, g.ncx), node_of(u0[1] + ly, g.ncy),
+                          node_of(u0[2] + lz, g.ncz));
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        A v = lacc[qn + m * Wn];
+        if (v != (A)0) atomicAdd(gacc + node + (int64_t)m * g.nn, v);
+      }
+    }
+  }
+}
+
+__global__ void k_pass_reset(int64_t nitems, int* boxes_next, int* vb_next) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < 6) vb_next[q] = q < 3 ? 0x7fffffff : -0x7fffffff - 1;
+  if (q >= nitems * 6) return;
+  boxes_next[q] = (q % 6) < 3 ? 0x7fffffff : -0x7fffffff;
+}
+
+int post(swap_ctx* c) {
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+// reset the box/velocity-bound outputs of P3
+int pl_box_reset(swap_ctx* c) {
+  int64_t n6 = std::max<int64_t>(c->max_items * 6, 6);
+  hipLaunchKernelGGL(k_pass_reset, dim3(nblk(n6, 256)), dim3(256), 0, c->stream, c->max_items,
+                     c->boxes_next, c->vb_next);
+  return post(c);
+}
+
+int pl_pass(swap_ctx* c, int mode) {
+  if (c->n == 0) return 0;
+  int64_t ns = c->n_sorted < 0 ? 0 : c->n_sorted;
+  int64_t tail = c->n - ns;
+  int64_t nb = (c->n_sorted < 0 ? 0 : c->max_items) + (tail + ITEM_MAX - 1) / ITEM_MAX;
+  if (nb == 0) return 0;
+  dim3 grid((unsigned)nb), blk(PASS_WG);
+  const int* dn = c->n_sorted < 0 ? c->d_zero : c->d_nitems;
+  auto launch = [&](auto R0) {
+    using R = decltype(R0);
+    PDev<R> P = pdev<R>(c);
+    PCoef<R> K = pcoef<R>(c);
+    const R* S = (const R*)c->S;
+    using A = typename AccOf<R>::T;
+    A* acc = (A*)c->macc;
+    if (mode == 1)
+      hipLaunchKernelGGL((k_pass<R, 1>), grid, blk, 0, c->stream, c->g, P, K, S, acc, c->items,
+                         dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next, c->dcnt);
+    else if (mode == 2)
+      hipLaunchKernelGGL((k_pass<R, 2>), grid, blk, 0, c->stream, c->g, P, K, S, acc + c->g.nn,
+                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
+                         c->dcnt);
+    else
+      hipLaunchKernelGGL((k_pass<R, 3>), grid, blk, 0, c->stream, c->g, P, K, S, acc, c->items,
+                         dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next, c->dcnt);
+  };
+  if (c->prec == 64) launch(0.0);
+  else launch(0.0f);
+  return post(c);
+}

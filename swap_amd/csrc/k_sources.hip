@@ -286,40 +286,51 @@ int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* inject
 }
 
 namespace {
-__device__ __forceinline__ float wave_maxf(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+// order-preserving int image of a float, for integer atomic min/max
+__device__ __forceinline__ int fkey_s(float f) {
+  int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ int wave_mini(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
   return v;
 }
+__device__ __forceinline__ int wave_maxi(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+// signed per-axis velocity bounds: out[0..2] min keys, out[3..5] max keys
 template <typename R>
-__global__ void k_vmax(int64_t n, PDev<R> P, float* out) {
-  float m[4] = {0, 0, 0, 0};
+__global__ void k_vbounds(int64_t n, PDev<R> P, int* out) {
+  int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int hi[3] = {-0x7fffffff - 1, -0x7fffffff - 1, -0x7fffffff - 1};
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * blockDim.x) {
-    float vx = fabsf((float)P.vx[p]), vy = fabsf((float)P.vy[p]), vz = fabsf((float)P.vz[p]);
-    m[0] = fmaxf(m[0], vx);
-    m[1] = fmaxf(m[1], vy);
-    m[2] = fmaxf(m[2], vz);
-    m[3] = fmaxf(m[3], sqrtf(vx * vx + vy * vy + vz * vz));
+    int k[3] = {fkey_s((float)P.vx[p]), fkey_s((float)P.vy[p]), fkey_s((float)P.vz[p])};
+    for (int a = 0; a < 3; a++) {
+      lo[a] = min(lo[a], k[a]);
+      hi[a] = max(hi[a], k[a]);
+    }
   }
-  for (int q = 0; q < 4; q++) {
-    float t = wave_maxf(m[q]);
-    if (__lane_id() == 0) atomicMax((unsigned int*)out + q, __float_as_uint(t));
+  for (int a = 0; a < 3; a++) {
+    int l = wave_mini(lo[a]), h = wave_maxi(hi[a]);
+    if (__lane_id() == 0) {
+      atomicMin(out + a, l);
+      atomicMax(out + 3 + a, h);
+    }
   }
+}
+__global__ void k_vbounds_init(int* out) {
+  if (threadIdx.x < 6) out[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : -0x7fffffff - 1;
 }
 }  // namespace
 
-int sr_vmax(swap_ctx* c, double vmax[4]) {
-  float* d = (float*)c->dmin;
-  HIPCHK(c, hipMemsetAsync(d, 0, 4 * sizeof(float), c->stream));
+int sr_vbounds(swap_ctx* c) {
+  hipLaunchKernelGGL(k_vbounds_init, dim3(1), dim3(64), 0, c->stream, c->vb);
   if (c->n > 0)
-    SDISPATCH(c, hipLaunchKernelGGL((k_vmax<R>), dim3(512), dim3(256), 0, c->stream, c->n,
-                                    pdev<R>(c), d));
-  if (int e = post(c)) return e;
-  float h[4];
-  HIPCHK(c, hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int q = 0; q < 4; q++) vmax[q] = h[q];
-  return 0;
+    SDISPATCH(c, hipLaunchKernelGGL((k_vbounds<R>), dim3(1024), dim3(256), 0, c->stream, c->n,
+                                    pdev<R>(c), c->vb));
+  return post(c);
 }
 
 int sr_outflow_compact(swap_ctx* c) {

@@ -95,9 +95,8 @@ int host_to_real(swap_ctx* c, void* d, size_t n, const double* hd, const float* 
 }
 
 int zero_moments(swap_ctx* c, int m0, int m1) {
-  size_t off = c->rsz * (size_t)c->g.nn * m0, len = c->rsz * (size_t)c->g.nn * (m1 - m0);
-  HIPCHK(c, hipMemsetAsync((char*)c->mhi + off, 0, len, c->stream));
-  HIPCHK(c, hipMemsetAsync((char*)c->mlo + off, 0, len, c->stream));
+  size_t off = 8 * (size_t)c->g.nn * m0, len = 8 * (size_t)c->g.nn * (m1 - m0);
+  HIPCHK(c, hipMemsetAsync((char*)c->macc + off, 0, len, c->stream));
   return 0;
 }
 
@@ -105,19 +104,6 @@ int zero_moments(swap_ctx* c, int m0, int m1) {
   do {                          \
     if (int e_ = (x)) return e_; \
   } while (0)
-
-// window margins for the x^{n+1/2} deposit: half-step displacement bound
-// from the current speeds (+10% of |v| for the rotation within the step)
-int update_margins(swap_ctx* c) {
-  double vm[4];
-  TRY(sr_vmax(c, vm));
-  const double d[3] = {c->cfg.dx, c->cfg.dy, c->cfg.dz};
-  for (int a = 0; a < 3; a++) {
-    double v = vm[a] + 0.1 * vm[3];
-    c->margin[a] = (int)std::ceil(0.5 * c->cfg.dt * v / d[a]) + 1;
-  }
-  return 0;
-}
 
 // stage timing: record an event on the context stream at a stage boundary
 struct Stage {
@@ -244,8 +230,7 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   DMF(c->ct, rs * 3 * nn);
   DMF(c->btc0, rs * 3 * nn);
   DMF(c->S, rs * 6 * nn);
-  DMF(c->mhi, rs * 4 * nn);
-  DMF(c->mlo, rs * 4 * nn);
+  DMF(c->macc, 8 * 4 * nn);
   c->scratch_bytes = std::max<size_t>(8 * nn, 1 << 16);
   DMF(c->scratch, c->scratch_bytes);
   DMF(c->dred, 64 * sizeof(double));
@@ -277,11 +262,16 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   c->ntile[2] = (g.ncz + TILE_Z - 1) / TILE_Z;
   c->ntiles = (int64_t)c->ntile[0] * c->ntile[1] * c->ntile[2];
   c->nkeys = c->ntiles * TILE_CELLS;
-  DMF(c->skey, 4 * cp);
-  DMF(c->sperm, 4 * cp);
+  DMF(c->sruns, 8 * cp);
+  DMF(c->srbase, 4 * cp);
   DMF(c->scnt, 4 * (size_t)(c->nkeys + 1));
   DMF(c->sstart, 4 * (size_t)(c->nkeys + 1));
   DMF(c->scur, 4 * (size_t)(c->nkeys + 1));
+  DMF(c->snrun, 4 * (size_t)(c->nkeys + 1));
+  DMF(c->srstart, 4 * (size_t)(c->nkeys + 1));
+  DMF(c->vb, 16 * sizeof(int));
+  c->vb_next = c->vb + 8;
+  DMF(c->d_zero, 4 * sizeof(int));
   DMF(c->sblk, 4 * (size_t)(c->nkeys / 4096 + 4));
   c->max_items = c->ntiles + c->cap / ITEM_MAX + 1;
   DMF(c->items, sizeof(Item) * (size_t)c->max_items);
@@ -299,11 +289,11 @@ int swap_destroy(swap_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   prof_clear(c);
-  void* ps[] = {c->b1, c->b12, c->b1p2, c->a, c->ct, c->btc0, c->cE, c->S, c->mhi, c->mlo,
+  void* ps[] = {c->b1, c->b12, c->b1p2, c->a, c->ct, c->btc0, c->cE, c->S, c->macc,
                 c->scratch, c->dred, c->dmin, c->dcnt, c->d_nitems, c->cell, c->cell2, c->bp,
                 c->bp2, c->mr, c->mr2, c->id, c->id2, c->Ep, c->vpl, c->vmi, c->wght, c->ijkp,
-                c->skey, c->sperm, c->scnt, c->sstart, c->scur, c->sblk, c->items, c->boxes,
-                c->boxes_next};
+                c->sruns, c->srbase, c->scnt, c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items,
+                c->boxes, c->boxes_next, c->vb, c->d_zero};
   for (void* p : ps) dfree(p);
   for (int f = 0; f < SWAP_F_COUNT; f++) dfree(c->fld[f]);
   for (int a = 0; a < 9; a++) {
@@ -367,6 +357,7 @@ int swap_load_particles(swap_ctx* c, int64_t n, const double* xp, const double* 
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (dtmp) HIPCHK(c, hipFree(dtmp));
   c->n_sorted = -1;
+  c->vb_valid = false;
   return e;
 }
 
@@ -412,6 +403,7 @@ int swap_load_particles_split(swap_ctx* c, int64_t n, const uint32_t* cell, cons
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_sorted = -1;
+  c->vb_valid = false;
   return 0;
 }
 
@@ -425,6 +417,7 @@ int swap_load_maxwellian(swap_ctx* c, int64_t n, double vsw, double vth, double 
   TRY(pl_maxwellian_full(c, 0, n, vsw, vth, (float)beta_p, (float)mrat, 1u));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_sorted = -1;
+  c->vb_valid = false;
   return 0;
 }
 
@@ -703,7 +696,11 @@ int swap_field_subcycles(swap_ctx* c, int ntf, double dtsub) {
 int swap_sort(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
   TRY(so_sort(c));
-  return update_margins(c);
+  if (!c->vb_valid) {
+    TRY(sr_vbounds(c));
+    c->vb_valid = true;
+  }
+  return 0;
 }
 
 int swap_prime(swap_ctx* c) {
@@ -751,7 +748,7 @@ int swap_step(swap_ctx* c, int nsteps) {
       TRY(fl_finalize_up(c));
       TRY(fl_update_src(c, c->fld[SWAP_F_UP], c->fld[SWAP_F_AJC], c->S));
       TRY(zero_moments(c, 0, 4));
-      if (c->n_sorted >= 0) TRY(pl_box_reset(c));
+      TRY(pl_box_reset(c));
     }
     // P3: final velocity, both half moves, moments at x^{n+1/2} (maind.f:404-413, 479)
     {
@@ -763,6 +760,9 @@ int swap_step(swap_ctx* c, int nsteps) {
       TRY(fl_finalize_np(c));
       TRY(fl_finalize_up(c));
     }
+    // P3 reduced the cell boxes of x^{n+1} and the bounds of v^{n+1/2}
+    std::swap(c->vb, c->vb_next);
+    c->vb_valid = c->n > 0;
     if (c->n_sorted >= 0) {
       std::swap(c->boxes, c->boxes_next);
       c->steps_since_sort++;

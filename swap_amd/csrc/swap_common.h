@@ -60,33 +60,37 @@ SWAP_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 // uniform in [0,1) with 24 random bits (exact in float and double)
 SWAP_HD double u24(uint32_t r) { return (double)(r >> 8) * (1.0 / 16777216.0); }
 
-// ---- compensated accumulation ----
-// FP32 moments are accumulated as (hi, lo) pairs: the atomic add that
-// returns the old value lets the adding thread recover the exact rounding
-// error of its own addition (Knuth's TwoSum) and add it into lo.
-__device__ __forceinline__ float two_sum_err(float a, float b, float s) {
-  float bb = s - a;
-  return (a - (s - bb)) + (b - bb);
-}
-__device__ __forceinline__ void acc_add(float* hi, float* lo, int64_t i, float v) {
-  if (v == 0.f) return;
-  float old = atomicAdd(hi + i, v);
-  float e = two_sum_err(old, v, old + v);
-  if (e != 0.f) atomicAdd(lo + i, e);
-}
-__device__ __forceinline__ void acc_add(double* hi, double* lo, int64_t i, double v) {
-  (void)lo;
-  if (v == 0.0) return;
-  atomicAdd(hi + i, v);
-}
+// ---- moment accumulation ----
+// FP32 deposits are summed exactly: each FP32 contribution (a particle's, or
+// a run's FP32 partial sum) is converted to a 64-bit integer count of
+// 2^-FIX_BITS units and added with integer atomics.  Integer addition is
+// exact and associative, so the per-node sums carry no accumulation error and
+// do not depend on the order in which atomics are served; the only rounding
+// is the FP32 -> fixed-point conversion of each contribution (2^-31
+// absolute, far below FP32's relative precision for the magnitudes here:
+// sums stay below 2^63 / 2^30 = 8.6e9).  The FP64 build accumulates doubles.
+constexpr int FIX_BITS = 30;
 template <typename R>
-__device__ __forceinline__ R acc_read(const R* hi, const R* lo, int64_t i);
+struct AccOf {
+  using T = unsigned long long;  // FP32 build: fixed point
+};
 template <>
-__device__ __forceinline__ float acc_read<float>(const float* hi, const float* lo, int64_t i) {
-  return hi[i] + lo[i];
+struct AccOf<double> {
+  using T = double;
+};
+__device__ __forceinline__ unsigned long long to_fix(float v) {
+  return (unsigned long long)(long long)rintf(v * (float)(1 << FIX_BITS));
 }
-template <>
-__device__ __forceinline__ double acc_read<double>(const double* hi, const double* lo, int64_t i) {
-  (void)lo;
-  return hi[i];
+__device__ __forceinline__ float from_fix(unsigned long long a) {
+  return (float)(long long)a * (1.0f / (float)(1 << FIX_BITS));
 }
+__device__ __forceinline__ void acc_add(unsigned long long* a, int64_t i, float v) {
+  if (v != 0.f) atomicAdd(a + i, to_fix(v));
+}
+__device__ __forceinline__ void acc_add(double* a, int64_t i, double v) {
+  if (v != 0.0) atomicAdd(a + i, v);
+}
+__device__ __forceinline__ float acc_read(const unsigned long long* a, int64_t i) {
+  return from_fix(a[i]);
+}
+__device__ __forceinline__ double acc_read(const double* a, int64_t i) { return a[i]; }

@@ -9,7 +9,7 @@
 // tile); one work item is a run of at most ITEM_MAX particles of one tile
 constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 4;
 constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
-constexpr int ITEM_MAX = 16384;
+constexpr int ITEM_MAX = 32768;
 constexpr int PASS_WG = 512;   // threads per particle-pass workgroup
 constexpr int PASS_KP = 8;     // consecutive particles per lane
 constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values
@@ -49,7 +49,7 @@ struct swap_ctx {
   void* fld[SWAP_F_COUNT] = {};  // real fields (B ids unused here)
   void *a = nullptr, *ct = nullptr, *btc0 = nullptr, *cE = nullptr;  // scratch
   void* S = nullptr;   // gather source: 6 components (aj-up at nodes, btc)
-  void *mhi = nullptr, *mlo = nullptr;  // moment accumulators [4][nn]
+  void* macc = nullptr;  // moment accumulators [4][nn], AccOf<R>::T (8 bytes)
   double dtsub = 0;
   int ntf = 0;
   // particles
@@ -64,8 +64,13 @@ struct swap_ctx {
   // sort and work items
   int ntile[3] = {0, 0, 0};
   int64_t ntiles = 0, nkeys = 0;
-  uint32_t *skey = nullptr, *scnt = nullptr, *sstart = nullptr, *scur = nullptr,
-           *sperm = nullptr, *sblk = nullptr;
+  uint32_t *scnt = nullptr, *sstart = nullptr, *scur = nullptr, *snrun = nullptr,
+           *srstart = nullptr, *sblk = nullptr;
+  uint2* sruns = nullptr;     // run records of the sort, one per run (<= n)
+  uint32_t* srbase = nullptr; // destination of the run headed at each index
+  int *vb = nullptr, *vb_next = nullptr;  // signed velocity bounds, float keys
+  bool vb_valid = false;
+  int* d_zero = nullptr;
   Item* items = nullptr;
   int* boxes = nullptr;       // [max_items][6]: cell box lo[3], hi[3]
   int* boxes_next = nullptr;
@@ -73,7 +78,6 @@ struct swap_ctx {
   int64_t max_items = 0;
   int64_t n_sorted = -1;      // particles covered by the item list (-1: none)
   int steps_since_sort = 0;
-  int margin[3] = {2, 1, 1};  // half-step displacement bound, cells
   // counters
   unsigned long long* dcnt = nullptr;  // [0] window misses [1] escapes [2] scratch
   int64_t injected = 0, escaped = 0;
@@ -137,7 +141,6 @@ int pl_box_reset(swap_ctx* c);
 
 // ---- sort (k_sort.hip) ----
 int so_sort(swap_ctx* c);
-int so_trivial_items(swap_ctx* c);
 
 // ---- sources / diagnostics (k_sources.hip) ----
 int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
@@ -146,7 +149,7 @@ int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* inject
                     int64_t* chex);
 int sr_diagnostics(swap_ctx* c, swap_diag* d);
 int sr_outflow_compact(swap_ctx* c);
-int sr_vmax(swap_ctx* c, double vmax[4]);
+int sr_vbounds(swap_ctx* c);  // signed per-axis velocity bounds into c->vb
 
 // helpers
 template <typename R>
