@@ -115,7 +115,8 @@ class Oracle:
         return np.zeros((ncomp, n), self.dtype)
 
     def coord(self, name):
-        n = {"x": self.nx, "y": self.ny, "z": self.nz}[name[1] if name[0] == "q" else name[1]]
+        axis = name[0] if name.endswith("rat") else name[1]  # qx, dx_grid, dx_cell, xrat
+        n = {"x": self.nx, "y": self.ny, "z": self.nz}[axis]
         out = np.zeros(n, self.dtype)
         self.lib.or_get_coord(self.h, self.COORDS.index(name), _p(out))
         return out

@@ -1,0 +1,532 @@
+"""CPU pins of the oracle (oracle/swap_oracle.c) -- no GPU.
+
+Each test fixes a property that a plausible mistake in the restatement (a
+dropped term, a wrong sign, index or tie rule, a transposed operand) breaks:
+the b0.dat fixture, published random-number known answers, closed forms
+(uniform drift, exact rotation, whistler limit), discrete identities (div of
+the curl, curl of a gradient) and brute force on small inputs.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PAR = O.initparameters()
+NC = (8, 6, 4)  # small grid: cells per axis
+
+
+def make(width=64, nc=NC, npart=None, ntsub=10):
+    n_p = npart or 8 * nc[0] * nc[1] * nc[2]
+    vol = nc[0] * nc[1] * nc[2] * PAR["dx"] ** 3
+    beta = n_p / (vol * 0.1e15)
+    o = O.Oracle(nc[0] + 2, nc[1] + 2, nc[2] + 2, PAR["dx"], PAR["dx"], PAR["dx"], PAR["dt"],
+                 ntsub, PAR["alpha"], beta, width=width)
+    return o
+
+
+def positions(o, n, rng):
+    """uniform positions in the periodic domain (q(1), q(n-1)] per axis"""
+    qx, qy, qz = o.coord("qx"), o.coord("qy"), o.coord("qz")
+    xp = o.parts(n)
+    for a, q in enumerate((qx, qy, qz)):
+        xp[a] = q[0] + (q[-2] - q[0]) * (1.0 - rng.random(n))
+    return xp
+
+
+def weights(o, xp):
+    n = xp.shape[1]
+    ijkp = np.zeros((3, n), np.int32)
+    w = o.parts(n, 8)
+    o.get_interp_weights(n, n, xp, ijkp, w)
+    return ijkp, w
+
+
+# ---------------------------------------------------------------- constants
+def test_initparameters_physics():
+    """inputs.f:125-172: dx = lambda_i/2, Omega_i dt = 0.05, alpha = mu0 q^2/m (km units)"""
+    q, mp, eps0, c = 1.6e-19, 1.67e-27, 8.85e-12, 3e8
+    b0 = 0.2e-9
+    omega = q * b0 / mp
+    wpi = math.sqrt(0.1e15 / 1e9 * q * q / (eps0 * mp))
+    assert PAR["omega_p"] == pytest.approx(omega, rel=1e-12)
+    assert PAR["lambda_i"] == pytest.approx(c / wpi / 1e3, rel=1e-12)
+    assert PAR["dx"] == pytest.approx(360.41, abs=0.01)  # km
+    assert PAR["dt"] * omega == pytest.approx(0.05, rel=1e-12)
+    assert PAR["dtsub_init"] == pytest.approx(PAR["dt"] / 10, rel=1e-15)
+    assert PAR["nu_init"] == pytest.approx(0.002 * omega, rel=1e-12)
+    assert PAR["alpha"] == pytest.approx(3.14159 * 4e-7 / 1e3 * q * q / mp, rel=1e-12)
+
+
+def test_b0_matches_fixture():
+    """grd6_setup's B0 (initial.f:36-63) in FP32 equals b0.dat bit for bit"""
+    fx = json.load(open(os.path.join(HERE, "golden", "b0_fixture.json")))
+    o = make(width=32)
+    p32 = O.initparameters(width=32)
+    b0 = o.grid()
+    o.grd6_b0(0.2e-9, p32["mion"], b0)
+    for c in range(3):
+        (entry,) = fx["components"][c]  # one distinct value per component
+        want = np.uint32(int(entry["bits"], 16))
+        assert np.all(b0[c].view(np.uint32) == want), c
+    assert fx["nz"] == 401 and fx["record_bytes"] == [4, 3 * 3 * 401 * 3 * 4]
+
+
+def test_grid_coordinates():
+    """grd7 (initial.f:147-301) with zero stretch: q(i) = i*d, uniform cells"""
+    o = make()
+    for ax, n in (("x", o.nx), ("y", o.ny), ("z", o.nz)):
+        q = o.coord("q" + ax)
+        assert np.allclose(q, PAR["dx"] * np.arange(1, n + 1), rtol=1e-14)
+        assert np.allclose(o.coord("d%s_grid" % ax), PAR["dx"], rtol=1e-12)
+        assert np.allclose(o.coord("d%s_cell" % ax), PAR["dx"], rtol=1e-12)
+        assert np.allclose(o.coord("%srat" % ax), 0.5, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- boundary.f
+def test_periodic_ghosts():
+    o = make()
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(o.grid().shape)
+    core = b[:, 1:-1, 1:-1, 1:-1].copy()
+    o.periodic(b)
+    assert np.array_equal(b[:, 1:-1, 1:-1, 1:-1], core)  # interior untouched
+    # b(1) = b(n-1), b(n) = b(2) on every axis, corners included (x, y, z order)
+    for ax in (1, 2, 3):
+        assert np.array_equal(np.take(b, 0, ax), np.take(b, -2, ax))
+        assert np.array_equal(np.take(b, -1, ax), np.take(b, 1, ax))
+    s = rng.standard_normal(o.grid(1).shape)
+    o.periodic_scalar(s)
+    for ax in (0, 1, 2):
+        assert np.array_equal(np.take(s, 0, ax), np.take(s, -2, ax))
+        assert np.array_equal(np.take(s, -1, ax), np.take(s, 1, ax))
+
+
+# ---------------------------------------------------------------- weights
+def test_weights_partition_of_unity_and_linear_exactness():
+    o = make()
+    rng = np.random.default_rng(2)
+    n = 5000
+    xp = positions(o, n, rng)
+    ijkp, w = weights(o, xp)
+    assert np.all(w >= 0)
+    assert np.allclose(w.sum(0), 1.0, rtol=0, atol=1e-13)
+    qx, qy, qz = o.coord("qx"), o.coord("qy"), o.coord("qz")
+    i, j, k = ijkp - 1  # 0-based
+    assert np.all((qx[i] < xp[0]) & (xp[0] <= qx[i + 1]))
+    assert np.all((qz[k] < xp[2]) & (xp[2] <= qz[k + 1]))
+    assert np.all((qy[j] <= xp[1] * (1 + 1e-15)) & (xp[1] < qy[j + 1] * (1 + 1e-15)))
+    # a linear field sampled at the corners is reproduced at the particle
+    a = np.array([0.3, -1.7, 2.2, 0.9])
+
+    def f(x, y, z):
+        return a[0] + a[1] * x / 1e3 + a[2] * y / 1e3 + a[3] * z / 1e3
+    di = np.array([0, 1, 0, 1, 0, 1, 0, 1])
+    dj = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    dk = np.array([0, 0, 1, 1, 0, 0, 1, 1])
+    g = sum(w[c] * f(qx[i + di[c]], qy[j + dj[c]], qz[k + dk[c]]) for c in range(8))
+    assert np.allclose(g, f(*xp), rtol=1e-12)
+
+
+def test_weights_tie_rules():
+    """x and z cells are right-closed (q(i) < x <= q(i+1)); y uses floor(y/dy)"""
+    o = make()
+    qx, qy, qz = o.coord("qx"), o.coord("qy"), o.coord("qz")
+    xp = o.parts(1)
+    xp[:, 0] = (qx[4], qy[3], qz[2])  # exactly on nodes (1-based 5, 4, 3)
+    ijkp, w = weights(o, xp)
+    assert tuple(ijkp[:, 0]) == (4, 4, 2)
+    # x at the right end of cell 4, y at the left end of cell 4, z at the right
+    # end of cell 2: all weight on corner (i+1, j, k+1), the 4th of the
+    # reference's corner order (gutsp.f:1422-1429)
+    assert w[3, 0] == pytest.approx(1.0, abs=1e-14)
+    assert np.allclose(np.delete(w[:, 0], 3), 0.0, atol=1e-14)
+
+
+# ---------------------------------------------------------------- moments
+def load(o, n, rng, v=None):
+    xp = positions(o, n, rng)
+    vp = o.parts(n)
+    vp[:] = rng.standard_normal((3, n)) * 30 if v is None else np.asarray(v)[:, None]
+    return xp, vp
+
+
+def test_update_np_total_charge():
+    """sum over unique nodes of np * cell volume * beta = number of particles"""
+    n = 3000
+    o = make(npart=n)
+    rng = np.random.default_rng(3)
+    xp, _ = load(o, n, rng)
+    ijkp, w = weights(o, xp)
+    npg = o.grid(1)
+    bp = np.ones(n)
+    o.update_np(n, n, ijkp, w, bp, npg)
+    beta = n / (np.prod(NC) * PAR["dx"] ** 3 * 0.1e15)
+    total = npg[1:-1, 1:-1, 1:-1].sum() * PAR["dx"] ** 3 * beta
+    assert total == pytest.approx(n, rel=1e-12)
+    # ghosts follow periodic_scalar (gutsp.f:1509)
+    assert np.array_equal(npg[:, :, 0], npg[:, :, -2])
+    # a per-particle weight 1/beta_p scales the contribution
+    np2 = o.grid(1)
+    o.update_np(n, n, ijkp, w, np.full(n, 0.5), np2)
+    assert np.allclose(np2, 2 * npg, rtol=1e-13)
+
+
+def test_update_up_uniform_drift_and_linearity():
+    """all particles at velocity V give up = V on every edge (gutsp.f:1649-1842)"""
+    o = make()
+    rng = np.random.default_rng(4)
+    n = 3000
+    V = np.array([400.0, -12.0, 3.5])
+    xp, vp = load(o, n, rng, V)
+    ijkp, w = weights(o, xp)
+    bp = np.ones(n)
+    npg, up = o.grid(1), o.grid()
+    o.update_np(n, n, ijkp, w, bp, npg)
+    o.update_up(n, n, ijkp, w, bp, vp, npg, up)
+    for c in range(3):
+        assert np.allclose(up[c, 1:-1, 1:-1, 1:-1], V[c], rtol=1e-12, atol=1e-12)
+    # linear in v at fixed positions
+    va, vb = rng.standard_normal((3, n)), rng.standard_normal((3, n))
+    ups = []
+    for v in (va, vb, va + 2 * vb):
+        u = o.grid()
+        o.update_up(n, n, ijkp, w, bp, np.ascontiguousarray(v), npg, u)
+        ups.append(u)
+    assert np.allclose(ups[2], ups[0] + 2 * ups[1], rtol=1e-11, atol=1e-12)
+
+
+def test_extrapol_and_improve_up():
+    """extrapol_up deposits 1.5 vp - 0.5 vp1 (gutsp.f:127-150); improve_up sets
+    vp1 = (v+ + v-)/2 and deposits it (gutsp.f:407-428)"""
+    o = make()
+    rng = np.random.default_rng(5)
+    n = 2000
+    xp, vp = load(o, n, rng)
+    vp1 = rng.standard_normal((3, n)) * 30
+    ijkp, w = weights(o, xp)
+    bp = np.ones(n)
+    npg = o.grid(1)
+    o.update_np(n, n, ijkp, w, bp, npg)
+    u1, u2 = o.grid(), o.grid()
+    o.extrapol_up(n, n, ijkp, w, bp, vp, vp1, npg, u1)
+    o.update_up(n, n, ijkp, w, bp, np.ascontiguousarray(1.5 * vp - 0.5 * vp1), npg, u2)
+    assert np.allclose(u1, u2, rtol=1e-12, atol=1e-12)
+    vpl, vmi = rng.standard_normal((3, n)), rng.standard_normal((3, n))
+    v1 = o.parts(n)
+    u3, u4 = o.grid(), o.grid()
+    o.improve_up(n, n, ijkp, w, bp, v1, vpl, vmi, npg, u3)
+    assert np.allclose(v1, 0.5 * (vpl + vmi), rtol=1e-15)
+    o.update_up(n, n, ijkp, w, bp, np.ascontiguousarray(0.5 * (vpl + vmi)), npg, u4)
+    assert np.allclose(u3, u4, rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- push
+def uniform_field(o, vec):
+    f = o.grid()
+    for c in range(3):
+        f[c] = vec[c]
+    return f
+
+
+def test_get_Ep_uniform_cross_product():
+    """Ep = ((aj - up) x B) * mrat, exact for uniform fields (gutsp.f:253-271)"""
+    o = make()
+    rng = np.random.default_rng(6)
+    n = 500
+    xp, _ = load(o, n, rng)
+    ijkp, w = weights(o, xp)
+    V = np.array([400.0, 20.0, -5.0])
+    J = np.array([0.7, -0.2, 0.1])
+    B = np.array([0.003, -0.001, 0.019161675])
+    for mrat in (1.0, 1.0 / 28):
+        Ep = o.parts(n)
+        o.get_Ep(n, n, ijkp, w, np.full(n, mrat), uniform_field(o, J), uniform_field(o, V),
+                 uniform_field(o, B), Ep)
+        want = np.cross(J - V, B) * mrat
+        assert np.allclose(Ep.T, want, rtol=1e-12, atol=1e-15)
+
+
+def test_vplus_vminus_exact_rotation():
+    """Crank-Nicolson rotation: |v+| = |v-|, parallel part kept, gyro-phase
+    2 atan(|B| dt / 2) per step (gutsp.f:371-397); v- = v + dt/2 Ep"""
+    o = make()
+    rng = np.random.default_rng(7)
+    n = 400
+    xp, vp = load(o, n, rng)
+    ijkp, w = weights(o, xp)
+    B = np.array([0.004, -0.002, 0.019161675])
+    Ep = rng.standard_normal((3, n)) * 1e-2
+    vpl, vmi = o.parts(n), o.parts(n)
+    o.get_vplus_vminus(n, n, ijkp, w, np.ones(n), Ep, uniform_field(o, B), vp, vpl, vmi)
+    dt = PAR["dt"]
+    assert np.allclose(vmi, vp + 0.5 * dt * Ep, rtol=1e-14)
+    assert np.allclose(np.linalg.norm(vpl, axis=0), np.linalg.norm(vmi, axis=0), rtol=1e-13)
+    bh = B / np.linalg.norm(B)
+    assert np.allclose(bh @ vpl, bh @ vmi, rtol=1e-12, atol=1e-10)
+    perp_m = vmi - np.outer(bh, bh @ vmi)
+    perp_p = vpl - np.outer(bh, bh @ vpl)
+    cosang = np.sum(perp_m * perp_p, 0) / (np.linalg.norm(perp_m, axis=0) ** 2)
+    sinang = (bh @ np.cross(perp_m.T, perp_p.T).T) / (np.linalg.norm(perp_m, axis=0) ** 2)
+    ang = np.arctan2(sinang, cosang)
+    # v x B rotation of a positive charge turns v_perp clockwise about B
+    want = -2 * math.atan(np.linalg.norm(B) * dt / 2)
+    assert np.allclose(ang, want, rtol=1e-10)
+
+
+def test_vp_final_and_move_half():
+    """get_vp_final: vp1 <- vp, vp <- v+ + dt/2 Ep (gutsp.f:433-449);
+    move_ion_half: x += dt/2 v, one periodic wrap into (q(1), q(n-1)]"""
+    o = make()
+    rng = np.random.default_rng(8)
+    n = 1000
+    xp, vp = load(o, n, rng)
+    vp = vp * 200  # several cells per half step
+    vold = vp.copy()
+    Ep, vpl = rng.standard_normal((3, n)), rng.standard_normal((3, n))
+    vp1 = o.parts(n)
+    o.get_vp_final(n, n, Ep, vp, vp1, vpl)
+    assert np.array_equal(vp1, vold)
+    assert np.allclose(vp, vpl + 0.5 * PAR["dt"] * Ep, rtol=1e-15)
+    x0 = xp.copy()
+    o.move_ion_half(n, n, xp, vp)
+    for a, ax in enumerate("xyz"):
+        q = o.coord("q" + ax)
+        L = q[-2] - q[0]
+        d = xp[a] - (x0[a] + 0.5 * PAR["dt"] * vp[a])
+        assert np.allclose(np.round(d / L) * L, d, atol=1e-6)
+        assert np.all(np.abs(np.round(d / L)) <= 1)
+        assert np.all((xp[a] > q[0]) & (xp[a] <= q[-2]))
+
+
+# ---------------------------------------------------------------- fields
+def periodic_ext(core):
+    """extend a periodic core array (..., nz-2, ny-2, nx-2) by one ghost layer
+    per side: index 0 = n-2, index n-1 = 1 (boundary.f:18-46)"""
+    out = core
+    for ax in (-1, -2, -3):
+        out = np.concatenate([np.take(out, [-1], ax), out, np.take(out, [0], ax)], ax)
+    return np.ascontiguousarray(out)
+
+
+def test_curlE_is_divergence_free():
+    """forward-difference curl (gutsf.f:262-303) has zero forward divergence"""
+    o = make()
+    rng = np.random.default_rng(9)
+    E = periodic_ext(rng.standard_normal((3, o.nz - 2, o.ny - 2, o.nx - 2)))
+    cE = o.grid()
+    o.curlE(E, cE)
+    d = PAR["dx"]
+    c = cE[:, 1:-1, 1:-1, 1:-1]
+    div = ((c[0, :-1, :-1, 1:] - c[0, :-1, :-1, :-1]) + (c[1, :-1, 1:, :-1] - c[1, :-1, :-1, :-1])
+           + (c[2, 1:, :-1, :-1] - c[2, :-1, :-1, :-1])) / d
+    assert np.max(np.abs(div)) < 1e-12 * np.max(np.abs(c)) / d
+    # a plane wave E_y = sin(k x) gives curl_z = dE_y/dx by a forward difference
+    nx = o.nx - 2
+    kx = 2 * np.pi / nx
+    ii = np.arange(nx)
+    Ey = np.zeros((3, o.nz - 2, o.ny - 2, nx))
+    Ey[1] = np.sin(kx * ii)
+    cE = o.grid()
+    o.curlE(periodic_ext(Ey), cE)
+    want = (np.sin(kx * (ii + 1)) - np.sin(kx * ii)) / d
+    assert np.allclose(cE[2, 1:-1, 1:-1, 1:-1], want, rtol=1e-10, atol=1e-14)
+    assert np.allclose(cE[0:2, 1:-1, 1:-1, 1:-1], 0, atol=1e-14)
+
+
+def test_curlB_of_gradient_vanishes_and_density_scaling():
+    """backward-difference curl (gutsf.f:202-239) annihilates a backward
+    gradient; aj = curl B / (alpha * edge density)"""
+    o = make()
+    rng = np.random.default_rng(10)
+    phi = periodic_ext(rng.standard_normal((o.nz - 2, o.ny - 2, o.nx - 2)))
+    d = PAR["dx"]
+    B = o.grid()
+    B[0, :, :, 1:] = np.diff(phi, axis=2) / d
+    B[1, :, 1:, :] = np.diff(phi, axis=1) / d
+    B[2, 1:, :, :] = np.diff(phi, axis=0) / d
+    npg = np.full(o.grid(1).shape, 1e14)
+    aj = o.grid()
+    o.curlB(B, npg, aj)
+    assert np.max(np.abs(aj[:, 1:-1, 1:-1, 1:-1])) < 1e-10 * np.max(np.abs(B)) / d / (
+        1e14 * PAR["alpha"])
+    # B_y = sin(k x): aj_z = (B_y(i) - B_y(i-1)) / dx / (alpha n)
+    nx = o.nx - 2
+    kx = 2 * np.pi / nx
+    ii = np.arange(nx)
+    core = np.zeros((3, o.nz - 2, o.ny - 2, nx))
+    core[1] = np.sin(kx * ii)
+    npg = np.full(o.grid(1).shape, 2e14)
+    aj = o.grid()
+    o.curlB(periodic_ext(core), npg, aj)
+    want = (np.sin(kx * ii) - np.sin(kx * (ii - 1))) / d / (PAR["alpha"] * 2e14)
+    assert np.allclose(aj[2, 1:-1, 1:-1, 1:-1], want, rtol=1e-10)
+
+
+def test_get_E_uniform_drift():
+    """uniform B, up = V, no current: E = -V x B everywhere (gutsf.f:994-1110)"""
+    o = make()
+    V = np.array([400.0, 10.0, -3.0])
+    B = np.array([0.0, 0.0, 0.019161675])
+    E, aj = o.grid(), o.grid()
+    bt, up = uniform_field(o, B), uniform_field(o, V)
+    npg = np.full(o.grid(1).shape, 1e14)
+    nu = np.full(o.grid(1).shape, PAR["nu_init"])
+    o.get_E(E, bt, aj, up, npg, nu)
+    want = -np.cross(V, B)
+    for c in range(3):
+        assert np.allclose(E[c, 1:-1, 1:-1, 1:-1], want[c], rtol=1e-12, atol=1e-15)
+
+
+def test_predict_correct_keep_uniform_equilibrium():
+    """uniform B at rest: E = 0, so predict_B/correct_B leave B unchanged and
+    f_update_tlev rotates the time levels (gutsf.f:10-35, 1115-1359)"""
+    o = make()
+    B = np.array([0.001, 0.0, 0.019161675])
+    b1, b12, b1p2, bt = (uniform_field(o, B) for _ in range(4))
+    E, aj, up = o.grid(), o.grid(), o.grid()
+    npg = np.full(o.grid(1).shape, 1e14)
+    nu = np.full(o.grid(1).shape, PAR["nu_init"])
+    o.predict_B(b1, b12, b1p2, bt, E, aj, up, npg, nu)
+    assert np.allclose(b1p2, b1, rtol=1e-14)
+    o.correct_B(b1, b1p2, E, aj, up, npg, nu)
+    assert np.allclose(b1p2, b1, rtol=1e-14)
+    assert np.allclose(E, 0, atol=1e-18)
+    b1p2[2] += 1e-3
+    o.f_update_tlev(b1, b12, b1p2, bt)
+    assert np.array_equal(b1, b1p2) and np.array_equal(bt, b1p2)
+
+
+def whistler_dt(B, n):
+    """dx / (phase speed) of the whistler branch at k = 2/dx:
+    omega = k^2 B/(alpha n) * (1 + sqrt(1 + 4 alpha n / (k^2 B))) / 2"""
+    k = 2.0 / PAR["dx"]
+    w1 = k * k * B / (PAR["alpha"] * n)
+    omega = 0.5 * w1 * (1.0 + math.sqrt(1.0 + 4.0 * B / w1))
+    return PAR["dx"] / (omega / k)
+
+
+@pytest.mark.parametrize("ratio, halvings", [(0.3, 0), (0.9, 0), (1.5, 1), (3.0, 2),
+                                             (5.0, 3), (1e30, None)])
+def test_check_time_step_halving_rule(ratio, halvings):
+    """gutsf.f:1776-1822: dtsub halves once per node whose bound dx/phi is at
+    most 2 dtsub, re-tested node by node with the halved value; with a
+    uniform field and ratio = 2 dtsub / bound that is floor(log2 ratio) + 1
+    halvings, capped by the node count"""
+    o = make(nc=(2, 2, 2))
+    B, n = 0.019161675, 1e14
+    D = whistler_dt(B, n)
+    dtsub0 = ratio * D / 2.0
+    o.set_dtsub(dtsub0, 10.0)
+    o.check_time_step(uniform_field(o, (0.0, 0.0, B)), np.full(o.grid(1).shape, n))
+    nodes = o.nx * o.ny * o.nz
+    h = nodes if halvings is None else halvings
+    dts, ntf = o.dtsub()
+    assert dts == pytest.approx(dtsub0 / 2 ** h, rel=1e-13)
+    assert ntf == 10.0 * 2 ** h
+
+
+# ---------------------------------------------------------------- build-only pieces
+def test_philox_known_answers():
+    """Random123 philox4x32_10 known-answer vectors"""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2,
+            (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, want in kat:
+        assert tuple(int(x) for x in O.philox4x32_10(ctr, key)) == want
+
+
+def test_stable_sort_matches_numpy():
+    rng = np.random.default_rng(11)
+    for nkeys in (1, 7, 1000):
+        key = rng.integers(0, nkeys, 20000).astype(np.uint32)
+        perm = O.stable_sort(key, nkeys)
+        assert np.array_equal(perm, np.argsort(key, kind="stable").astype(np.uint32))
+    assert O.stable_sort(np.zeros(0, np.uint32), 4).size == 0
+
+
+def test_split_move_matches_fp64_positions():
+    """split-precision half move: cell index + FP32 fraction in (0,1], periodic
+    wrap; the absolute position agrees with an FP64 move to FP32 rounding"""
+    rng = np.random.default_rng(12)
+    nc = np.array([64, 32, 32])
+    n = 20000
+    i, j, k = (rng.integers(0, m, n) for m in nc)
+    cell = (i | (j << 11) | (k << 21)).astype(np.uint32)
+    f = (1.0 - rng.random((3, n))).astype(np.float32)
+    v = (rng.standard_normal((3, n)) * 30 + np.array([[400], [0], [0]])).astype(np.float32)
+    s = np.float32(0.5 * PAR["dt"] / PAR["dx"])
+    cell2, fx, fy, fz = cell.copy(), f[0].copy(), f[1].copy(), f[2].copy()
+    O.split_move(cell2, fx, fy, fz, v[0].copy(), v[1].copy(), v[2].copy(), [s, s, s], nc)
+    new = [cell2 & 2047, (cell2 >> 11) & 1023, cell2 >> 21]
+    old = [i, j, k]
+    for a, fa in enumerate((fx, fy, fz)):
+        assert np.all((fa > 0) & (fa <= 1))
+        assert np.all((new[a] >= 0) & (new[a] < nc[a]))
+        want = np.mod(old[a] + f[a].astype(np.float64) + float(s) * v[a].astype(np.float64),
+                      nc[a])
+        got = new[a] + fa.astype(np.float64)
+        d = np.abs(got - want)
+        d = np.minimum(d, nc[a] - d)  # periodic distance
+        assert np.max(d) < 4e-7 * 4, a
+
+
+def test_load_maxwellian_statistics_and_determinism():
+    """part_init.f:174-224: uniform positions, v = vsw + vth sqrt(-ln u1) cos(pi u2)
+    per component (a Gaussian of sigma vth/sqrt(2))"""
+    n, nc = 200000, (4, 4, 4)
+    cell, frac, v = O.load_maxwellian(n, 20261203, nc, 400.0, 30.0)
+    cell2, frac2, v2 = O.load_maxwellian(n, 20261203, nc, 400.0, 30.0)
+    assert np.array_equal(cell, cell2) and np.array_equal(v, v2) and np.array_equal(frac, frac2)
+    assert np.all((frac > 0) & (frac <= 1))
+    sig = 30.0 / math.sqrt(2)
+    for c, mean in enumerate((400.0, 0.0, 0.0)):
+        assert abs(v[c].mean() - mean) < 5 * sig / math.sqrt(n)
+        assert v[c].std() == pytest.approx(sig, rel=0.01)
+    i, j, k = cell & 2047, (cell >> 11) & 1023, cell >> 21
+    counts = np.bincount((i + 4 * (j + 4 * k)).astype(np.int64), minlength=64)
+    expect = n / 64
+    assert np.all(np.abs(counts - expect) < 5 * math.sqrt(expect))
+    _, _, v3 = O.load_maxwellian(n, 7, nc, 400.0, 30.0)
+    assert not np.array_equal(v, v3)
+
+
+# ---------------------------------------------------------------- composite step
+def seeded_state(o, width, ppc=4, seed=13):
+    rng = np.random.default_rng(seed)
+    n = ppc * (o.nx - 2) * (o.ny - 2) * (o.nz - 2)
+    dt = o.dtype
+    st = {k: o.grid() for k in ("b1", "b12", "b1p2", "bt", "btc", "up", "aj", "E")}
+    st["np"] = o.grid(1)
+    st["nu"] = np.full(o.grid(1).shape, PAR["nu_init"], dt)
+    for b in ("b1", "b12", "b1p2", "bt"):
+        st[b][2] = 0.019161675
+    xp = positions(o, n, rng).astype(dt)
+    vp = (rng.standard_normal((3, n)) * 21.2 + np.array([[400.0], [0], [0]])).astype(dt)
+    st.update(xp=xp, vp=vp, vp1=vp.copy(), vplus=o.parts(n), vminus=o.parts(n),
+              Ep=o.parts(n), wght=o.parts(n, 8), ijkp=np.zeros((3, n), np.int32), Ni_tot=n)
+    return st
+
+
+def test_step_fp32_tracks_fp64_and_conserves_charge():
+    """one maind.f:385-479 step in FP32 and FP64 from the same load"""
+    out = {}
+    for width in (64, 32):
+        o = make(width=width, npart=4 * int(np.prod(NC)))
+        st = seeded_state(o, width)
+        o.step(st)
+        out[width] = st
+    a, b = out[64], out[32]
+    for name, tol in (("xp", 1e-6), ("vp", 1e-4), ("np", 1e-4), ("b1", 1e-4)):
+        x, y = a[name].astype(np.float64), b[name].astype(np.float64)
+        assert np.max(np.abs(x - y)) / np.max(np.abs(x)) < tol, name
+    n = a["Ni_tot"]
+    beta = n / (np.prod(NC) * PAR["dx"] ** 3 * 0.1e15)
+    total = a["np"][1:-1, 1:-1, 1:-1].sum() * PAR["dx"] ** 3 * beta
+    assert total == pytest.approx(n, rel=1e-12)
