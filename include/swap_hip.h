@@ -142,6 +142,12 @@ int swap_load_maxwellian(swap_ctx *ctx, int64_t n, double vsw, double vth,
 /* host copy of one particle array (see SWAP_P_*); dtype: FP64 for XP, the
    context precision for real arrays, uint32/int32 for integer arrays */
 int swap_get_particle_array(swap_ctx *ctx, int which, void *host);
+/* host -> device for one array of the n loaded particles, with the layouts
+   and dtypes of swap_get_particle_array.  XP re-splits the positions into
+   cell + fraction.  EP, VPLUS, VMINUS, WGHT and IJKP fill the routine-level
+   arrays, so a shim that keeps the reference's host arrays (xp, vp, vp1,
+   Ep, vplus, vminus of gutsp.f) can pass them in before each routine. */
+int swap_set_particle_array(swap_ctx *ctx, int which, const void *host);
 
 /* ---- fields ---- */
 int swap_set_field(swap_ctx *ctx, int field, const double *host);

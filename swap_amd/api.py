@@ -82,6 +82,7 @@ def load_library():
         "swap_load_particles_split": ([vp, i64, vp, vp, vp, vp, vp, vp], i32),
         "swap_load_maxwellian": ([vp, i64, dbl, dbl, dbl, dbl], i32),
         "swap_get_particle_array": ([vp, i32, vp], i32),
+        "swap_set_particle_array": ([vp, i32, vp], i32),
         "swap_set_field": ([vp, i32, vp], i32),
         "swap_get_field": ([vp, i32, vp], i32),
         "swap_set_field_f32": ([vp, i32, vp], i32),
@@ -199,8 +200,7 @@ class Swap:
         self._ck(self.lib.swap_load_maxwellian(self.h, n, vsw, vth, beta_p, mrat),
                  "load_maxwellian")
 
-    def particle_array(self, which):
-        n = self.n
+    def _parray_spec(self, which):
         ncomp = {P_XP: 3, P_VP: 3, P_VP1: 3, P_EP: 3, P_VPLUS: 3, P_VMINUS: 3, P_IJKP: 3,
                  P_WGHT: 8, P_CELL: 1, P_FRAC: 3, P_BETA: 1, P_MRAT: 1, P_ID: 1}[which]
         if which == P_XP:
@@ -213,10 +213,20 @@ class Swap:
             dt = np.float32
         else:
             dt = self.rdtype
-        out = np.zeros((ncomp, n) if ncomp > 1 else (n,), dt)
+        return ((ncomp, self.n) if ncomp > 1 else (self.n,)), dt
+
+    def particle_array(self, which):
+        shape, dt = self._parray_spec(which)
+        out = np.zeros(shape, dt)
         self._ck(self.lib.swap_get_particle_array(self.h, which, out.ctypes.data),
                  "get_particle_array")
         return out
+
+    def set_particle_array(self, which, arr):
+        shape, dt = self._parray_spec(which)
+        a = np.ascontiguousarray(np.broadcast_to(arr, shape), dt)
+        self._ck(self.lib.swap_set_particle_array(self.h, which, a.ctypes.data),
+                 "set_particle_array")
 
     def xp(self):
         return self.particle_array(P_XP)

@@ -160,6 +160,8 @@ int ensure_routine_scratch(swap_ctx* c) {
 #define PL(kern, n, ...) \
   hipLaunchKernelGGL(kern, dim3(nblk((n), 256)), dim3(256), 0, c->stream, __VA_ARGS__)
 
+int pl_ensure_scratch(swap_ctx* c) { return ensure_routine_scratch(c); }
+
 int pl_weights(swap_ctx* c) {
   if (int e = ensure_routine_scratch(c)) return e;
   if (c->n == 0) return 0;

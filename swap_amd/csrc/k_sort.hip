@@ -5,22 +5,62 @@
 // tile, so a work item (a run of one tile's particles) touches a compact
 // block of nodes.
 //
-// The sort works on runs: a run is a maximal group of consecutive particles
-// inside one wavefront that share a key (input that was sorted a few steps
-// ago is made of long runs).
-//   A: count particles and runs per key (one atomic per run);
+// The sort is two-level.  The particles are cut into chunks of SCH
+// consecutive indices; a chunk's keys are first sorted stably inside LDS
+// (a 4-bit LSD radix sort), which turns the chunk into segments, one per
+// distinct key.  Globally only segments are counted:
+//   A: per chunk, LDS sort; record each segment (key, start, length), count
+//      particles and segments per key (two atomics per segment) and store
+//      each sorted position's source index and segment;
 //   scans of both counts;
-//   B: write each run (head index, length) into its key's run list;
-//   C: per key, order its runs by head index and turn lengths into
+//   B: write each segment (chunk position, length) into its key's list;
+//   C: per key, order its segments by chunk and turn lengths into
 //      destination offsets;
-//   D: every particle finds its run's offset and is written once, with all
-//      its arrays, to the destination.
-// The result is exactly the stable order (runs of a key ordered by their
-// first index, indices inside a run consecutive), independent of the order
-// in which atomics were served.
+//   D: per chunk, walk the sorted positions: every particle is written
+//      once, with all its arrays, to its segment's offset + its rank, so
+//      the writes of a segment are contiguous.
+// A chunk holds at most one segment per key, so ordering a key's segments
+// by chunk and keeping the LDS sort stable gives exactly the stable order,
+// independent of the order in which atomics were served.  Particles drift
+// several cells per step, which breaks runs of equal keys within a
+// wavefront into pieces of one or two; grouping by chunk keeps the number
+// of global atomics and of scattered writes at the number of segments.
 #include "swap_internal.h"
 
 namespace {
+
+constexpr int ST = 512, SE = 8, SCH = ST * SE;  // chunk: 4096 particles
+constexpr int SCNT = 16 * ST;                   // radix counters, bin-major
+__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
+
+// exclusive scan of one value per thread over the workgroup; *total gets the
+// sum.  Ends with a barrier, so wsum may be reused right away.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum,
+                                                     uint32_t* total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t s = 0;
+    for (int q = 0; q < ST / 64; q++) {
+      uint32_t u = wsum[q];
+      wsum[q] = s;
+      s += u;
+    }
+    wsum[ST / 64] = s;
+  }
+  __syncthreads();
+  uint32_t r = x - v + wsum[w];
+  *total = wsum[ST / 64];
+  __syncthreads();
+  return r;
+}
 
 __device__ __forceinline__ uint32_t tile_key(uint32_t c, int ntx, int nty) {
   int i = cell_i(c), j = cell_j(c), k = cell_k(c);
@@ -30,67 +70,159 @@ __device__ __forceinline__ uint32_t tile_key(uint32_t c, int ntx, int nty) {
          TILE_X * ((uint32_t)(j % TILE_Y) + TILE_Y * (uint32_t)(k % TILE_Z));
 }
 
-struct Run {
-  int head;  // lane of this lane's run head
-  int len;   // run length (meaningful on the head lane)
-};
-__device__ __forceinline__ Run wave_runs(uint32_t key) {
-  const int lane = __lane_id();
-  uint32_t prev = __shfl_up(key, 1);
-  bool head = lane == 0 || prev != key;
-  unsigned long long hm = __ballot(head);
-  unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
-  Run r;
-  r.head = 63 - __clzll(hm & le);
-  unsigned long long above = hm & ~le;
-  int nh = above ? __ffsll((long long)above) - 1 : 64;
-  r.len = nh - lane;
-  return r;
-}
+// A: one workgroup per chunk.  Keys below 2^nbits; padding positions get
+// the largest nbits-bit key and, the sort being stable, end up last.
+__global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restrict__ cell,
+                                              int ntx, int nty, int nbits, uint32_t* cnt,
+                                              uint32_t* nrun, uint32_t* segkey,
+                                              uint32_t* seginfo, uint32_t* nsegc,
+                                              uint32_t* perm) {
+  __shared__ uint32_t skey[SCH];
+  __shared__ uint16_t sidx[SCH];
+  __shared__ uint32_t scan[SCNT + SCNT / 16];
+  __shared__ uint32_t wsum[ST / 64 + 1];
+  const int t = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * SCH;
+  const int m = (int)min((int64_t)SCH, n - c0);
+  const uint32_t padkey = nbits >= 32 ? 0xFFFFFFFFu : ((1u << nbits) - 1u);
+  uint32_t key[SE], idx[SE];
+  if (t * SE + SE <= m) {
+    const uint4* src = (const uint4*)(cell + c0) + 2 * t;  // c0 is a multiple of 4096
+    uint4 a = src[0], b = src[1];
+    uint32_t cs[SE] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int e = 0; e < SE; e++) key[e] = tile_key(cs[e], ntx, nty);
+  } else {
+#pragma unroll
+    for (int e = 0; e < SE; e++) {
+      int l = t * SE + e;
+      key[e] = l < m ? tile_key(cell[c0 + l], ntx, nty) : padkey;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < SE; e++) idx[e] = (uint32_t)(t * SE + e);
 
-__device__ __forceinline__ uint32_t key_at(int64_t p, int64_t n, const uint32_t* cell, int ntx,
-                                           int nty) {
-  return p < n ? tile_key(cell[p], ntx, nty) : 0xFFFFFFFFu;
-}
+  // stable LSD radix sort, 4 bits per pass; thread t holds positions
+  // t*SE .. t*SE+SE-1 before and after every pass
+  for (int sh = 0; sh < nbits; sh += 4) {
+    int d[SE], r[SE];
+#pragma unroll
+    for (int e = 0; e < SE; e++) d[e] = (int)((key[e] >> sh) & 15u);
+#pragma unroll
+    for (int b = 0; b < 16; b++) scan[pad16(b * ST + t)] = 0;
+#pragma unroll
+    for (int e = 0; e < SE; e++) {
+      int rr = 0;
+#pragma unroll
+      for (int e2 = 0; e2 < e; e2++) rr += d[e2] == d[e] ? 1 : 0;
+      r[e] = rr;
+    }
+#pragma unroll
+    for (int e = 0; e < SE; e++) {
+      bool last = true;
+#pragma unroll
+      for (int e2 = e + 1; e2 < SE; e2++) last = last && d[e2] != d[e];
+      if (last) scan[pad16(d[e] * ST + t)] = (uint32_t)(r[e] + 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counters in (digit, thread) order: thread t
+    // rakes entries 16t .. 16t+15
+    uint32_t v[16], s = 0, tot;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      v[j] = scan[pad16(16 * t + j)];
+      s += v[j];
+    }
+    uint32_t pre = block_excl_scan(s, wsum, &tot);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      scan[pad16(16 * t + j)] = pre;
+      pre += v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < SE; e++) {
+      int pos = (int)scan[pad16(d[e] * ST + t)] + r[e];
+      skey[pos] = key[e];
+      sidx[pos] = (uint16_t)idx[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < SE; e++) {
+      key[e] = skey[t * SE + e];
+      idx[e] = sidx[t * SE + e];
+    }
+  }
+  if (nbits == 0) {
+#pragma unroll
+    for (int e = 0; e < SE; e++) skey[t * SE + e] = key[e];
+  }
+  __syncthreads();
 
-__global__ void k_sortA(int64_t n, const uint32_t* cell, int ntx, int nty, uint32_t* cnt,
-                        uint32_t* nrun) {
-  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  uint32_t k = key_at(p, n, cell, ntx, nty);
-  Run r = wave_runs(k);
-  if (p < n && r.head == __lane_id()) {
-    atomicAdd(cnt + k, (uint32_t)r.len);
+  // segments: a head is a valid position whose key differs from the previous
+  bool hd[SE];
+  uint32_t heads = 0;
+#pragma unroll
+  for (int e = 0; e < SE; e++) {
+    int p = t * SE + e;
+    hd[e] = p < m && (p == 0 || skey[p - 1] != key[e]);
+    heads += hd[e] ? 1u : 0u;
+  }
+  uint32_t nseg;
+  const uint32_t s0 = block_excl_scan(heads, wsum, &nseg);
+  {
+    uint32_t s = s0;
+#pragma unroll
+    for (int e = 0; e < SE; e++)
+      if (hd[e]) scan[s++] = (uint32_t)(t * SE + e);
+    if (t == 0) scan[nseg] = (uint32_t)m;
+  }
+  __syncthreads();
+  for (uint32_t q = t; q < nseg; q += ST) {
+    uint32_t st = scan[q], len = scan[q + 1] - st;
+    uint32_t k = skey[st];
+    segkey[c0 + q] = k;
+    seginfo[c0 + q] = st | (len << 12);
+    atomicAdd(cnt + k, len);
     atomicAdd(nrun + k, 1u);
   }
+  int sid = (int)s0 - 1;  // a thread's leading non-heads continue the previous segment
+#pragma unroll
+  for (int e = 0; e < SE; e++) {
+    int p = t * SE + e;
+    if (hd[e]) sid++;
+    if (p < m) perm[c0 + p] = idx[e] | ((uint32_t)sid << 12);
+  }
+  if (t == 0) nsegc[blockIdx.x] = nseg;
 }
 
-__global__ void k_sortB(int64_t n, const uint32_t* cell, int ntx, int nty, const uint32_t* rstart,
-                        uint32_t* rcur, uint2* runs) {
-  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  uint32_t k = key_at(p, n, cell, ntx, nty);
-  Run r = wave_runs(k);
-  if (p < n && r.head == __lane_id()) {
+// B: each segment into its key's list
+__global__ void k_sortB(const uint32_t* nsegc, const uint32_t* segkey, const uint32_t* seginfo,
+                        const uint32_t* rstart, uint32_t* rcur, uint2* runs) {
+  const int64_t c0 = (int64_t)blockIdx.x * SCH;
+  const uint32_t ns = nsegc[blockIdx.x];
+  for (uint32_t q = threadIdx.x; q < ns; q += blockDim.x) {
+    uint32_t k = segkey[c0 + q];
     uint32_t slot = rstart[k] + atomicAdd(rcur + k, 1u);
-    runs[slot] = make_uint2((uint32_t)p, (uint32_t)r.len);
+    runs[slot] = make_uint2((uint32_t)(c0 + q), seginfo[c0 + q] >> 12);
   }
 }
 
-// one thread per run: its destination is the key's start plus the lengths of
-// the key's runs with a smaller head index; stored at the head's index
-__global__ void k_sortC(const uint32_t* cell, int ntx, int nty, const uint32_t* kstart,
-                        const uint32_t* rstart, int64_t nkeys, const uint2* runs,
-                        uint32_t* rbase) {
+// C: one thread per segment record: its destination is the key's start plus
+// the lengths of the key's segments from earlier chunks
+__global__ void k_sortC(const uint32_t* segkey, const uint32_t* kstart, const uint32_t* rstart,
+                        int64_t nkeys, const uint2* runs, uint32_t* segbase) {
   const uint32_t nruns = rstart[nkeys];
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nruns;
        q += (int64_t)gridDim.x * blockDim.x) {
     uint2 me = runs[q];
-    uint32_t k = tile_key(cell[me.x], ntx, nty);
+    uint32_t k = segkey[me.x];
     uint32_t base = kstart[k];
     for (uint32_t x = rstart[k]; x < rstart[k + 1]; x++) {
       uint2 o = runs[x];
       if (o.x < me.x) base += o.y;
     }
-    rbase[me.x] = base;
+    segbase[me.x] = base;
   }
 }
 
@@ -105,23 +237,53 @@ struct Move {
   uint32_t* id2;
 };
 
-template <typename R>
-__global__ void k_sortD(int64_t n, int ntx, int nty, const uint32_t* rbase, Move Mv) {
-  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  uint32_t c = p < n ? Mv.cell[p] : 0u;
-  uint32_t k = p < n ? tile_key(c, ntx, nty) : 0xFFFFFFFFu;
-  Run r = wave_runs(k);
-  uint32_t base = 0;
-  if (p < n && r.head == __lane_id()) base = rbase[p];
-  base = __shfl(base, r.head);
-  if (p >= n) return;
-  uint32_t d = base + (uint32_t)(__lane_id() - r.head);
-  Mv.cell2[d] = c;
+// D: one workgroup per chunk walks its sorted positions; writes of a
+// segment are contiguous.  Every thread resolves its DPT positions first;
+// then each array of the chunk is read once, coalesced, into LDS and
+// written out in sorted order from there.
+constexpr int DT = 256, DPT = SCH / DT;
+template <typename T>
+__device__ __forceinline__ void move_array(const T* __restrict__ src, T* __restrict__ dst,
+                                           T* stage, int64_t c0, int m,
+                                           const uint32_t (&s)[DPT], const uint32_t (&d)[DPT],
+                                           int cnt) {
+  __syncthreads();  // the previous array's reads of stage are done
 #pragma unroll
-  for (int a = 0; a < 9; a++) ((R*)Mv.p2[a])[d] = ((const R*)Mv.p[a])[p];
-  if (Mv.bp) Mv.bp2[d] = Mv.bp[p];
-  if (Mv.mr) Mv.mr2[d] = Mv.mr[p];
-  if (Mv.id) Mv.id2[d] = Mv.id[p];
+  for (int i = 0; i < DPT; i++) {
+    int p = (int)threadIdx.x + i * DT;
+    if (p < m) stage[p] = src[c0 + p];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < DPT; i++)
+    if (i < cnt) dst[d[i]] = stage[s[i]];
+}
+template <typename R>
+__global__ __launch_bounds__(DT) void k_sortD(int64_t n, const uint32_t* __restrict__ perm,
+                                              const uint32_t* __restrict__ seginfo,
+                                              const uint32_t* __restrict__ segbase, Move Mv) {
+  __shared__ R stage[SCH];
+  const int64_t c0 = (int64_t)blockIdx.x * SCH;
+  const int m = (int)min((int64_t)SCH, n - c0);
+  const int cnt = m > (int)threadIdx.x ? (m - (int)threadIdx.x + DT - 1) / DT : 0;
+  uint32_t s[DPT], d[DPT];
+#pragma unroll
+  for (int i = 0; i < DPT; i++) {
+    if (i < cnt) {
+      int p = (int)threadIdx.x + i * DT;
+      uint32_t w = perm[c0 + p];
+      uint32_t sg = w >> 12;
+      s[i] = w & 4095u;
+      d[i] = segbase[c0 + sg] + (uint32_t)p - (seginfo[c0 + sg] & 4095u);
+    }
+  }
+  move_array(Mv.cell, Mv.cell2, (uint32_t*)stage, c0, m, s, d, cnt);
+#pragma unroll
+  for (int a = 0; a < 9; a++)
+    move_array((const R*)Mv.p[a], (R*)Mv.p2[a], stage, c0, m, s, d, cnt);
+  if (Mv.bp) move_array(Mv.bp, Mv.bp2, (float*)stage, c0, m, s, d, cnt);
+  if (Mv.mr) move_array(Mv.mr, Mv.mr2, (float*)stage, c0, m, s, d, cnt);
+  if (Mv.id) move_array(Mv.id, Mv.id2, (uint32_t*)stage, c0, m, s, d, cnt);
 }
 
 // ---- exclusive scan of uint32 counts (3 phases) ----
@@ -232,29 +394,33 @@ int so_sort(swap_ctx* c) {
   const int64_t n = c->n, nk = c->nkeys;
   const int ntx = c->ntile[0], nty = c->ntile[1];
   if (n > 0) {
+    const int64_t nch = (n + SCH - 1) / SCH;
+    int nbits = 0;
+    while (nbits < 32 && (1ull << nbits) < (unsigned long long)nk) nbits++;
     HIPCHK(c, hipMemsetAsync(c->scnt, 0, sizeof(uint32_t) * nk, c->stream));
     HIPCHK(c, hipMemsetAsync(c->snrun, 0, sizeof(uint32_t) * nk, c->stream));
-    hipLaunchKernelGGL(k_sortA, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, c->cell, ntx, nty,
-                       c->scnt, c->snrun);
+    hipLaunchKernelGGL(k_sortA, dim3((unsigned)nch), dim3(ST), 0, c->stream, n, c->cell, ntx,
+                       nty, nbits, c->scnt, c->snrun, c->segkey, c->seginfo, c->nsegc,
+                       c->srbase);
     if (int e = post(c)) return e;
     if (int e = scan(c, nk, c->scnt, c->sstart)) return e;
     if (int e = scan(c, nk, c->snrun, c->srstart)) return e;
     HIPCHK(c, hipMemsetAsync(c->scur, 0, sizeof(uint32_t) * nk, c->stream));
-    hipLaunchKernelGGL(k_sortB, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, c->cell, ntx, nty,
-                       c->srstart, c->scur, c->sruns);
-    hipLaunchKernelGGL(k_sortC, dim3(8192), dim3(256), 0, c->stream, c->cell, ntx, nty,
-                       c->sstart, c->srstart, nk, c->sruns, c->srbase);
+    hipLaunchKernelGGL(k_sortB, dim3((unsigned)nch), dim3(256), 0, c->stream, c->nsegc,
+                       c->segkey, c->seginfo, c->srstart, c->scur, c->sruns);
+    hipLaunchKernelGGL(k_sortC, dim3(8192), dim3(256), 0, c->stream, c->segkey, c->sstart,
+                       c->srstart, nk, c->sruns, c->segbase);
     Move Mv;
     Mv.cell = c->cell; Mv.cell2 = c->cell2;
     for (int a = 0; a < 9; a++) { Mv.p[a] = c->p[a]; Mv.p2[a] = c->p2[a]; }
     Mv.bp = c->bp; Mv.bp2 = c->bp2; Mv.mr = c->mr; Mv.mr2 = c->mr2;
     Mv.id = c->id; Mv.id2 = c->id2;
     if (c->prec == 64)
-      hipLaunchKernelGGL(k_sortD<double>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, ntx,
-                         nty, c->srbase, Mv);
+      hipLaunchKernelGGL(k_sortD<double>, dim3((unsigned)nch), dim3(256), 0, c->stream, n,
+                         c->srbase, c->seginfo, c->segbase, Mv);
     else
-      hipLaunchKernelGGL(k_sortD<float>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, ntx,
-                         nty, c->srbase, Mv);
+      hipLaunchKernelGGL(k_sortD<float>, dim3((unsigned)nch), dim3(256), 0, c->stream, n,
+                         c->srbase, c->seginfo, c->segbase, Mv);
     if (int e = post(c)) return e;
     std::swap(c->cell, c->cell2);
     for (int a = 0; a < 9; a++) std::swap(c->p[a], c->p2[a]);

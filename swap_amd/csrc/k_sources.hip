@@ -36,6 +36,20 @@ __global__ void k_load_abs(int64_t n, int64_t off, PDev<R> P, const double* xp, 
 }
 
 template <typename R>
+__global__ void k_set_abs(int64_t n, PDev<R> P, const double* xp, double dx, double dy, double dz,
+                          int ncx, int ncy, int ncz) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int ci, cj, ck;
+  double f0, f1, f2;
+  split_pos(xp[p], dx, ncx, ci, f0);
+  split_pos(xp[p + n], dy, ncy, cj, f1);
+  split_pos(xp[p + 2 * n], dz, ncz, ck, f2);
+  P.cell[p] = pack_cell(ci, cj, ck);
+  P.fx[p] = (R)f0; P.fy[p] = (R)f1; P.fz[p] = (R)f2;
+}
+
+template <typename R>
 __global__ void k_export_abs(int64_t n, PDev<R> P, double* xp, double dx, double dy, double dz) {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -199,6 +213,14 @@ int pl_load_abs(swap_ctx* c, int64_t n, const double* xp_d, const double* v_d,
   SDISPATCH(c, hipLaunchKernelGGL((k_load_abs<R>), dim3(nblk(n, 256)), dim3(256), 0, c->stream, n,
                                   (int64_t)0, pdev<R>(c), xp_d, v_d, v1_d, c->cfg.dx, c->cfg.dy,
                                   c->cfg.dz, c->cfg.ncx, c->cfg.ncy, c->cfg.ncz));
+  return post(c);
+}
+
+int pl_set_abs(swap_ctx* c, const double* xp_d) {
+  if (c->n == 0) return 0;
+  SDISPATCH(c, hipLaunchKernelGGL((k_set_abs<R>), dim3(nblk(c->n, 256)), dim3(256), 0, c->stream,
+                                  c->n, pdev<R>(c), xp_d, c->cfg.dx, c->cfg.dy, c->cfg.dz,
+                                  c->cfg.ncx, c->cfg.ncy, c->cfg.ncz));
   return post(c);
 }
 

@@ -264,6 +264,10 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   c->nkeys = c->ntiles * TILE_CELLS;
   DMF(c->sruns, 8 * cp);
   DMF(c->srbase, 4 * cp);
+  DMF(c->segkey, 4 * cp);
+  DMF(c->seginfo, 4 * cp);
+  DMF(c->segbase, 4 * cp);
+  DMF(c->nsegc, 4 * (cp / 4096 + 2));
   DMF(c->scnt, 4 * (size_t)(c->nkeys + 1));
   DMF(c->sstart, 4 * (size_t)(c->nkeys + 1));
   DMF(c->scur, 4 * (size_t)(c->nkeys + 1));
@@ -292,7 +296,8 @@ int swap_destroy(swap_ctx* c) {
   void* ps[] = {c->b1, c->b12, c->b1p2, c->a, c->ct, c->btc0, c->cE, c->S, c->macc,
                 c->scratch, c->dred, c->dmin, c->dcnt, c->d_nitems, c->cell, c->cell2, c->bp,
                 c->bp2, c->mr, c->mr2, c->id, c->id2, c->Ep, c->vpl, c->vmi, c->wght, c->ijkp,
-                c->sruns, c->srbase, c->scnt, c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items,
+                c->sruns, c->srbase, c->segkey, c->seginfo, c->segbase, c->nsegc, c->scnt,
+                c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items,
                 c->boxes, c->boxes_next, c->vb, c->d_zero};
   for (void* p : ps) dfree(p);
   for (int f = 0; f < SWAP_F_COUNT; f++) dfree(c->fld[f]);
@@ -477,6 +482,70 @@ int swap_get_particle_array(swap_ctx* c, int which, void* host) {
     case SWAP_P_ID:
       if (!c->id) return set_err(c, SWAP_ESTATE, "ids are kept only with SWAP_TRACK_IDS");
       HIPCHK(c, hipMemcpyAsync(host, c->id, 4 * n, hipMemcpyDeviceToHost, c->stream));
+      break;
+    default:
+      return set_err(c, SWAP_EINVAL, "unknown particle array %d", which);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int swap_set_particle_array(swap_ctx* c, int which, const void* host) {
+  if (!c || !host) return SWAP_EINVAL;
+  size_t n = (size_t)c->n, rn = c->rsz * n;
+  if (!n) return 0;
+  switch (which) {
+    case SWAP_P_XP: {
+      double* d = nullptr;
+      HIPCHK(c, hipMalloc(&d, 24 * n));
+      HIPCHK(c, hipMemcpyAsync(d, host, 24 * n, hipMemcpyHostToDevice, c->stream));
+      int e = pl_set_abs(c, d);
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipFree(d));
+      c->n_sorted = -1;
+      return e;
+    }
+    case SWAP_P_VP:
+    case SWAP_P_VP1:
+    case SWAP_P_FRAC: {
+      int base = which == SWAP_P_VP ? 3 : (which == SWAP_P_VP1 ? 6 : 0);
+      for (int a = 0; a < 3; a++)
+        HIPCHK(c, hipMemcpyAsync(c->p[base + a], (const char*)host + a * rn, rn,
+                                 hipMemcpyHostToDevice, c->stream));
+      if (which == SWAP_P_VP) c->vb_valid = false;
+      break;
+    }
+    case SWAP_P_EP:
+    case SWAP_P_VPLUS:
+    case SWAP_P_VMINUS:
+    case SWAP_P_WGHT:
+    case SWAP_P_IJKP: {
+      TRY(pl_ensure_scratch(c));
+      void* dst = which == SWAP_P_EP ? c->Ep
+                  : which == SWAP_P_VPLUS ? c->vpl
+                  : which == SWAP_P_VMINUS ? c->vmi
+                  : which == SWAP_P_WGHT ? c->wght : (void*)c->ijkp;
+      int nc = which == SWAP_P_WGHT ? 8 : 3;
+      size_t es = which == SWAP_P_IJKP ? 4 : c->rsz;
+      for (int a = 0; a < nc; a++)
+        HIPCHK(c, hipMemcpyAsync((char*)dst + a * es * c->cap, (const char*)host + a * es * n,
+                                 es * n, hipMemcpyHostToDevice, c->stream));
+      break;
+    }
+    case SWAP_P_CELL:
+      HIPCHK(c, hipMemcpyAsync(c->cell, host, 4 * n, hipMemcpyHostToDevice, c->stream));
+      c->n_sorted = -1;
+      break;
+    case SWAP_P_BETA:
+    case SWAP_P_MRAT: {
+      float* dst = which == SWAP_P_BETA ? c->bp : c->mr;
+      if (!dst) return set_err(c, SWAP_ESTATE, "beta_p/mrat arrays need SWAP_SPECIES");
+      HIPCHK(c, hipMemcpyAsync(dst, host, 4 * n, hipMemcpyHostToDevice, c->stream));
+      break;
+    }
+    case SWAP_P_ID:
+      if (!c->id) return set_err(c, SWAP_ESTATE, "ids are kept only with SWAP_TRACK_IDS");
+      HIPCHK(c, hipMemcpyAsync(c->id, host, 4 * n, hipMemcpyHostToDevice, c->stream));
       break;
     default:
       return set_err(c, SWAP_EINVAL, "unknown particle array %d", which);

@@ -66,8 +66,10 @@ struct swap_ctx {
   int64_t ntiles = 0, nkeys = 0;
   uint32_t *scnt = nullptr, *sstart = nullptr, *scur = nullptr, *snrun = nullptr,
            *srstart = nullptr, *sblk = nullptr;
-  uint2* sruns = nullptr;     // run records of the sort, one per run (<= n)
-  uint32_t* srbase = nullptr; // destination of the run headed at each index
+  uint2* sruns = nullptr;     // segment records of the sort (<= n)
+  uint32_t* srbase = nullptr; // per sorted chunk position: source | segment << 12
+  uint32_t *segkey = nullptr, *seginfo = nullptr, *segbase = nullptr;  // per chunk segment
+  uint32_t* nsegc = nullptr;  // segments per chunk
   int *vb = nullptr, *vb_next = nullptr;  // signed velocity bounds, float keys
   bool vb_valid = false;
   int* d_zero = nullptr;
@@ -135,6 +137,8 @@ int pl_pass(swap_ctx* c, int mode);  // fused passes 1, 2, 3
 int pl_load_abs(swap_ctx* c, int64_t n, const double* xp_d, const double* v_d,
                 const double* v1_d);
 int pl_export_abs(swap_ctx* c, double* xp_d);
+int pl_set_abs(swap_ctx* c, const double* xp_d);  // positions only
+int pl_ensure_scratch(swap_ctx* c);               // Ep, vplus, vminus, wght, ijkp
 int pl_maxwellian_full(swap_ctx* c, int64_t off, int64_t n, double vsw, double vth, float bpv,
                        float mrv, uint32_t stream);
 int pl_box_reset(swap_ctx* c);
