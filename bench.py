@@ -25,6 +25,23 @@ METRIC = "particle-steps/s incl. deposit, % HBM peak; s/step vs plain at equal e
 # algorithmic bytes of each particle pass: per particle (SoA state read and
 # written) and per grid node (moments written, 6-component gather source read)
 PASS_BYTES = {"P1": (40, 16), "P2": (28, 12 + 24), "P3": (68, 16 + 24)}
+COPY_PROBE_BYTES = 1 << 30
+# measured HBM bytes per launch of each pass (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+# scaled by the same counters on a plain copy), written from a profiled run
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pass_traffic.json")
+
+
+def measured_traffic(kernel, cfg):
+    """HBM bytes per launch of `kernel` from the committed counter summary,
+    if it was measured on this workload; else None"""
+    try:
+        t = json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    if t.get("config") != cfg:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["fetch_bytes"] + k["write_bytes"]
 
 
 def cpu_baseline(steps=15):
@@ -66,11 +83,20 @@ def main():
     ap.add_argument("--sort-every", type=int, default=2)
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        x = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
 
     from swap_amd import physics
     ncx, ncy, ncz = args.cells
@@ -101,6 +127,9 @@ def main():
     alg = bp * npart + bn * nn
     achieved = alg / (ms_tot / cnt * 1e-3) / 1e9
     step_alg = 136 * npart + 116 * nn
+    kname = "k_pass<float,%s>" % dom[1]
+    tcfg = {"cells": [ncx, ncy, ncz], "ppc": args.ppc, "sort_every": args.sort_every,
+            "precision": args.precision}
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -112,9 +141,9 @@ def main():
                                % (ncx, ncy, ncz, args.ppc, npart),
                    "cells": [ncx, ncy, ncz], "ppc": args.ppc, "n_particles": npart,
                    "sort_every": args.sort_every, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_pass<float,%s>" % dom[1],
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, tcfg),
                      "alg_bytes_per_launch": alg,
                      "avg_launch_ms": ms_tot / cnt},
         "step_roofline": {"alg_bytes": step_alg, "achieved_GBs": step_alg / (ms_step * 1e-3) /

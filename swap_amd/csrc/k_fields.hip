@@ -286,7 +286,7 @@ __global__ void k_update_src(Geo g, FC f, const R* up, const R* ajc, R* upc, R* 
 template <typename R>
 __global__ void k_finalize_np(Geo g, const typename AccOf<R>::T* a, R qn, R* np) {
   GRID_LOOP_HEAD
-  np[o] = (R)acc_read(a, gidx(g, ci, cj, ck)) * qn;
+  np[o] = (R)(acc_read(a, gidx(g, ci, cj, ck), fix_bits(0)) * (double)qn);
 }
 
 // update_up's division by np and node->edge average (gutsp.f:1704-1756,
@@ -301,8 +301,8 @@ __global__ void k_finalize_up(Geo g, const typename AccOf<R>::T* a, const R* np,
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     R n1 = np[qn1[m]];
-    R c0 = n0 != (R)0 ? (R)acc_read(a, q0 + m * g.nn) * qn / n0 : (R)0;
-    R c1 = n1 != (R)0 ? (R)acc_read(a, qn1[m] + m * g.nn) * qn / n1 : (R)0;
+    R c0 = n0 != (R)0 ? (R)acc_read(a, q0 + m * g.nn, fix_bits(1)) * qn / n0 : (R)0;
+    R c1 = n1 != (R)0 ? (R)acc_read(a, qn1[m] + m * g.nn, fix_bits(1)) * qn / n1 : (R)0;
     up[o + m * g.nn] = (R)0.5 * (c0 + c1);
   }
 }

@@ -47,9 +47,10 @@ __global__ void k_deposit_simple(Geo g, int64_t n, PDev<R> P, int mode,
     int u = ci + (cc & 1), v = cj + ((cc >> 2) & 1), t = ck + ((cc >> 1) & 1);
     int64_t node = gidx(g, node_of(u, g.ncx), node_of(v, g.ncy), node_of(t, g.ncz));
     R wr = w[cc] * r;
-    if (mode == 0) acc_add(acc, node, wr);
+    if (mode == 0) acc_add(acc, node, wr, fix_bits(0));
     else
-      for (int m = 0; m < 3; m++) acc_add(acc, node + (int64_t)(m + 1) * g.nn, wr * q[m]);
+      for (int m = 0; m < 3; m++)
+        acc_add(acc, node + (int64_t)(m + 1) * g.nn, wr * q[m], fix_bits(m + 1));
   }
 }
 

@@ -8,19 +8,29 @@
 // (gutsp.f:1704-1756) is done once per node after the sums.
 //
 // One workgroup runs one work item: a run of one tile's particles after the
-// counting sort.  Lane l of a wavefront takes particle base + l (coalesced
-// loads).  Particles of one cell sit on neighbouring lanes, so the deposit is
-// reduced across the wavefront first: every one of the 8 corners x M moments
-// goes through one segmented inclusive scan over the lanes (DPP row shifts
-// and row broadcasts, segments = runs of lanes with the same cell), and only
-// the last lane of each run adds its run's sums into the LDS window.  The
-// window covers the item's deposit footprint and accumulates compensated
-// FP32 (hi, lo) pairs: the returning LDS add gives the exact rounding error
-// of each addition (TwoSum), which goes into lo.  The window is added to HBM
-// once per item, again compensated.  The window of P1/P2 is the item's cell
-// box at x^n; P3's is that box widened by the half-step displacement range
-// from signed per-axis velocity bounds (reduced by the previous P3).
-// Particles outside the window deposit straight to HBM (counted as misses).
+// counting sort, processed in chunks of PASS_WG * PASS_KP particles.  Lane t
+// of a chunk takes PASS_KP consecutive particles and walks them in order,
+// summing each particle's 8 corners x M moments in registers while the
+// deposit cell stays the same.  A lane keeps the sums of its first cell
+// aside when it moves on, adds the sums of any middle cell into the LDS
+// window at once, and at the end of the chunk the sums of its last cell go
+// through one segmented scan across the wavefront (runs of lanes in the same
+// cell, which is what sorted particles give) so that only a run's last lane
+// adds them.  Window and HBM additions are exact fixed-point integer atomics
+// (FP64 in the FP64 build).
+// After a move (no sort since) the cells change from particle to particle
+// and the additions degrade to one per particle.  The window covers the
+// item's deposit footprint and is added to HBM once per item.  The window of P1/P2
+// is the item's cell box at x^n; P3's is that box widened by the half-step
+// displacement range from signed per-axis velocity bounds (reduced by the
+// previous P3).  Particles outside the window deposit straight to HBM
+// (counted as misses).
+//
+// Right after a sort, P2 and P3 gather the 6-component node source
+// S = (ajc - upc, btc) from an LDS copy of the item's node box (one 32-byte
+// record per node), filled with coalesced loads when the item starts; on
+// later steps the boxes have drifted apart and the gather reads S through
+// the caches.
 #include "particle_math.h"
 
 using namespace pm;
@@ -41,19 +51,10 @@ __device__ __forceinline__ int unwrap(int c, int centre, int nc) {
 }
 
 // window accumulation: exact fixed point in the FP32 build, FP64 otherwise
-__device__ __forceinline__ void win_add(unsigned long long* a, int i, float v) {
-  atomicAdd(a + i, to_fix(v));
+__device__ __forceinline__ void win_add(unsigned long long* a, int i, float v, int bits) {
+  atomicAdd(a + i, to_fix(v, bits));
 }
-__device__ __forceinline__ void win_add(double* a, int i, double v) { atomicAdd(a + i, v); }
-
-__device__ __forceinline__ int wave_min(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return v;
-}
+__device__ __forceinline__ void win_add(double* a, int i, double v, int) { atomicAdd(a + i, v); }
 
 // ---- segmented inclusive scan over the 64 lanes of a wavefront ----
 // Lane L belongs to the run starting at lane s(L).  Hillis-Steele inside each
@@ -97,7 +98,19 @@ __device__ __forceinline__ T segscan(T v, const SegCond& c) {
   return v;
 }
 
-template <typename R, int MODE>
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// KP particles per lane: PASS_KP right after a sort (cells in runs), 1
+// otherwise (the cells of neighbouring particles differ; the scan across the
+// wavefront then merges neighbours that still share a cell)
+template <typename R, int MODE, int KP>
 __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
                                                   const R* __restrict__ S,
                                                   typename AccOf<R>::T* __restrict__ gacc,
@@ -108,8 +121,13 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
                                                   const int* __restrict__ vb, int* vb_next,
                                                   unsigned long long* dcnt) {
   constexpr int M = NMom<MODE>::value;
+  // the LDS gather tile pays off while the item's box is compact (right
+  // after a sort); drifted boxes gather from HBM/L2
+  constexpr bool LTILE = MODE != 1 && sizeof(R) == 4 && KP > 1;
+  constexpr int GM0 = MODE == 2 ? 1 : 0;  // moment index of window moment 0
   using A = typename AccOf<R>::T;
   __shared__ A lacc[WIN_VALUES];
+  __shared__ float4 gtile[LTILE ? 2 * GTILE_NODES : 1];
   __shared__ int sred[12];
 
   const int tid = threadIdx.x;
@@ -140,6 +158,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     if (i0 >= n_total) return;
     i1 = i0 + ITEM_MAX < n_total ? i0 + ITEM_MAX : n_total;
   }
+  const bool boxed = windowed;
   int lo_m[3] = {0, 0, 0}, hi_m[3] = {0, 0, 0};
   if (MODE == 3 && windowed) {
     // the cell shift of a half move by d in [dlo, dhi] cells lies in
@@ -158,44 +177,67 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       }
     }
   }
-  int u0[3], W[3], cen[3];
+  int u0[3], W[3], cen[3], T[3];
   for (int a = 0; a < 3; a++) {
     u0[a] = bl[a] + lo_m[a];
     W[a] = bh[a] + hi_m[a] - u0[a] + 2;
     cen[a] = (bl[a] + bh[a]) >> 1;
+    T[a] = bh[a] - bl[a] + 2;  // gather tile: nodes bl .. bh+1
   }
   int64_t Wn64 = (int64_t)W[0] * W[1] * W[2];
   if (windowed && (W[0] < 2 || W[1] < 2 || W[2] < 2 || Wn64 * M > WIN_VALUES)) windowed = false;
   const int Wn = windowed ? (int)Wn64 : 0;
+  const int Tn = T[0] * T[1] * T[2];
+  const bool tiled = LTILE && boxed && Tn <= GTILE_NODES;
   if (windowed)
     for (int q = tid; q < Wn * M; q += PASS_WG) lacc[q] = (A)0;
+  if constexpr (LTILE) {
+    if (tiled)
+      for (int q = tid; q < Tn; q += PASS_WG) {
+        int a = q % T[0], b = (q / T[0]) % T[1], c = q / (T[0] * T[1]);
+        int64_t node = gidx(g, node_of(bl[0] + a, g.ncx), node_of(bl[1] + b, g.ncy),
+                            node_of(bl[2] + c, g.ncz));
+        gtile[2 * q] = make_float4(S[node], S[node + g.nn], S[node + 2 * g.nn],
+                                   S[node + 3 * g.nn]);
+        gtile[2 * q + 1] = make_float4(S[node + 4 * g.nn], S[node + 5 * g.nn], 0.f, 0.f);
+      }
+  }
   if (MODE == 3 && tid < 12) sred[tid] = (tid % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;
   __syncthreads();
 
   const int coff[8] = {0, 1, W[0] * W[1], 1 + W[0] * W[1], W[0], 1 + W[0],
                        W[0] + W[0] * W[1], 1 + W[0] + W[0] * W[1]};
+  const int toff[8] = {0, 1, T[0] * T[1], 1 + T[0] * T[1], T[0], 1 + T[0],
+                       T[0] + T[0] * T[1], 1 + T[0] + T[0] * T[1]};
   int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
   int bmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
   float vlo[3] = {3e38f, 3e38f, 3e38f}, vhi[3] = {-3e38f, -3e38f, -3e38f};
   unsigned long long misses = 0;
 
-  // The trip count is uniform across the workgroup (the segmented scan needs
-  // every lane of a wavefront).
-  for (int64_t base = i0; base < i1; base += PASS_WG) {
-    const int64_t p = base + tid;
-    const bool valid = p < i1;
-    uint32_t c = valid ? P.cell[p] : 0u;
-    R w8[8], q[M];
-    int ux = 0, uy = 0, uz = 0;
-    R f0 = 0, f1 = 0, f2 = 0, vn[3] = {0, 0, 0};
-    if (valid) {
+  for (int64_t chunk = i0; chunk < i1; chunk += (int64_t)PASS_WG * KP) {
+    const int64_t pb = chunk + (int64_t)tid * KP;
+    int cur = -1;   // window cell of the running sums acc
+    int hkey = -1;  // window cell of hd: the lane's first cell, once the lane left it
+    R acc[8][M], hd[8][M];
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        acc[cc][m] = (R)0;
+        if (KP > 1) hd[cc][m] = (R)0;
+      }
+#pragma unroll 1
+    for (int k = 0; k < KP; k++) {
+      const int64_t p = pb + k;
+      if (p >= i1) break;
+      const uint32_t c = P.cell[p];
       const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
-      f0 = P.fx[p]; f1 = P.fy[p]; f2 = P.fz[p];
+      R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
       const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
+      R w8[8], q[M], vn[3] = {0, 0, 0};
       weights8(f0, f1, f2, w8);
-      ux = unwrap(ci, cen[0], g.ncx);
-      uy = unwrap(cj, cen[1], g.ncy);
-      uz = unwrap(ck, cen[2], g.ncz);
+      int ux = unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
+          uz = unwrap(ck, cen[2], g.ncz);
       const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
       if constexpr (MODE == 1) {
         q[0] = r;
@@ -203,11 +245,25 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         q[2] = r * ((R)1.5 * v[1] - (R)0.5 * P.v1y[p]);
         q[3] = r * ((R)1.5 * v[2] - (R)0.5 * P.v1z[p]);
       } else {
-        R A[3], B[3], E[3], vm[3], vpl[3];
-        gather6(g, S, gidx(g, ci, cj, ck), w8, A, B);
+        R Ag[3], Bg[3], E[3], vm[3], vpl[3];
+        const int tx = ux - bl[0], ty = uy - bl[1], tz = uz - bl[2];
+        if (tiled && (unsigned)tx < (unsigned)(T[0] - 1) && (unsigned)ty < (unsigned)(T[1] - 1) &&
+            (unsigned)tz < (unsigned)(T[2] - 1)) {
+          const int tb = tx + T[0] * (ty + T[1] * tz);
+#pragma unroll
+          for (int m = 0; m < 3; m++) Ag[m] = Bg[m] = (R)0;
+#pragma unroll
+          for (int cc = 0; cc < 8; cc++) {
+            const float4 a = gtile[2 * (tb + toff[cc])], b = gtile[2 * (tb + toff[cc]) + 1];
+            Ag[0] += w8[cc] * a.x; Ag[1] += w8[cc] * a.y; Ag[2] += w8[cc] * a.z;
+            Bg[0] += w8[cc] * a.w; Bg[1] += w8[cc] * b.x; Bg[2] += w8[cc] * b.y;
+          }
+        } else {
+          gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
+        }
         const R mr = P.mr ? (R)P.mr[p] : (R)1;
-        ep_of(A, B, mr, E);
-        rotate(K.dt, v, E, B, mr, vm, vpl);
+        ep_of(Ag, Bg, mr, E);
+        rotate(K.dt, v, E, Bg, mr, vm, vpl);
         if constexpr (MODE == 2) {
 #pragma unroll
           for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
@@ -228,46 +284,51 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
           q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
         }
       }
-    }
-    // deposit key: the window cell, or a distinct negative number
-    const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
-    const bool inw = valid && windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
-                     ly < W[1] - 1 && lz < W[2] - 1;
-    const int key = inw ? lx + W[0] * (ly + W[1] * lz) : -1 - lane;
-    if (valid && !inw) {
-      misses++;
+      const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
+      const bool inw = windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
+                       ly < W[1] - 1 && lz < W[2] - 1;
+      if (inw) {
+        const int key = lx + W[0] * (ly + W[1] * lz);
+        if (KP > 1 && key != cur) {
+          if (cur >= 0) {
+            if (hkey < 0) {  // keep the first cell's sums for the end of the chunk
+              hkey = cur;
 #pragma unroll
-      for (int cc = 0; cc < 8; cc++) {
-        int64_t node = gidx(g, node_of(ux + (cc & 1), g.ncx),
-                            node_of(uy + ((cc >> 2) & 1), g.ncy),
-                            node_of(uz + ((cc >> 1) & 1), g.ncz));
+              for (int cc = 0; cc < 8; cc++)
 #pragma unroll
-        for (int m = 0; m < M; m++) acc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
-      }
-    }
-    // runs of equal keys on neighbouring lanes; the last lane of a run adds
-    // the run's segmented sums into the window
-    const int prevkey = __shfl_up(key, 1);
-    const unsigned long long hm = __ballot(lane == 0 || prevkey != key);
-    const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
-    const int s = 63 - __clzll(hm & le);
-    const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
-    if (__ballot(inw)) {
-      const SegCond sc = seg_cond(lane, s);
-      R sv[8][M];
+                for (int m = 0; m < M; m++) {
+                  hd[cc][m] = acc[cc][m];
+                  acc[cc][m] = (R)0;
+                }
+            } else {  // a cell in the middle of the lane's particles
 #pragma unroll
-      for (int cc = 0; cc < 8; cc++)
+              for (int cc = 0; cc < 8; cc++)
 #pragma unroll
-        for (int m = 0; m < M; m++) sv[cc][m] = segscan(inw ? w8[cc] * q[m] : (R)0, sc);
-      if (tail && inw) {
+                for (int m = 0; m < M; m++) {
+                  win_add(lacc, cur + coff[cc] + m * Wn, acc[cc][m], fix_bits(GM0 + m));
+                  acc[cc][m] = (R)0;
+                }
+            }
+          }
+        }
+        cur = key;
 #pragma unroll
         for (int cc = 0; cc < 8; cc++)
 #pragma unroll
-          for (int m = 0; m < M; m++) win_add(lacc, key + coff[cc] + m * Wn, sv[cc][m]);
+          for (int m = 0; m < M; m++) acc[cc][m] += w8[cc] * q[m];
+      } else {
+        misses++;
+#pragma unroll
+        for (int cc = 0; cc < 8; cc++) {
+          int64_t node = gidx(g, node_of(ux + (cc & 1), g.ncx),
+                              node_of(uy + ((cc >> 2) & 1), g.ncy),
+                              node_of(uz + ((cc >> 1) & 1), g.ncz));
+#pragma unroll
+          for (int m = 0; m < M; m++)
+            acc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m], fix_bits(GM0 + m));
+        }
       }
-    }
-    if constexpr (MODE == 3) {
-      if (valid) {
+      if constexpr (MODE == 3) {
         ux += half_move(f0, vn[0], K.s[0]);
         uy += half_move(f1, vn[1], K.s[1]);
         uz += half_move(f2, vn[2], K.s[2]);
@@ -277,6 +338,35 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
         bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
       }
+    }
+    // the first cell of a lane that moved on is its own
+    if (KP > 1 && hkey >= 0) {
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+        for (int m = 0; m < M; m++)
+          win_add(lacc, hkey + coff[cc] + m * Wn, hd[cc][m], fix_bits(GM0 + m));
+    }
+    // the last cells: runs of lanes with the same cell are summed across the
+    // wavefront first (sorted particles), and only each run's last lane adds
+    // into the window; lanes that share no cell with a neighbour add directly
+    const int prevkey = __shfl_up(cur, 1);
+    const unsigned long long hm = __ballot(lane == 0 || prevkey != cur);
+    if (hm != ~0ull) {
+      const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+      const SegCond sc = seg_cond(lane, 63 - __clzll(hm & le));
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[cc][m] = segscan(acc[cc][m], sc);
+    }
+    const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
+    if (tail && cur >= 0) {
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+        for (int m = 0; m < M; m++)
+          win_add(lacc, cur + coff[cc] + m * Wn, acc[cc][m], fix_bits(GM0 + m));
     }
   }
   if (misses) atomicAdd(dcnt, misses);
@@ -353,16 +443,31 @@ int pl_pass(swap_ctx* c, int mode) {
     const R* S = (const R*)c->S;
     using A = typename AccOf<R>::T;
     A* acc = (A*)c->macc;
-    if (mode == 1)
-      hipLaunchKernelGGL((k_pass<R, 1>), grid, blk, 0, c->stream, c->g, P, K, S, acc, c->items,
-                         dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next, c->dcnt);
+    const bool runs = c->n_sorted == c->n && c->steps_since_sort == 0;
+    if (mode == 1 && runs)
+      hipLaunchKernelGGL((k_pass<R, 1, PASS_KP>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
+                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
+                         c->dcnt);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_pass<R, 1, 1>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
+                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
+                         c->dcnt);
+    else if (mode == 2 && runs)
+      hipLaunchKernelGGL((k_pass<R, 2, PASS_KP>), grid, blk, 0, c->stream, c->g, P, K, S,
+                         acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
+                         c->vb_next, c->dcnt);
     else if (mode == 2)
-      hipLaunchKernelGGL((k_pass<R, 2>), grid, blk, 0, c->stream, c->g, P, K, S, acc + c->g.nn,
+      hipLaunchKernelGGL((k_pass<R, 2, 1>), grid, blk, 0, c->stream, c->g, P, K, S,
+                         acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
+                         c->vb_next, c->dcnt);
+    else if (runs)
+      hipLaunchKernelGGL((k_pass<R, 3, PASS_KP>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
                          c->dcnt);
     else
-      hipLaunchKernelGGL((k_pass<R, 3>), grid, blk, 0, c->stream, c->g, P, K, S, acc, c->items,
-                         dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next, c->dcnt);
+      hipLaunchKernelGGL((k_pass<R, 3, 1>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
+                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
+                         c->dcnt);
   };
   if (c->prec == 64) launch(0.0);
   else launch(0.0f);

@@ -61,15 +61,19 @@ SWAP_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 SWAP_HD double u24(uint32_t r) { return (double)(r >> 8) * (1.0 / 16777216.0); }
 
 // ---- moment accumulation ----
-// FP32 deposits are summed exactly: each FP32 contribution (a particle's, or
-// a run's FP32 partial sum) is converted to a 64-bit integer count of
-// 2^-FIX_BITS units and added with integer atomics.  Integer addition is
-// exact and associative, so the per-node sums carry no accumulation error and
-// do not depend on the order in which atomics are served; the only rounding
-// is the FP32 -> fixed-point conversion of each contribution (2^-31
-// absolute, far below FP32's relative precision for the magnitudes here:
-// sums stay below 2^63 / 2^30 = 8.6e9).  The FP64 build accumulates doubles.
-constexpr int FIX_BITS = 30;
+// FP32 deposits are summed exactly: each FP32 contribution (a particle's
+// product w q, or an FP32 partial sum over a few neighbouring particles) is
+// rounded to a 64-bit integer count of 2^-bits units and added with integer
+// atomics, in LDS and in HBM.  Integer addition is exact and associative, so
+// the node sums carry no accumulation error and do not depend on the order
+// in which atomics are served; the only rounding is that of each
+// contribution to the fixed grid: 2^-26 for the density moment (weights
+// times 1/beta_p), 2^-18 km/s for the velocity moments -- below 1e-9 of the
+// node sums of the workloads here.  A contribution below 2^31 units (|w r|
+// < 32, |w r v| < 8192 km/s) converts in five operations; larger ones take
+// an exact 64-bit path.  The FP64 build accumulates doubles.
+constexpr int FIX_BITS_N = 26, FIX_BITS_V = 18;
+SWAP_HD constexpr int fix_bits(int moment) { return moment == 0 ? FIX_BITS_N : FIX_BITS_V; }
 template <typename R>
 struct AccOf {
   using T = unsigned long long;  // FP32 build: fixed point
@@ -78,19 +82,26 @@ template <>
 struct AccOf<double> {
   using T = double;
 };
-__device__ __forceinline__ unsigned long long to_fix(float v) {
-  return (unsigned long long)(long long)rintf(v * (float)(1 << FIX_BITS));
+__device__ __noinline__ unsigned long long to_fix_wide(float x) {
+  return (unsigned long long)(long long)x;
 }
-__device__ __forceinline__ float from_fix(unsigned long long a) {
-  return (float)(long long)a * (1.0f / (float)(1 << FIX_BITS));
+// round(v 2^bits) as a two's-complement 64-bit integer
+__device__ __forceinline__ unsigned long long to_fix(float v, int bits) {
+  const float x = rintf(v * __int_as_float((127 + bits) << 23));
+  if (__builtin_expect(fabsf(x) < 2147483648.0f, 1))
+    return (unsigned long long)(long long)(int)x;
+  return to_fix_wide(x);
 }
-__device__ __forceinline__ void acc_add(unsigned long long* a, int64_t i, float v) {
-  if (v != 0.f) atomicAdd(a + i, to_fix(v));
+__device__ __forceinline__ double from_fix(unsigned long long a, int bits) {
+  return (double)(long long)a * (double)__int_as_float((127 - bits) << 23);
 }
-__device__ __forceinline__ void acc_add(double* a, int64_t i, double v) {
+__device__ __forceinline__ void acc_add(unsigned long long* a, int64_t i, float v, int bits) {
+  if (v != 0.f) atomicAdd(a + i, to_fix(v, bits));
+}
+__device__ __forceinline__ void acc_add(double* a, int64_t i, double v, int) {
   if (v != 0.0) atomicAdd(a + i, v);
 }
-__device__ __forceinline__ float acc_read(const unsigned long long* a, int64_t i) {
-  return from_fix(a[i]);
+__device__ __forceinline__ double acc_read(const unsigned long long* a, int64_t i, int bits) {
+  return from_fix(a[i], bits);
 }
-__device__ __forceinline__ double acc_read(const double* a, int64_t i) { return a[i]; }
+__device__ __forceinline__ double acc_read(const double* a, int64_t i, int) { return a[i]; }

@@ -11,8 +11,9 @@ constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 4;
 constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
 constexpr int ITEM_MAX = 32768;
 constexpr int PASS_WG = 512;   // threads per particle-pass workgroup
-constexpr int PASS_KP = 8;     // consecutive particles per lane
-constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values
+constexpr int PASS_KP = 4;     // consecutive particles per lane
+constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values (64 KB)
+constexpr int GTILE_NODES = 512;  // LDS gather tile, nodes of 32 B (16 KB)
 
 struct Item {
   int32_t start, end, tile, pad;
