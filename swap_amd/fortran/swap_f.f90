@@ -293,6 +293,14 @@ contains
     Ni_tot = n
   end subroutine swap_ref_load
 
+  ! device particles -> host arrays (Ni_max,3), rows 1..Ni_tot
+  subroutine swap_ref_get_particles(xp, vp, vp1)
+    real, intent(inout) :: xp(:,:), vp(:,:), vp1(:,:)
+    call xp_get(xp)
+    call pget(SWAP_P_VP, vp)
+    call pget(SWAP_P_VP1, vp1)
+  end subroutine swap_ref_get_particles
+
   ! resident time stepping: maind.f:385-479 for nsteps steps
   subroutine swap_ref_step(nsteps)
     integer, intent(in) :: nsteps

@@ -56,6 +56,22 @@ def test_python_binding_covers_the_header():
     assert not unbound, unbound
 
 
+def test_fortran_shim_binds_declared_symbols_and_reference_names():
+    """swap_amd/fortran/swap_f.f90 binds only symbols the header declares and
+    gives every hot-path routine of the reference under its own name"""
+    src = open(os.path.join(ROOT, "swap_amd", "fortran", "swap_f.f90")).read().lower()
+    bound = set(re.findall(r'bind\(c,\s*name="(swap_\w+)"\)', src))
+    declared = {n.lower() for n in declared_functions()}
+    assert bound and bound <= declared, bound - declared
+    subs = set(re.findall(r"^\s*subroutine\s+(\w+)", src, flags=re.M))
+    for r in ("get_interp_weights", "update_np", "update_up", "extrapol_up", "get_ep",
+              "get_vplus_vminus", "improve_up", "get_vp_final", "move_ion_half", "curlb",
+              "curle", "predict_b", "correct_b", "f_update_tlev", "check_time_step",
+              "edge_to_center", "edge_to_face", "face_to_center", "periodic",
+              "periodic_scalar"):
+        assert r in subs, r
+
+
 def c_layout(struct, fields):
     body = "\n".join('  printf("%%s %%zu\\n", "%s", offsetof(%s, %s));' % (f, struct, f)
                      for f in fields)
