@@ -367,6 +367,53 @@ def test_curlB_of_gradient_vanishes_and_density_scaling():
     assert np.allclose(aj[2, 1:-1, 1:-1, 1:-1], want, rtol=1e-10)
 
 
+def _node_coords(o):
+    d = PAR["dx"]
+    z, y, x = np.meshgrid(np.arange(1, o.nz + 1) * d, np.arange(1, o.ny + 1) * d,
+                          np.arange(1, o.nx + 1) * d, indexing="ij")
+    return x, y, z, d
+
+
+LIN = np.array([[0.3, 1.1, -0.7, 0.4], [-1.2, 0.5, 0.9, -0.3], [0.8, -0.6, 0.2, 1.3]])
+
+
+def _lin(c, x, y, z):
+    a = LIN[c]
+    return a[0] + (a[1] * x + a[2] * y + a[3] * z) / 1e3
+
+
+def test_interpolations_exact_on_linear_fields():
+    """grid_interp.f: B on faces (x-component at (x_i, y_j+d/2, z_k+d/2) ...)
+    -> nodes (edge_to_center :35-51); edge-centred vectors (x at x_i+d/2) ->
+    nodes (face_to_center :163-170); nodes -> half-way (edge_to_face
+    :119-127): exact for linear fields away from the wrapped ghosts"""
+    o = make()
+    x, y, z, d = _node_coords(o)
+    h = d / 2
+    bt = np.ascontiguousarray(np.stack([_lin(0, x, y + h, z + h), _lin(1, x + h, y, z + h),
+                                        _lin(2, x + h, y + h, z)]))
+    btc = o.grid()
+    o.edge_to_center(bt, btc)
+    want = np.stack([_lin(c, x, y, z) for c in range(3)])
+    s = (slice(None), slice(2, -1), slice(2, -1), slice(2, -1))
+    assert np.allclose(btc[s], want[s], rtol=1e-12, atol=1e-12)
+    bt = np.ascontiguousarray(np.stack([_lin(0, x, y + h, z + h), _lin(1, x + h, y, z + h),
+                                        _lin(2, x + h, y + h, z)]))
+    btmf = o.grid()
+    o.edge_to_face(bt, btmf)
+    wantf = np.stack([_lin(0, x + h, y, z), _lin(1, x, y + h, z), _lin(2, x, y, z + h)])
+    s2 = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+    assert np.allclose(btmf[s2], wantf[s2], rtol=1e-12, atol=1e-12)
+    v = np.ascontiguousarray(np.stack([_lin(0, x + h, y, z), _lin(1, x, y + h, z),
+                                       _lin(2, x, y, z + h)]))
+    vc = o.grid()
+    o.face_to_center(v, vc)
+    assert np.allclose(vc[s], want[s], rtol=1e-12, atol=1e-12)
+    # outputs are periodic (boundary.f applied to the result)
+    for ax in (1, 2, 3):
+        assert np.array_equal(np.take(vc, 0, ax), np.take(vc, -2, ax))
+
+
 def test_get_E_uniform_drift():
     """uniform B, up = V, no current: E = -V x B everywhere (gutsf.f:994-1110)"""
     o = make()
