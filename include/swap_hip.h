@@ -207,8 +207,10 @@ typedef struct {
   double tau_photo;      /* s */
   double sigma_chex;     /* km^2 (chem_rates.f:817: 8e-26) */
   double mrat_pu;        /* 1/28 */
-  double beta_min;       /* smallest macro-weight beta_p */
-  int max_per_cell;      /* cap of new macro ions per cell per step */
+  double beta_min;       /* smallest beta_p of a new ion (in [0, 1]) */
+  int max_per_cell;      /* a node with n > max_per_cell new ions gets
+                            m = max(max_per_cell, ceil(beta_min n)) ions of
+                            beta_p = m/n, so the deposit is unchanged */
   double rmin;           /* cloud core cut-off radius, km */
 } swap_cloud;
 int swap_ionize_cloud(swap_ctx *ctx, int step, const swap_cloud *cloud, int64_t *injected,

@@ -68,6 +68,11 @@ def _load(width):
         "or_split_move": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
         "or_stable_sort": ([i64, vp, C.c_uint32, vp], None),
         "or_load_maxwellian": ([i64, C.c_uint64, C.c_uint32, vp, dbl, dbl, vp, vp, vp], None),
+        "or_cloud_counts": ([vp, dbl, dbl, dbl, dbl, dbl, vp, i32, C.c_uint64, C.c_uint32, vp,
+                             vp], None),
+        "or_cloud_ions": ([vp, vp, C.c_uint64, C.c_uint32, vp, vp, i64], None),
+        "or_chex": ([i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, C.c_uint64,
+                     C.c_uint32, vp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -197,6 +202,50 @@ def stable_sort(key, nkeys):
     perm = np.zeros(key.size, np.uint32)
     lib.or_stable_sort(key.size, key.ctypes.data, nkeys, perm.ctypes.data)
     return perm
+
+
+CLOUD_KEYS = ("cx", "cy", "cz", "N0", "r0", "H", "tau_photo", "rmin", "sigma_chex", "mrat_pu",
+              "beta_min")
+
+
+def cloud_counts(nc, d, beta, dt, cloud, seed, step):
+    """(count uint32[nin], beta_p float32[nin]) per interior node, i fastest"""
+    lib, _ = _load(64)
+    nc = np.asarray(nc, np.int32)
+    cl = np.array([cloud[k] for k in CLOUD_KEYS], np.float64)
+    nin = int(np.prod(nc))
+    cnt = np.zeros(nin, np.uint32)
+    bp = np.zeros(nin, np.float32)
+    lib.or_cloud_counts(nc.ctypes.data, d[0], d[1], d[2], beta, dt, cl.ctypes.data,
+                        int(cloud["max_per_cell"]), seed, step, cnt.ctypes.data, bp.ctypes.data)
+    return cnt, bp
+
+
+def cloud_ions(nc, cnt, seed, step):
+    """(cell uint32[m], frac float64[3,m]) of the new ions in node order"""
+    lib, _ = _load(64)
+    nc = np.asarray(nc, np.int32)
+    cnt = np.ascontiguousarray(cnt, np.uint32)
+    m = int(cnt.sum())
+    cell = np.zeros(m, np.uint32)
+    frac = np.zeros((3, m))
+    lib.or_cloud_ions(nc.ctypes.data, cnt.ctypes.data, seed, step, cell.ctypes.data,
+                      frac.ctypes.data, m)
+    return cell, frac
+
+
+def chex(cell, f, v, d, dt, cloud, seed, step):
+    """charge-exchange decision (uint8) per ion; f, v float32 [3, n]"""
+    lib, _ = _load(64)
+    cell = np.ascontiguousarray(cell, np.uint32)
+    f = [np.ascontiguousarray(a, np.float32) for a in f]
+    v = [np.ascontiguousarray(a, np.float32) for a in v]
+    cl = np.array([cloud[k] for k in CLOUD_KEYS], np.float64)
+    hit = np.zeros(cell.size, np.uint8)
+    lib.or_chex(cell.size, cell.ctypes.data, f[0].ctypes.data, f[1].ctypes.data,
+                f[2].ctypes.data, v[0].ctypes.data, v[1].ctypes.data, v[2].ctypes.data, d[0],
+                d[1], d[2], dt, cl.ctypes.data, seed, step, hit.ctypes.data)
+    return hit
 
 
 def load_maxwellian(n, seed, nc, vsw, vth, stream=1):

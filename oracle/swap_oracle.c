@@ -805,3 +805,90 @@ void or_load_maxwellian(int64_t n, uint64_t seed, uint32_t stream, const int nc[
     }
   }
 }
+
+/* pickup-ion cloud: photoionization counts and new ions per interior node,
+   and charge exchange, following Ionize_pluto_mp (chem_rates.f:290-503) and
+   res_chex (chem_rates.f:805-843) with the neutral profile
+   n_n = N0 (r0/r)^2 exp(-r/H), r >= rmin.  cl = {cx, cy, cz, N0, r0, H,
+   tau_photo, rmin, sigma_chex, mrat_pu, beta_min}; grid positions q(i) = i d.
+   A node with more than max_per_cell new ions gets m = max(max_per_cell,
+   ceil(beta_min n)) ions (at most n) of weight beta_p = m/n. */
+static double or_cloud_nn(const double *cl, double x, double y, double z) {
+  double r = sqrt((x - cl[0]) * (x - cl[0]) + (y - cl[1]) * (y - cl[1]) +
+                  (z - cl[2]) * (z - cl[2]));
+  r = r > cl[7] ? r : cl[7];
+  double a = cl[4] / r;
+  return cl[3] * a * a * exp(-r / cl[5]);
+}
+
+void or_cloud_counts(const int nc[3], double dx, double dy, double dz, double beta, double dt,
+                     const double *cl, int max_per_cell, uint64_t seed, uint32_t step,
+                     uint32_t *cnt, float *bp) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  double rate = dx * dy * dz * beta * dt / cl[6]; /* chem_rates.f:352 */
+  int64_t q = 0;
+  for (int k = 1; k <= nc[2]; k++)
+    for (int j = 1; j <= nc[1]; j++)
+      for (int i = 1; i <= nc[0]; i++, q++) {
+        double npofr = rate * or_cloud_nn(cl, (i + 1) * dx, (j + 1) * dy, (k + 1) * dz);
+        uint32_t n;
+        if (npofr >= 1.0) {
+          n = npofr < 4.0e9 ? (uint32_t)llround(npofr) : 4000000000u; /* :354 */
+        } else {
+          uint32_t ctr[4] = {(uint32_t)q, (uint32_t)((uint64_t)q >> 32), 0x38000u + step,
+                             0xFFFFFFFFu}, r[4];
+          or_philox4x32_10(ctr, key, r);
+          n = npofr > or_u24(r[0]) ? 1u : 0u; /* :391 */
+        }
+        float w = 1.0f;
+        if (n > (uint32_t)max_per_cell) {
+          double mm = (double)max_per_cell, lo = ceil(cl[10] * (double)n);
+          if (lo > mm) mm = lo < (double)n ? lo : (double)n;
+          w = (float)(mm / (double)n);
+          n = (uint32_t)mm;
+        }
+        cnt[q] = n;
+        bp[q] = w;
+      }
+}
+
+void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t step,
+                   uint32_t *cell, double *frac, int64_t m) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  int64_t q = 0, p = 0;
+  for (int k = 1; k <= nc[2]; k++)
+    for (int j = 1; j <= nc[1]; j++)
+      for (int i = 1; i <= nc[0]; i++, q++)
+        for (uint32_t l = 0; l < cnt[q]; l++, p++) {
+          uint32_t ctr[4] = {(uint32_t)q, (uint32_t)((uint64_t)q >> 32), 0x30000u + step, l},
+                   r[4];
+          or_philox4x32_10(ctr, key, r);
+          const int s[3] = {i, j, k};
+          int ijk[3];
+          for (int a = 0; a < 3; a++) {
+            double t = (double)s[a] + or_u24(r[a]) - 0.5; /* chem_rates.f:363-365 */
+            if (t > (double)nc[a]) t -= (double)nc[a];
+            double ci = ceil(t) - 1.0;
+            ijk[a] = (int)ci;
+            frac[a * m + p] = t - ci;
+          }
+          cell[p] = (uint32_t)ijk[0] | ((uint32_t)ijk[1] << 11) | ((uint32_t)ijk[2] << 21);
+        }
+}
+
+void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, const float *fz,
+             const float *vx, const float *vy, const float *vz, double dx, double dy, double dz,
+             double dt, const double *cl, uint64_t seed, uint32_t step, uint8_t *hit) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t p = 0; p < n; p++) {
+    uint32_t c = cell[p];
+    double x = ((double)(c & 2047u) + 1.0 + (double)fx[p]) * dx;
+    double y = ((double)((c >> 11) & 1023u) + 1.0 + (double)fy[p]) * dy;
+    double z = ((double)(c >> 21) + 1.0 + (double)fz[p]) * dz;
+    double a = vx[p], b = vy[p], e = vz[p];
+    double prob = dt * or_cloud_nn(cl, x, y, z) * cl[8] * sqrt(a * a + b * b + e * e);
+    uint32_t ctr[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), 0x40000u + step, 0u}, r[4];
+    or_philox4x32_10(ctr, key, r);
+    hit[p] = or_u24(r[0]) < prob ? 1 : 0;
+  }
+}

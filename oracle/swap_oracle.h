@@ -126,4 +126,18 @@ void or_stable_sort(int64_t n, const uint32_t *key, uint32_t nkeys,
 void or_load_maxwellian(int64_t n, uint64_t seed, uint32_t stream, const int nc[3], double vsw,
                         double vth, uint32_t *cell, double *frac, double *v);
 
+/* pickup-ion cloud (chem_rates.f:290-503, 805-843 restated with the
+   build's RNG): cl = {cx, cy, cz, N0, r0, H, tau_photo, rmin, sigma_chex,
+   mrat_pu, beta_min}.  Counts and weights per interior node (i fastest), the new
+   ions' split positions in node order (m ions, frac [3][m]), and the charge
+   exchange decision per existing ion. */
+void or_cloud_counts(const int nc[3], double dx, double dy, double dz, double beta, double dt,
+                     const double *cl, int max_per_cell, uint64_t seed, uint32_t step,
+                     uint32_t *cnt, float *bp);
+void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t step,
+                   uint32_t *cell, double *frac, int64_t m);
+void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, const float *fz,
+             const float *vx, const float *vy, const float *vz, double dx, double dy, double dz,
+             double dt, const double *cl, uint64_t seed, uint32_t step, uint8_t *hit);
+
 #endif

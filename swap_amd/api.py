@@ -353,6 +353,10 @@ class Swap:
         return n.value
 
     def ionize_cloud(self, step, cloud):
+        """photoionization + charge exchange of a neutral cloud; cloud is a
+        Cloud or a dict of its fields; returns (injected, charge exchanges)"""
+        if isinstance(cloud, dict):
+            cloud = Cloud(**cloud)
         n, x = C.c_int64(), C.c_int64()
         self._ck(self.lib.swap_ionize_cloud(self.h, step, C.byref(cloud), C.byref(n),
                                             C.byref(x)), "ionize_cloud")

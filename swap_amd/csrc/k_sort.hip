@@ -390,6 +390,10 @@ int scan(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out) {
 
 }  // namespace
 
+int so_scan_u32(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out) {
+  return scan(c, n, in, out);
+}
+
 int so_sort(swap_ctx* c) {
   const int64_t n = c->n, nk = c->nkeys;
   const int ntx = c->ntile[0], nty = c->ntile[1];

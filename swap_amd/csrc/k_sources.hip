@@ -299,12 +299,184 @@ int sr_diagnostics(swap_ctx* c, swap_diag* d) {
   return 0;
 }
 
+namespace {
+
+// ---- spherical neutral cloud: photoionization and charge exchange ----
+// Neutral density n_n(r) = N0 (r0/r)^2 exp(-r/H), with r held at >= rmin
+// (the 1/r^2 form of Ionize_pluto_mp, chem_rates.f:351, with the exp(-r/H)
+// photoionization shell of chem_rates.f:114-116).
+struct CloudDev {
+  double cx, cy, cz, N0, r0, H, rmin, sigma, beta_min;
+  double rate;  // vol beta dt / tau_photo: new ions per unit neutral density
+  float mrat;
+  int max_per_cell;
+};
+__device__ __forceinline__ double cloud_nn(const CloudDev& cl, double x, double y, double z) {
+#pragma clang fp contract(off)  // the same roundings as the CPU restatement
+  double r = sqrt((x - cl.cx) * (x - cl.cx) + (y - cl.cy) * (y - cl.cy) +
+                  (z - cl.cz) * (z - cl.cz));
+  r = r > cl.rmin ? r : cl.rmin;
+  double a = cl.r0 / r;
+  return cl.N0 * a * a * exp(-r / cl.H);
+}
+
+// per interior node (i fastest): the node's new-ion count and weight.  The
+// expected number npofr = vol beta n_n dt / tau (chem_rates.f:352); at least
+// one: nint(npofr) ions (:354); below one: one ion with probability npofr
+// (:391).  More than max_per_cell become max_per_cell heavier ions
+// (beta_p = max/count), so the deposited charge is unchanged.
+__global__ void k_cloud_count(Geo g, double dx, double dy, double dz, CloudDev cl, uint32_t k0,
+                              uint32_t k1, uint32_t stream, uint32_t* cnt, float* bp) {
+#pragma clang fp contract(off)
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t nint_nodes = (int64_t)g.ncx * g.ncy * g.ncz;
+  if (q >= nint_nodes) return;
+  int i = 1 + (int)(q % g.ncx), j = 1 + (int)((q / g.ncx) % g.ncy),
+      k = 1 + (int)(q / ((int64_t)g.ncx * g.ncy));
+  double npofr = cl.rate * cloud_nn(cl, (i + 1) * dx, (j + 1) * dy, (k + 1) * dz);
+  uint32_t n;
+  if (npofr >= 1.0) {
+    n = npofr < 4.0e9 ? (uint32_t)llround(npofr) : 4000000000u;
+  } else {
+    uint32_t r[4] = {(uint32_t)q, (uint32_t)((uint64_t)q >> 32), stream, 0xFFFFFFFFu};
+    philox4x32_10(r, k0, k1);
+    n = npofr > u24(r[0]) ? 1u : 0u;
+  }
+  float w = 1.0f;
+  if (n > (uint32_t)cl.max_per_cell) {
+    // m heavier ions carry the n: beta_p = m/n, with m raised so beta_p >= beta_min
+    double m = (double)cl.max_per_cell, lo = ceil(cl.beta_min * (double)n);
+    if (lo > m) m = lo < (double)n ? lo : (double)n;
+    w = (float)(m / (double)n);
+    n = (uint32_t)m;
+  }
+  cnt[q] = n;
+  bp[q] = w;
+}
+
+// the new ions of each node at rest, uniform in the node's dual cell
+// (xp = q(i) + (u - 0.5) dx, chem_rates.f:363-365), mrat = 1/m_pu
+template <typename R>
+__global__ void k_cloud_fill(Geo g, int64_t n0, PDev<R> P, CloudDev cl, uint32_t k0, uint32_t k1,
+                             uint32_t stream, const uint32_t* cnt, const uint32_t* off,
+                             const float* bpn, float* bp, float* mr, uint32_t* id) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t nint_nodes = (int64_t)g.ncx * g.ncy * g.ncz;
+  if (q >= nint_nodes) return;
+  const int s[3] = {1 + (int)(q % g.ncx), 1 + (int)((q / g.ncx) % g.ncy),
+                    1 + (int)(q / ((int64_t)g.ncx * g.ncy))};
+  const int nc[3] = {g.ncx, g.ncy, g.ncz};
+  for (uint32_t l = 0; l < cnt[q]; l++) {
+    uint32_t r[4] = {(uint32_t)q, (uint32_t)((uint64_t)q >> 32), stream, l};
+    philox4x32_10(r, k0, k1);
+    int ijk[3];
+    double f[3];
+    for (int a = 0; a < 3; a++) {
+      double t = (double)s[a] + u24(r[a]) - 0.5;  // cell coordinate, in (0.5, nc + 0.5)
+      if (t > (double)nc[a]) t -= (double)nc[a];
+      double ci = ceil(t) - 1.0;
+      ijk[a] = (int)ci;
+      f[a] = t - ci;
+    }
+    int64_t p = n0 + off[q] + l;
+    P.cell[p] = pack_cell(ijk[0], ijk[1], ijk[2]);
+    P.fx[p] = (R)f[0]; P.fy[p] = (R)f[1]; P.fz[p] = (R)f[2];
+    P.vx[p] = (R)0; P.vy[p] = (R)0; P.vz[p] = (R)0;
+    P.v1x[p] = (R)0; P.v1y[p] = (R)0; P.v1z[p] = (R)0;
+    bp[p] = bpn[q];
+    mr[p] = cl.mrat;
+    if (id) id[p] = (uint32_t)p;
+  }
+}
+
+// res_chex (chem_rates.f:805-843) with the cloud's density at the ion:
+// probability dt n_n sigma |v|; a hit stops the ion (v = 0) and makes it a
+// cloud ion (mrat = 1/m_pu)
+template <typename R>
+__global__ void k_chex(int64_t n, PDev<R> P, double dx, double dy, double dz, double dt,
+                       CloudDev cl, uint32_t k0, uint32_t k1, uint32_t stream, float* mr,
+                       unsigned long long* count) {
+#pragma clang fp contract(off)
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t c = P.cell[p];
+  double x = ((double)cell_i(c) + 1.0 + (double)P.fx[p]) * dx;
+  double y = ((double)cell_j(c) + 1.0 + (double)P.fy[p]) * dy;
+  double z = ((double)cell_k(c) + 1.0 + (double)P.fz[p]) * dz;
+  double vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
+  double prob = dt * cloud_nn(cl, x, y, z) * cl.sigma * sqrt(vx * vx + vy * vy + vz * vz);
+  uint32_t r[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};
+  philox4x32_10(r, k0, k1);
+  if (u24(r[0]) < prob) {
+    P.vx[p] = (R)0; P.vy[p] = (R)0; P.vz[p] = (R)0;
+    mr[p] = cl.mrat;
+    atomicAdd(count, 1ull);
+  }
+}
+
+}  // namespace
+
 int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* injected,
                     int64_t* chex) {
-  (void)step; (void)cl;
   *injected = 0;
   *chex = 0;
-  return set_err(c, SWAP_EINVAL, "ionize_cloud: not built yet");
+  if (!(c->cfg.flags & SWAP_SPECIES))
+    return set_err(c, SWAP_EINVAL, "ionize_cloud needs SWAP_SPECIES (beta_p, mrat arrays)");
+  if (!(cl->N0 >= 0) || !(cl->r0 > 0) || !(cl->H > 0) || !(cl->tau_photo > 0) ||
+      !(cl->rmin > 0) || cl->max_per_cell < 1 || !(cl->mrat_pu > 0) || !(cl->sigma_chex >= 0) ||
+      !(cl->beta_min >= 0 && cl->beta_min <= 1))
+    return set_err(c, SWAP_EINVAL, "ionize_cloud: bad cloud parameters");
+  const Geo& g = c->g;
+  const int64_t nin = (int64_t)g.ncx * g.ncy * g.ncz;
+  if (!c->ccnt) {
+    HIPCHK(c, hipMalloc(&c->ccnt, 4 * (size_t)(nin + 1)));
+    HIPCHK(c, hipMalloc(&c->coff, 4 * (size_t)(nin + 1)));
+    HIPCHK(c, hipMalloc(&c->cbp, 4 * (size_t)(nin + 1)));
+  }
+  CloudDev d;
+  d.cx = cl->cx; d.cy = cl->cy; d.cz = cl->cz;
+  d.N0 = cl->N0; d.r0 = cl->r0; d.H = cl->H; d.rmin = cl->rmin; d.sigma = cl->sigma_chex;
+  d.beta_min = cl->beta_min;
+  d.rate = c->cfg.dx * c->cfg.dy * c->cfg.dz * c->cfg.beta * c->cfg.dt / cl->tau_photo;
+  d.mrat = (float)cl->mrat_pu;
+  d.max_per_cell = cl->max_per_cell;
+  const uint32_t k0 = (uint32_t)c->cfg.seed, k1 = (uint32_t)(c->cfg.seed >> 32);
+  // charge exchange of the ions present (before this step's new ions)
+  if (c->n > 0 && cl->sigma_chex > 0) {
+    HIPCHK(c, hipMemsetAsync(c->dcnt + 3, 0, sizeof(unsigned long long), c->stream));
+    SDISPATCH(c, hipLaunchKernelGGL((k_chex<R>), dim3(nblk(c->n, 256)), dim3(256), 0, c->stream,
+                                    c->n, pdev<R>(c), c->cfg.dx, c->cfg.dy, c->cfg.dz,
+                                    c->cfg.dt, d, k0, k1, 0x40000u + (uint32_t)step, c->mr,
+                                    c->dcnt + 3));
+    if (int e = post(c)) return e;
+    unsigned long long h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, c->dcnt + 3, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *chex = (int64_t)h;
+  }
+  // photoionization: counts per node, offsets, then the new ions
+  hipLaunchKernelGGL(k_cloud_count, dim3(nblk(nin, 256)), dim3(256), 0, c->stream, g, c->cfg.dx,
+                     c->cfg.dy, c->cfg.dz, d, k0, k1, 0x38000u + (uint32_t)step, c->ccnt,
+                     c->cbp);
+  if (int e = post(c)) return e;
+  if (int e = so_scan_u32(c, nin, c->ccnt, c->coff)) return e;
+  uint32_t tot = 0;
+  HIPCHK(c, hipMemcpyAsync(&tot, c->coff + nin, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->n + (int64_t)tot > c->cap)
+    return set_err(c, SWAP_ECAP, "ionize_cloud: %lld + %u ions exceed capacity %lld",
+                   (long long)c->n, tot, (long long)c->cap);
+  if (tot > 0) {
+    SDISPATCH(c, hipLaunchKernelGGL((k_cloud_fill<R>), dim3(nblk(nin, 256)), dim3(256), 0,
+                                    c->stream, g, c->n, pdev<R>(c), d, k0, k1,
+                                    0x30000u + (uint32_t)step, c->ccnt, c->coff, c->cbp, c->bp,
+                                    c->mr, c->id));
+    if (int e = post(c)) return e;
+  }
+  c->n += tot;
+  c->injected += tot;
+  *injected = tot;
+  return 0;
 }
 
 namespace {

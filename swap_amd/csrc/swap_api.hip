@@ -297,7 +297,8 @@ int swap_destroy(swap_ctx* c) {
                 c->scratch, c->dred, c->dmin, c->dcnt, c->d_nitems, c->cell, c->cell2, c->bp,
                 c->bp2, c->mr, c->mr2, c->id, c->id2, c->Ep, c->vpl, c->vmi, c->wght, c->ijkp,
                 c->sruns, c->srbase, c->segkey, c->seginfo, c->segbase, c->nsegc, c->scnt,
-                c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items,
+                c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items, c->ccnt,
+                c->coff, c->cbp,
                 c->boxes, c->boxes_next, c->vb, c->d_zero};
   for (void* p : ps) dfree(p);
   for (int f = 0; f < SWAP_F_COUNT; f++) dfree(c->fld[f]);

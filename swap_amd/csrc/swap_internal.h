@@ -81,8 +81,11 @@ struct swap_ctx {
   int64_t max_items = 0;
   int64_t n_sorted = -1;      // particles covered by the item list (-1: none)
   int steps_since_sort = 0;
+  // cloud source scratch (allocated on first use): per interior node
+  uint32_t *ccnt = nullptr, *coff = nullptr;
+  float* cbp = nullptr;
   // counters
-  unsigned long long* dcnt = nullptr;  // [0] window misses [1] escapes [2] scratch
+  unsigned long long* dcnt = nullptr;  // [0] window misses [1] escapes [2] scratch [3] chex
   int64_t injected = 0, escaped = 0;
   double* dred = nullptr;     // FP64 reduction scratch
   float* dmin = nullptr;      // check_time_step minimum
@@ -146,6 +149,8 @@ int pl_box_reset(swap_ctx* c);
 
 // ---- sort (k_sort.hip) ----
 int so_sort(swap_ctx* c);
+// exclusive scan of n uint32 counts (n <= nkeys) into out[0..n], out[n] = total
+int so_scan_u32(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out);
 
 // ---- sources / diagnostics (k_sources.hip) ----
 int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,

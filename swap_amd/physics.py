@@ -40,6 +40,26 @@ def workload(ncx, ncy, ncz, ppc, vsw=400.0, vth=30.0, **kw):
     return p
 
 
+def cloud(p, N0_rate=50.0, r0_cells=4.0, H_cells=16.0, tau_photo=1.0e6, m_pu=28.0,
+          max_per_cell=8, beta_min=0.05, rmin_cells=0.5):
+    """The pickup-ion source of the cloud workload: a spherical neutral cloud
+    n_n(r) = N0 (r0/r)^2 exp(-r/H) at the domain centre, photoionization time
+    tau_photo, charge exchange sigma = 8e-26 km^2 (chem_rates.f:817), 28-amu
+    ions (mrat = 1/28, cf. chem_rates.f:379-380).  N0 is set so that a node at
+    r = r0 produces N0_rate new ions per step; at most max_per_cell (heavier,
+    beta_p < 1) ions are born per node and step, so ppc near the core grows
+    to hundreds while the solar-wind background stays at its own ppc."""
+    d = p["dx"]
+    rate = d ** 3 * p["beta"] * p["dt"] / tau_photo  # new ions per unit density
+    r0 = r0_cells * d
+    H = H_cells * d
+    N0 = N0_rate / (rate * math.exp(-r0 / H))
+    return {"cx": (p["ncx"] / 2 + 1) * d, "cy": (p["ncy"] / 2 + 1) * d,
+            "cz": (p["ncz"] / 2 + 1) * d, "N0": N0, "r0": r0, "H": H,
+            "tau_photo": tau_photo, "sigma_chex": 8e-26, "mrat_pu": 1.0 / m_pu,
+            "beta_min": beta_min, "max_per_cell": max_per_cell, "rmin": rmin_cells * d}
+
+
 def make_swap(p, capacity=None, **kw):
     """A Swap context with B = B0 zhat everywhere (grd6_setup + f_update_tlev)."""
     from .api import Swap, B1, B12

@@ -544,6 +544,87 @@ def test_load_maxwellian_statistics_and_determinism():
     assert not np.array_equal(v, v3)
 
 
+# ---------------------------------------------------------------- pickup cloud
+def _cloud_setup(rate=3.0, **kw):
+    from swap_amd import physics
+    p = physics.workload(16, 8, 8, 4)
+    cl = physics.cloud(p, N0_rate=rate, **kw)
+    return p, cl
+
+
+def _npofr(p, cl, nc):
+    """expected new ions per interior node: vol beta dt / tau n_n(r)
+    (chem_rates.f:351-352) with n_n(r) = N0 (r0/r)^2 exp(-r/H)"""
+    d = p["dx"]
+    k, j, i = np.meshgrid(np.arange(1, nc[2] + 1), np.arange(1, nc[1] + 1),
+                          np.arange(1, nc[0] + 1), indexing="ij")
+    r = np.sqrt(((i + 1) * d - cl["cx"]) ** 2 + ((j + 1) * d - cl["cy"]) ** 2 +
+                ((k + 1) * d - cl["cz"]) ** 2)
+    r = np.maximum(r, cl["rmin"])
+    nn = cl["N0"] * (cl["r0"] / r) ** 2 * np.exp(-r / cl["H"])
+    return (d ** 3 * p["beta"] * p["dt"] / cl["tau_photo"] * nn).ravel()
+
+
+def test_cloud_counts_follow_the_neutral_profile():
+    p, cl = _cloud_setup()
+    nc = (16, 8, 8)
+    cnt, bp = O.cloud_counts(nc, (p["dx"],) * 3, p["beta"], p["dt"], cl, 5, 3)
+    lam = _npofr(p, cl, nc)
+    big = lam >= 1
+    # at least one expected ion: nint(npofr) ions (chem_rates.f:354), carried by
+    # at most max_per_cell heavier ones (deposited weight cnt / beta_p unchanged)
+    assert np.all(np.abs(cnt[big] / bp[big] - np.rint(lam[big])) < 1e-5 * np.rint(lam[big]))
+    assert np.all((cnt[big] <= cl["max_per_cell"]) | (bp[big] >= cl["beta_min"] - 1e-7))
+    assert np.all(bp <= 1) and np.all(bp >= cl["beta_min"] - 1e-7)
+    # below one: one ion with probability npofr (:391) -- a binomial total
+    small = ~big
+    mu, var = lam[small].sum(), (lam[small] * (1 - lam[small])).sum()
+    assert abs(cnt[small].sum() - mu) < 5 * math.sqrt(var) + 1
+    assert np.all(cnt[small] <= 1) and np.all(bp[small] == 1)
+    assert big.any() and small.any() and (cnt[big] == cl["max_per_cell"]).any()
+    # a different step draws differently, the same step identically
+    cnt2, _ = O.cloud_counts(nc, (p["dx"],) * 3, p["beta"], p["dt"], cl, 5, 3)
+    cnt3, _ = O.cloud_counts(nc, (p["dx"],) * 3, p["beta"], p["dt"], cl, 5, 4)
+    assert np.array_equal(cnt, cnt2) and not np.array_equal(cnt[small], cnt3[small])
+
+
+def test_cloud_ions_sit_in_their_node_dual_cell():
+    p, cl = _cloud_setup()
+    nc = (16, 8, 8)
+    cnt, _ = O.cloud_counts(nc, (p["dx"],) * 3, p["beta"], p["dt"], cl, 5, 3)
+    cell, frac = O.cloud_ions(nc, cnt, 5, 3)
+    assert cell.size == cnt.sum()
+    node = np.repeat(np.arange(cnt.size), cnt)
+    s = [node % nc[0] + 1, (node // nc[0]) % nc[1] + 1, node // (nc[0] * nc[1]) + 1]
+    got = [cell & 2047, (cell >> 11) & 1023, cell >> 21]
+    for a in range(3):
+        assert np.all((frac[a] > 0) & (frac[a] <= 1))
+        t = got[a] + frac[a]  # cell coordinate; node s sits at t = s (q(i) = i d)
+        dlt = np.mod(t - s[a] + nc[a] / 2, nc[a]) - nc[a] / 2
+        assert np.all(np.abs(dlt) <= 0.5)
+        assert abs(np.mean(dlt)) < 5 * math.sqrt(1 / 12 / dlt.size)
+
+
+def test_chex_probability():
+    """res_chex (chem_rates.f:805-843): hit with probability dt n_n sigma |v|"""
+    p, cl = _cloud_setup()
+    x = np.array([9.5, 5.5, 5.5]) * p["dx"]  # cell (8, 4, 4) + 1/2: x = (c + 1 + f) d
+    r = max(np.linalg.norm(x - np.array([cl["cx"], cl["cy"], cl["cz"]])), cl["rmin"])
+    nn = cl["N0"] * (cl["r0"] / r) ** 2 * math.exp(-r / cl["H"])
+    sigma = 0.3 / (p["dt"] * nn * 400.0)  # a cross-section that makes the test sharp
+    cl = dict(cl, sigma_chex=sigma)
+    prob = p["dt"] * nn * sigma * 400.0
+    n = 40000
+    cell = np.full(n, 8 | (4 << 11) | (4 << 21), np.uint32)
+    f = np.full((3, n), 0.5, np.float32)
+    v = np.zeros((3, n), np.float32)
+    v[0] = 400.0
+    hit = O.chex(cell, f, v, (p["dx"],) * 3, p["dt"], cl, 9, 2)
+    assert abs(hit.mean() - prob) < 5 * math.sqrt(prob * (1 - prob) / n)
+    v[:] = 0
+    assert not O.chex(cell, f, v, (p["dx"],) * 3, p["dt"], cl, 9, 2).any()
+
+
 # ---------------------------------------------------------------- composite step
 def seeded_state(o, width, ppc=4, seed=13):
     rng = np.random.default_rng(seed)
