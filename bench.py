@@ -9,6 +9,10 @@ all field subcycles.  Particles are generated on the device before timing.
 
 Prints one JSON line (the driver's contract) with the roofline of the
 dominant kernel and a CPU baseline (the FP64 oracle on a bounded sample).
+
+Other workloads: --workload pickup (a neutral cloud injecting 28-amu ions
+before every step), --precision 64 (the same kernels in FP64), --cells/--ppc
+(e.g. 512 256 256 / 32 for the 1.07e9-particle precision form).
 """
 import argparse
 import json
@@ -82,6 +86,9 @@ def main():
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--sort-every", type=int, default=2)
     ap.add_argument("--precision", type=int, default=32)
+    ap.add_argument("--workload", choices=("uniform", "pickup"), default="uniform",
+                    help="pickup: a neutral cloud at the domain centre injects 28-amu ions "
+                         "(photoionization, charge exchange) before every step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates HBM byte counters)")
@@ -98,27 +105,46 @@ def main():
         torch.cuda.synchronize()
         del x, y
 
-    from swap_amd import physics
+    from swap_amd import api, physics
     ncx, ncy, ncz = args.cells
     p = physics.workload(ncx, ncy, ncz, args.ppc)
     npart = p["npart"]
     nn = (ncx + 2) * (ncy + 2) * (ncz + 2)
-    s = physics.make_swap(p, precision=args.precision, sort_every=args.sort_every)
+    pickup = args.workload == "pickup"
+    cloud = physics.cloud(p) if pickup else None
+    # the cloud adds a few tenths of a percent of ions per step
+    cap = npart + (npart // 50 + 2000000) * pickup
+    s = physics.make_swap(p, capacity=cap, precision=args.precision,
+                          sort_every=args.sort_every, flags=api.SPECIES if pickup else 0)
     s.load_maxwellian(npart, p["vsw"], p["vth"])
     print("bench: loaded %d particles on %dx%dx%d cells" % (npart, ncx, ncy, ncz),
           file=sys.stderr, flush=True)
-    s.step(args.warmup)
+
+    def run(k, step0):
+        """k steps; returns the particle-steps done"""
+        if not pickup:
+            s.step(k)
+            return s.n * k
+        done = 0
+        for i in range(k):
+            s.ionize_cloud(step0 + i, cloud)  # Ionize_pluto_mp before the step
+            done += s.n
+            s.step(1)
+        return done
+
+    run(args.warmup, 0)
     s.sync()
     s.profile(True)
     t0 = time.perf_counter()
-    s.step(args.steps)
+    psteps = run(args.steps, args.warmup)
     s.sync()
     elapsed = time.perf_counter() - t0
     prof = s.profile_read()
     s.profile(False)
     diag = s.diagnostics()
     ms_step = elapsed * 1e3 / args.steps
-    value = npart * args.steps / elapsed
+    value = psteps / elapsed
+    npart = psteps // args.steps  # mean particles per timed step
 
     # dominant particle pass and its roofline
     dom = max(PASS_BYTES, key=lambda k: prof[k][0])
@@ -127,7 +153,7 @@ def main():
     alg = bp * npart + bn * nn
     achieved = alg / (ms_tot / cnt * 1e-3) / 1e9
     step_alg = 136 * npart + 116 * nn
-    kname = "k_pass<float,%s>" % dom[1]
+    kname = "k_pass<%s,%s>" % ("float" if args.precision == 32 else "double", dom[1])
     tcfg = {"cells": [ncx, ncy, ncz], "ppc": args.ppc, "sort_every": args.sort_every,
             "precision": args.precision}
     out = {
@@ -135,10 +161,14 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32" if args.precision == 32 else "fp64",
-        "data": "synthetic: seeded drifting Maxwellian (Philox), uniform B0",
-        "config": {"workload": "%dx%dx%d cells x %d ppc (%d protons), 10 B subcycles/step, "
-                               "periodic, split-precision FP32 particles, FP64 B advance"
-                               % (ncx, ncy, ncz, args.ppc, npart),
+        "data": "synthetic: seeded drifting Maxwellian (Philox), uniform B0" +
+                (", neutral-cloud pickup ions" if pickup else ""),
+        "config": {"workload": "%dx%dx%d cells x %d ppc (%d %s), 10 B subcycles/step, "
+                               "periodic, %s particles, FP64 B advance"
+                               % (ncx, ncy, ncz, args.ppc, npart,
+                                  "ions incl. cloud pickup ions (mean)" if pickup else "protons",
+                                  "split-precision FP32" if args.precision == 32
+                                  else "FP64"),
                    "cells": [ncx, ncy, ncz], "ppc": args.ppc, "n_particles": npart,
                    "sort_every": args.sort_every, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname,
@@ -151,6 +181,9 @@ def main():
         "stages_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
         "window_misses": diag["window_misses"],
     }
+    if pickup:
+        out["config"]["cloud"] = cloud
+        out["config"]["injected"] = diag["injected"]
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

@@ -443,7 +443,7 @@ int pl_pass(swap_ctx* c, int mode) {
     const R* S = (const R*)c->S;
     using A = typename AccOf<R>::T;
     A* acc = (A*)c->macc;
-    const bool runs = c->n_sorted == c->n && c->steps_since_sort == 0;
+    const bool runs = c->n_sorted >= 0 && c->steps_since_sort == 0;
     if (mode == 1 && runs)
       hipLaunchKernelGGL((k_pass<R, 1, PASS_KP>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,

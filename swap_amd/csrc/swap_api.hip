@@ -785,8 +785,12 @@ int swap_prime(swap_ctx* c) {
 int swap_step(swap_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return SWAP_EINVAL;
   for (int s = 0; s < nsteps; s++) {
+    // sort on schedule, or when particles were loaded (nothing sorted), or
+    // when injected ions appended past the sorted range exceed 1/16 of all
+    // (the passes take such a tail without windows until the next sort)
     bool due = c->cfg.sort_every > 0 &&
-               (c->n_sorted != c->n || c->steps_since_sort >= c->cfg.sort_every);
+               (c->n_sorted < 0 || c->steps_since_sort >= c->cfg.sort_every ||
+                (c->n - c->n_sorted) * 16 > c->n);
     if (due) {
       Stage t(c, SWAP_K_SORT);
       TRY(swap_sort(c));
