@@ -219,6 +219,50 @@ int swap_ionize_cloud(swap_ctx *ctx, int step, const swap_cloud *cloud, int64_t 
 /* ---- diagnostics (FP64 reductions; Energy_diag part_init.f:14-139) ---- */
 int swap_diagnostics(swap_ctx *ctx, swap_diag *out);
 
+/* ---- learned moment denoiser (build-only; no reference counterpart) ----
+   A 3-D convolutional net, 4 -> 32 -> 32 -> 32 -> 4 channels, 3x3x3 kernels
+   with periodic padding, ReLU between layers and a residual output, that maps
+   the moments (np, up) of a low-ppc deposit to those of a high-ppc one.
+   Convolutions run on MFMA with BF16 operands and FP32 accumulation; master
+   weights are FP32 and trained with Adam by the library's own forward and
+   backward kernels, on pairs the library generates from its own particles.
+   Channels are normalised: np/n_scale - 1 and up/v_scale.  Parameters, flat
+   FP32 in layer order: W[l][tap][c_in][c_out] (tap = dx+1 + 3(dy+1) + 9(dz+1))
+   then b[l][c_out]. */
+typedef struct {
+  double n_scale;   /* density unit of the normalisation (e.g. n0) */
+  double v_scale;   /* velocity unit of the normalisation, km/s */
+  int max_pairs;    /* training pairs kept on the device */
+  int stride;       /* low-ppc input: particles with id % stride == 0, weight x stride */
+  uint64_t seed;    /* weight initialisation and pair draws */
+} swap_dn_config;
+int64_t swap_dn_num_params(void);
+int swap_dn_create(swap_ctx *ctx, const swap_dn_config *cfg);
+/* append a training pair from the current particles (needs SWAP_TRACK_IDS):
+   input = the stride subsample's moments, target = all particles' moments;
+   the context's np/up fields are left as they were */
+int swap_dn_add_pair(swap_ctx *ctx);
+int swap_dn_num_pairs(swap_ctx *ctx);
+/* iters Adam steps (learning rate lr) on pairs drawn from [0, ntrain); *loss
+   = mean squared error (normalised units) of the last step */
+int swap_dn_train(swap_ctx *ctx, int iters, double lr, int ntrain, double *loss);
+/* mean squared error over pairs [p0, p1) of the raw input and of the net's
+   output against the target.  Opens the gate of swap_dn_enable when the net
+   is better on these pairs, closes it otherwise. */
+int swap_dn_eval(swap_ctx *ctx, int p0, int p1, double *err_raw, double *err_net);
+/* on = 1: swap_step replaces each deposit's np/up by the net's output; refused
+   (SWAP_ESTATE) while the gate is closed, so the step keeps plain deposition */
+int swap_dn_enable(swap_ctx *ctx, int on);
+/* one forward + backward on a host input/target ([4][ncz][ncy][ncx] FP32,
+   normalised): *loss and the parameter gradients (flat FP32) -- the check of
+   the backward kernels against a CPU computation */
+int swap_dn_grad(swap_ctx *ctx, const float *x, const float *target, double *loss,
+                 float *grads);
+/* the net's output for a host input ([4][ncz][ncy][ncx] FP32, normalised) */
+int swap_dn_infer(swap_ctx *ctx, const float *x, float *y);
+int swap_dn_get_params(swap_ctx *ctx, float *params);
+int swap_dn_set_params(swap_ctx *ctx, const float *params);
+
 /* ---- build-only instrumentation: HIP events on the context stream around
    each stage of swap_step; no reference counterpart (the reference times
    only the whole run, maind.f:648-654) ---- */

@@ -58,6 +58,11 @@ class Cloud(C.Structure):
                 ("max_per_cell", C.c_int), ("rmin", C.c_double)]
 
 
+class DnConfig(C.Structure):
+    _fields_ = [("n_scale", C.c_double), ("v_scale", C.c_double), ("max_pairs", C.c_int),
+                ("stride", C.c_int), ("seed", C.c_uint64)]
+
+
 _lib = None
 
 
@@ -118,6 +123,17 @@ def load_library():
         "swap_diagnostics": ([vp, C.POINTER(Diag)], i32),
         "swap_profile": ([vp, i32], i32),
         "swap_profile_read": ([vp, vp, vp, i32], i32),
+        "swap_dn_num_params": ([], i64),
+        "swap_dn_create": ([vp, C.POINTER(DnConfig)], i32),
+        "swap_dn_add_pair": ([vp], i32),
+        "swap_dn_num_pairs": ([vp], i32),
+        "swap_dn_train": ([vp, i32, dbl, i32, C.POINTER(dbl)], i32),
+        "swap_dn_eval": ([vp, i32, i32, C.POINTER(dbl), C.POINTER(dbl)], i32),
+        "swap_dn_enable": ([vp, i32], i32),
+        "swap_dn_grad": ([vp, vp, vp, C.POINTER(dbl), vp], i32),
+        "swap_dn_infer": ([vp, vp, vp], i32),
+        "swap_dn_get_params": ([vp, vp], i32),
+        "swap_dn_set_params": ([vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -361,6 +377,58 @@ class Swap:
         self._ck(self.lib.swap_ionize_cloud(self.h, step, C.byref(cloud), C.byref(n),
                                             C.byref(x)), "ionize_cloud")
         return n.value, x.value
+
+    # ---- learned moment denoiser ----
+    def dn_create(self, n_scale, v_scale, max_pairs=64, stride=32, seed=7):
+        cfg = DnConfig(n_scale, v_scale, max_pairs, stride, seed)
+        self._ck(self.lib.swap_dn_create(self.h, C.byref(cfg)), "dn_create")
+
+    def dn_add_pair(self):
+        self._ck(self.lib.swap_dn_add_pair(self.h), "dn_add_pair")
+
+    def dn_num_pairs(self):
+        return self.lib.swap_dn_num_pairs(self.h)
+
+    def dn_train(self, iters, lr, ntrain):
+        loss = C.c_double()
+        self._ck(self.lib.swap_dn_train(self.h, iters, lr, ntrain, C.byref(loss)), "dn_train")
+        return loss.value
+
+    def dn_eval(self, p0, p1):
+        a, b = C.c_double(), C.c_double()
+        self._ck(self.lib.swap_dn_eval(self.h, p0, p1, C.byref(a), C.byref(b)), "dn_eval")
+        return a.value, b.value
+
+    def dn_enable(self, on=True):
+        self._ck(self.lib.swap_dn_enable(self.h, 1 if on else 0), "dn_enable")
+
+    def _grid4(self, a):
+        return np.ascontiguousarray(np.broadcast_to(a, (4, self.nz - 2, self.ny - 2, self.nx - 2)),
+                                    np.float32)
+
+    def dn_grad(self, x, target):
+        x, t = self._grid4(x), self._grid4(target)
+        g = np.zeros(self.lib.swap_dn_num_params(), np.float32)
+        loss = C.c_double()
+        self._ck(self.lib.swap_dn_grad(self.h, x.ctypes.data, t.ctypes.data, C.byref(loss),
+                                       g.ctypes.data), "dn_grad")
+        return loss.value, g
+
+    def dn_infer(self, x):
+        x = self._grid4(x)
+        y = np.zeros_like(x)
+        self._ck(self.lib.swap_dn_infer(self.h, x.ctypes.data, y.ctypes.data), "dn_infer")
+        return y
+
+    def dn_params(self):
+        p = np.zeros(self.lib.swap_dn_num_params(), np.float32)
+        self._ck(self.lib.swap_dn_get_params(self.h, p.ctypes.data), "dn_get_params")
+        return p
+
+    def dn_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.lib.swap_dn_num_params()
+        self._ck(self.lib.swap_dn_set_params(self.h, p.ctypes.data), "dn_set_params")
 
     STAGES = ("P1", "P2", "P3", "sort", "field", "glue")
 

@@ -1,0 +1,663 @@
+// k_denoise.hip -- learned moment denoiser: a 3-D convolutional net trained
+// and run on the device with the library's own kernels.
+//
+// Net: 4 -> 32 -> 32 -> 32 -> 4 channels, 3x3x3 kernels with periodic
+// padding, ReLU after the first three layers, residual output (out = in +
+// conv4(...)).  Activations are channels-last BF16 [voxel][channel] over the
+// interior nodes (i fastest).  Every convolution is an implicit GEMM on MFMA
+// (v_mfma_f32_16x16x16_bf16, FP32 accumulation): M = voxels, N = output
+// channels, K = 27 taps x input channels.  A wavefront owns 16 voxels and all
+// output channels; lane l supplies voxel l%16 and the four K entries
+// 4(l/16)..4(l/16)+3 of the A operand, output channel l%16 of the B operand,
+// and receives outputs (voxel 4(l/16)+e, channel l%16).
+//
+// Backward: the loss gradient is propagated with the same kernel through the
+// tap-flipped, transposed weights (the adjoint of a periodic convolution),
+// masked by the ReLU of the layer below; weight gradients are a second GEMM
+// (M = K entries, N = output channels, reduction over voxels) accumulated
+// into FP32 with atomics.  FP32 master weights, Adam.
+#include "swap_internal.h"
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+using bf16 = uint16_t;
+
+// FP32 -> BF16, round to nearest even (finite values)
+__device__ __forceinline__ bf16 f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int NL = 4;
+constexpr int CH[NL + 1] = {4, 32, 32, 32, 4};
+constexpr int64_t wcount(int l) { return 27LL * CH[l] * CH[l + 1]; }
+constexpr int64_t layer_params(int l) { return wcount(l) + CH[l + 1]; }
+constexpr int64_t param_off(int l) { return l == 0 ? 0 : param_off(l - 1) + layer_params(l - 1); }
+constexpr int64_t NPARAM = param_off(NL);
+constexpr int k16(int cin) { return (27 * cin + 15) / 16 * 16; }
+constexpr int r16(int c) { return (c + 15) / 16 * 16; }
+
+enum { EPI_RELU = 0, EPI_RESID = 1, EPI_MASK = 2 };
+
+struct G3 {
+  int nx, ny, nz;
+  int64_t V;
+};
+__device__ __forceinline__ int64_t nbr(const G3& g, int x, int y, int z, int t) {
+  int a = x + t % 3 - 1, b = y + (t / 3) % 3 - 1, c = z + t / 9 - 1;
+  a = a < 0 ? a + g.nx : (a >= g.nx ? a - g.nx : a);
+  b = b < 0 ? b + g.ny : (b >= g.ny ? b - g.ny : b);
+  c = c < 0 ? c + g.nz : (c >= g.nz ? c - g.nz : c);
+  return (int64_t)a + (int64_t)g.nx * ((int64_t)b + (int64_t)g.ny * c);
+}
+
+// Y = epilogue(conv(X, W)); Wt is [r16(COUT)][k16(CIN)], zero padded
+template <int CIN, int COUT, int EPI>
+__global__ __launch_bounds__(256) void k_conv(G3 g, const bf16* __restrict__ X,
+                                              const bf16* __restrict__ Wt,
+                                              const float* __restrict__ bias,
+                                              const bf16* __restrict__ mask,
+                                              const float* __restrict__ resid,
+                                              bf16* __restrict__ Yb, float* __restrict__ Yf) {
+  constexpr int K = 27 * CIN, KP = k16(CIN), NT = r16(COUT) / 16;
+  const int lane = threadIdx.x & 63;
+  const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (v0 >= g.V) return;
+  const int m = lane & 15, kq = (lane >> 4) * 4;
+  const int64_t vm = v0 + m;
+  const bool vok = vm < g.V;
+  const int x = (int)(vm % g.nx), y = (int)((vm / g.nx) % g.ny), z = (int)(vm / ((int64_t)g.nx * g.ny));
+  f4 acc[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) acc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int kk = 0; kk < KP; kk += 16) {
+    const int k0 = kk + kq;
+    s4 a = s4{0, 0, 0, 0};
+    if (vok && k0 < K) {
+      const int t = k0 / CIN, ci = k0 % CIN;
+      a = *(const s4*)(X + nbr(g, x, y, z, t) * CIN + ci);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const s4 b = *(const s4*)(Wt + (int64_t)(nt * 16 + m) * KP + k0);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) {
+    const int co = nt * 16 + m;
+    if (co >= COUT) continue;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int64_t v = v0 + kq + e;
+      if (v >= g.V) continue;
+      float val = acc[nt][e];
+      const int64_t o = v * COUT + co;
+      if constexpr (EPI == EPI_RELU) {
+        val += bias[co];
+        Yb[o] = f2bf(val > 0.f ? val : 0.f);
+      } else if constexpr (EPI == EPI_RESID) {
+        Yf[o] = val + bias[co] + resid[o];
+      } else {
+        Yb[o] = f2bf(bf2f(mask[o]) > 0.f ? val : 0.f);
+      }
+    }
+  }
+}
+
+// dW[k][co] += sum over a chunk of voxels of X[nbr(v, tap(k))][ci(k)] dY[v][co]
+template <int CIN, int COUT>
+__global__ __launch_bounds__(64) void k_wgrad(G3 g, const bf16* __restrict__ X,
+                                             const bf16* __restrict__ dY, float* dW,
+                                             int64_t chunk) {
+  constexpr int K = 27 * CIN;
+  const int lane = threadIdx.x;
+  const int m = lane & 15, kq = (lane >> 4) * 4;
+  const int k = blockIdx.y * 16 + m, co = blockIdx.z * 16 + m;
+  const bool kok = k < K, cok = co < COUT;
+  const int t = kok ? k / CIN : 0, ci = kok ? k % CIN : 0;
+  const int64_t vb0 = (int64_t)blockIdx.x * chunk;
+  const int64_t vb1 = vb0 + chunk < g.V ? vb0 + chunk : g.V;
+  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t vb = vb0; vb < vb1; vb += 16) {
+    s4 a, b;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int64_t v = vb + kq + e;
+      const bool ok = v < vb1;
+      short av = 0, bv = 0;
+      if (ok && kok) {
+        const int x = (int)(v % g.nx), y = (int)((v / g.nx) % g.ny),
+                  z = (int)(v / ((int64_t)g.nx * g.ny));
+        av = (short)X[nbr(g, x, y, z, t) * CIN + ci];
+      }
+      if (ok && cok) bv = (short)dY[v * COUT + co];
+      a[e] = av;
+      b[e] = bv;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int kk = blockIdx.y * 16 + kq + e;
+    if (kk < K && cok) atomicAdd(dW + (int64_t)kk * COUT + co, acc[e]);
+  }
+}
+
+// db[co] += sum_v dY[v][co]
+template <int COUT>
+__global__ __launch_bounds__(256) void k_bgrad(int64_t V, const bf16* __restrict__ dY, float* db) {
+  __shared__ float s[256];
+  constexpr int PER = 256 / COUT;
+  const int co = threadIdx.x % COUT, r = threadIdx.x / COUT;
+  float sum = 0.f;
+  for (int64_t v = (int64_t)blockIdx.x * PER + r; v < V; v += (int64_t)gridDim.x * PER)
+    sum += bf2f(dY[v * COUT + co]);
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x < COUT) {
+    float t = 0.f;
+    for (int q = 0; q < PER; q++) t += s[q * COUT + threadIdx.x];
+    atomicAdd(db + threadIdx.x, t);
+  }
+}
+
+// forward and backward weight layouts from the FP32 master W[tap][ci][co]:
+// Wt[co][tap*CIN + ci] (forward), Wb[ci][tap'*COUT + co] = W[26 - tap'][ci][co]
+// (the adjoint: input gradient = conv of the output gradient)
+template <int CIN, int COUT>
+__global__ void k_pack(const float* __restrict__ W, bf16* Wt, bf16* Wb) {
+  constexpr int KF = k16(CIN), KB = k16(COUT);
+  constexpr int64_t NF = (int64_t)r16(COUT) * KF, NB = (int64_t)r16(CIN) * KB;
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < NF) {
+    int co = (int)(q / KF), k = (int)(q % KF);
+    float w = 0.f;
+    if (co < COUT && k < 27 * CIN) w = W[((int64_t)(k / CIN) * CIN + k % CIN) * COUT + co];
+    Wt[q] = f2bf(w);
+  } else if (q < NF + NB) {
+    int64_t r = q - NF;
+    int ci = (int)(r / KB), k = (int)(r % KB);
+    float w = 0.f;
+    if (ci < CIN && k < 27 * COUT) {
+      int t = k / COUT, co = k % COUT;
+      w = W[((int64_t)(26 - t) * CIN + ci) * COUT + co];
+    }
+    Wb[r] = f2bf(w);
+  }
+}
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// mean squared error over n values; gradient 2 (out - tgt) / n as BF16
+__global__ void k_mse(int64_t n, const float* __restrict__ out, const float* __restrict__ tgt,
+                      bf16* __restrict__ grad, double* loss) {
+  double s = 0.0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    float d = out[q] - tgt[q];
+    if (grad) grad[q] = f2bf(2.f * d / (float)n);
+    s += (double)d * d;
+  }
+  s = wsum(s);
+  if (__lane_id() == 0) atomicAdd(loss, s / (double)n);
+}
+
+__global__ void k_adam(int64_t n, float* p, const float* g, float* m, float* v, float lr,
+                       float c1, float c2) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  float gq = g[q];
+  float mq = 0.9f * m[q] + 0.1f * gq;
+  float vq = 0.999f * v[q] + 0.001f * gq * gq;
+  m[q] = mq;
+  v[q] = vq;
+  p[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + 1e-8f);
+}
+
+__global__ void k_f2bf(int64_t n, const float* __restrict__ x, bf16* __restrict__ y) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < n) y[q] = f2bf(x[q]);
+}
+
+// He-normal weights (Box-Muller on Philox draws), zero biases; the last
+// layer starts small, so the residual net starts near the identity
+__global__ void k_init(float* p, uint32_t k0, uint32_t k1) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= NPARAM) return;
+  int l = 0;
+  while (l + 1 < NL && q >= param_off(l + 1)) l++;
+  int64_t r = q - param_off(l);
+  if (r >= wcount(l)) {
+    p[q] = 0.f;
+    return;
+  }
+  uint32_t c[4] = {(uint32_t)q, 0u, 0x50000u, 0u};
+  philox4x32_10(c, k0, k1);
+  double u1 = 1.0 - u24(c[0]), u2 = u24(c[1]);
+  double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  double sd = sqrt(2.0 / (27.0 * CH[l])) * (l == NL - 1 ? 0.1 : 1.0);
+  p[q] = (float)(z * sd);
+}
+
+// moments (storage layout, ghosts included) -> normalised [voxel][4] FP32
+template <typename R>
+__global__ void k_dn_in(Geo g, const R* __restrict__ np, const R* __restrict__ up, float nsc,
+                        float vsc, float* __restrict__ X) {
+  int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t V = (int64_t)g.ncx * g.ncy * g.ncz;
+  if (v >= V) return;
+  int i = 1 + (int)(v % g.ncx), j = 1 + (int)((v / g.ncx) % g.ncy),
+      k = 1 + (int)(v / ((int64_t)g.ncx * g.ncy));
+  int64_t s = gidx(g, i, j, k);
+  X[4 * v] = (float)np[s] / nsc - 1.f;
+  for (int c = 0; c < 3; c++) X[4 * v + 1 + c] = (float)up[s + c * g.nn] / vsc;
+}
+
+// net output -> moments at every storage node (ghosts = periodic images)
+template <typename R>
+__global__ void k_dn_out(Geo g, const float* __restrict__ Y, float nsc, float vsc, int with_np,
+                         R* np, R* up) {
+  int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= g.nn) return;
+  int i = (int)(s % g.nx), j = (int)((s / g.nx) % g.ny), k = (int)(s / ((int64_t)g.nx * g.ny));
+  int64_t v = (int64_t)(canon(i, g.nx) - 1) +
+              (int64_t)g.ncx * ((int64_t)(canon(j, g.ny) - 1) + (int64_t)g.ncy * (canon(k, g.nz) - 1));
+  if (with_np) np[s] = (R)((Y[4 * v] + 1.f) * nsc);
+  for (int c = 0; c < 3; c++) up[s + c * g.nn] = (R)(Y[4 * v + 1 + c] * vsc);
+}
+
+int post(swap_ctx* c) {
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+struct DnState {
+  swap_dn_config cfg;
+  G3 g3;
+  float* params = nullptr;  // FP32 master weights, flat
+  float* grads = nullptr;
+  float* am = nullptr;      // Adam moments
+  float* av = nullptr;
+  bf16* Wt[NL] = {};
+  bf16* Wb[NL] = {};
+  bf16* act[NL] = {};       // act[0] input (4 ch), act[1..3] post-ReLU (32 ch)
+  bf16* gr[NL + 1] = {};    // gradients w.r.t. the inputs of layer l (pre-ReLU of l-1); gr[NL] = dOut
+  float* xf = nullptr;      // normalised input [V][4]
+  float* out = nullptr;     // output [V][4]
+  double* dloss = nullptr;
+  float* pin = nullptr;     // pairs: inputs [max_pairs][V][4]
+  float* ptg = nullptr;     // pairs: targets
+  int npairs = 0;
+  int64_t t = 0;            // Adam step
+  bool gate = false, enabled = false;
+};
+
+namespace {
+
+int dn_pack(swap_ctx* c) {
+  DnState* d = c->dn;
+  auto pk = [&](auto kern, int l, int64_t nf, int64_t nb) {
+    hipLaunchKernelGGL(kern, dim3(nblk(nf + nb, 256)), dim3(256), 0, c->stream,
+                       d->params + param_off(l), d->Wt[l], d->Wb[l]);
+  };
+  pk(k_pack<4, 32>, 0, (int64_t)r16(32) * k16(4), (int64_t)r16(4) * k16(32));
+  pk(k_pack<32, 32>, 1, (int64_t)r16(32) * k16(32), (int64_t)r16(32) * k16(32));
+  pk(k_pack<32, 32>, 2, (int64_t)r16(32) * k16(32), (int64_t)r16(32) * k16(32));
+  pk(k_pack<32, 4>, 3, (int64_t)r16(4) * k16(32), (int64_t)r16(32) * k16(4));
+  return post(c);
+}
+
+// forward from d->xf to d->out, keeping the activations
+int dn_forward(swap_ctx* c) {
+  DnState* d = c->dn;
+  const G3 g = d->g3;
+  const int64_t V = g.V;
+  dim3 grid((unsigned)((V + 63) / 64)), blk(256);
+  hipLaunchKernelGGL(k_f2bf, dim3(nblk(4 * V, 256)), dim3(256), 0, c->stream, 4 * V, d->xf,
+                     d->act[0]);
+  const float* b[NL];
+  for (int l = 0; l < NL; l++) b[l] = d->params + param_off(l) + wcount(l);
+  hipLaunchKernelGGL((k_conv<4, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[0], d->Wt[0],
+                     b[0], nullptr, nullptr, d->act[1], nullptr);
+  hipLaunchKernelGGL((k_conv<32, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[1],
+                     d->Wt[1], b[1], nullptr, nullptr, d->act[2], nullptr);
+  hipLaunchKernelGGL((k_conv<32, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[2],
+                     d->Wt[2], b[2], nullptr, nullptr, d->act[3], nullptr);
+  hipLaunchKernelGGL((k_conv<32, 4, EPI_RESID>), grid, blk, 0, c->stream, g, d->act[3],
+                     d->Wt[3], b[3], nullptr, d->xf, nullptr, d->out);
+  return post(c);
+}
+
+// loss against tgt and the parameter gradients into d->grads
+int dn_backward(swap_ctx* c, const float* tgt) {
+  DnState* d = c->dn;
+  const G3 g = d->g3;
+  const int64_t V = g.V;
+  dim3 grid((unsigned)((V + 63) / 64)), blk(256);
+  HIPCHK(c, hipMemsetAsync(d->grads, 0, sizeof(float) * NPARAM, c->stream));
+  HIPCHK(c, hipMemsetAsync(d->dloss, 0, sizeof(double), c->stream));
+  hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, 4 * V, d->out, tgt, d->gr[NL],
+                     d->dloss);
+  const int64_t chunk = 2048;
+  const unsigned nch = (unsigned)((V + chunk - 1) / chunk);
+  float* gw[NL];
+  float* gb[NL];
+  for (int l = 0; l < NL; l++) {
+    gw[l] = d->grads + param_off(l);
+    gb[l] = gw[l] + wcount(l);
+  }
+  // layer 4 (32 -> 4, no ReLU)
+  hipLaunchKernelGGL((k_wgrad<32, 4>), dim3(nch, k16(32) / 16, 1), dim3(64), 0, c->stream, g,
+                     d->act[3], d->gr[4], gw[3], chunk);
+  hipLaunchKernelGGL(k_bgrad<4>, dim3(256), dim3(256), 0, c->stream, V, d->gr[4], gb[3]);
+  hipLaunchKernelGGL((k_conv<4, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[4], d->Wb[3],
+                     nullptr, d->act[3], nullptr, d->gr[3], nullptr);
+  // layers 3 and 2 (32 -> 32)
+  for (int l = 2; l >= 1; l--) {
+    hipLaunchKernelGGL((k_wgrad<32, 32>), dim3(nch, k16(32) / 16, 2), dim3(64), 0, c->stream, g,
+                       d->act[l], d->gr[l + 1], gw[l], chunk);
+    hipLaunchKernelGGL(k_bgrad<32>, dim3(256), dim3(256), 0, c->stream, V, d->gr[l + 1], gb[l]);
+    hipLaunchKernelGGL((k_conv<32, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[l + 1],
+                       d->Wb[l], nullptr, d->act[l], nullptr, d->gr[l], nullptr);
+  }
+  // layer 1 (4 -> 32); no gradient to the input
+  hipLaunchKernelGGL((k_wgrad<4, 32>), dim3(nch, k16(4) / 16, 2), dim3(64), 0, c->stream, g,
+                     d->act[0], d->gr[1], gw[0], chunk);
+  hipLaunchKernelGGL(k_bgrad<32>, dim3(256), dim3(256), 0, c->stream, V, d->gr[1], gb[0]);
+  return post(c);
+}
+
+int dn_moments_in(swap_ctx* c, float* X) {
+  DnState* d = c->dn;
+  if (c->prec == 64)
+    hipLaunchKernelGGL(k_dn_in<double>, dim3(nblk(d->g3.V, 256)), dim3(256), 0, c->stream, c->g,
+                       (const double*)c->fld[SWAP_F_NP], (const double*)c->fld[SWAP_F_UP],
+                       (float)d->cfg.n_scale, (float)d->cfg.v_scale, X);
+  else
+    hipLaunchKernelGGL(k_dn_in<float>, dim3(nblk(d->g3.V, 256)), dim3(256), 0, c->stream, c->g,
+                       (const float*)c->fld[SWAP_F_NP], (const float*)c->fld[SWAP_F_UP],
+                       (float)d->cfg.n_scale, (float)d->cfg.v_scale, X);
+  return post(c);
+}
+
+int zero_acc(swap_ctx* c, int m0, int m1) {
+  size_t off = 8 * (size_t)c->g.nn * m0, len = 8 * (size_t)c->g.nn * (m1 - m0);
+  HIPCHK(c, hipMemsetAsync((char*)c->macc + off, 0, len, c->stream));
+  return 0;
+}
+
+// np and up of the current particles (all, or the stride subsample) into the
+// context's np/up fields
+int dn_deposit(swap_ctx* c, int stride) {
+  if (int e = zero_acc(c, 0, 4)) return e;
+  if (int e = pl_deposit_simple(c, 0, stride)) return e;
+  if (int e = fl_finalize_np(c)) return e;
+  if (int e = pl_deposit_simple(c, 1, stride)) return e;
+  return fl_finalize_up(c);
+}
+
+int dn_need(swap_ctx* c) {
+  if (!c->dn) return set_err(c, SWAP_ESTATE, "denoiser: swap_dn_create first");
+  return 0;
+}
+
+// [4][V] host -> [V][4] device
+int host_to_dev(swap_ctx* c, const float* x, float* dst) {
+  const int64_t V = c->dn->g3.V;
+  std::vector<float> t(4 * V);
+  for (int64_t v = 0; v < V; v++)
+    for (int q = 0; q < 4; q++) t[4 * v + q] = x[q * V + v];
+  HIPCHK(c, hipMemcpyAsync(dst, t.data(), 16 * V, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+// ---------------- C-ABI ----------------
+
+extern "C" int64_t swap_dn_num_params(void) { return NPARAM; }
+
+extern "C" int swap_dn_create(swap_ctx* c, const swap_dn_config* cfg) {
+  if (!c || !cfg) return SWAP_EINVAL;
+  if (!(cfg->n_scale > 0) || !(cfg->v_scale > 0) || cfg->max_pairs < 0 || cfg->stride < 1)
+    return set_err(c, SWAP_EINVAL, "dn_create: bad config");
+  dn_destroy(c);
+  DnState* d = new DnState();
+  c->dn = d;
+  d->cfg = *cfg;
+  d->g3 = G3{c->g.ncx, c->g.ncy, c->g.ncz, (int64_t)c->g.ncx * c->g.ncy * c->g.ncz};
+  const int64_t V = d->g3.V;
+  auto A = [&](void** p, size_t b) -> int {
+    hipError_t e = hipMalloc(p, b ? b : 16);
+    if (e != hipSuccess)
+      return set_err(c, SWAP_ENOMEM, "dn_create: hipMalloc(%zu): %s", b, hipGetErrorString(e));
+    return 0;
+  };
+#define DA(p, b) \
+  if (int e_ = A((void**)&(p), (b))) return e_
+  DA(d->params, 4 * NPARAM);
+  DA(d->grads, 4 * NPARAM);
+  DA(d->am, 4 * NPARAM);
+  DA(d->av, 4 * NPARAM);
+  for (int l = 0; l < NL; l++) {
+    DA(d->Wt[l], 2 * (size_t)r16(CH[l + 1]) * k16(CH[l]));
+    DA(d->Wb[l], 2 * (size_t)r16(CH[l]) * k16(CH[l + 1]));
+    DA(d->act[l], 2 * (size_t)V * CH[l]);
+    DA(d->gr[l + 1], 2 * (size_t)V * CH[l + 1]);
+  }
+  DA(d->xf, 16 * (size_t)V);
+  DA(d->out, 16 * (size_t)V);
+  DA(d->dloss, sizeof(double));
+  DA(d->pin, 16 * (size_t)V * (size_t)cfg->max_pairs);
+  DA(d->ptg, 16 * (size_t)V * (size_t)cfg->max_pairs);
+#undef DA
+  HIPCHK(c, hipMemsetAsync(d->am, 0, 4 * NPARAM, c->stream));
+  HIPCHK(c, hipMemsetAsync(d->av, 0, 4 * NPARAM, c->stream));
+  hipLaunchKernelGGL(k_init, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, d->params,
+                     (uint32_t)cfg->seed, (uint32_t)(cfg->seed >> 32));
+  if (int e = post(c)) return e;
+  if (int e = dn_pack(c)) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+void dn_destroy(swap_ctx* c) {
+  DnState* d = c->dn;
+  if (!d) return;
+  void* ps[] = {d->params, d->grads, d->am, d->av, d->xf, d->out, d->dloss, d->pin, d->ptg};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  for (int l = 0; l < NL; l++) {
+    void* q[] = {d->Wt[l], d->Wb[l], d->act[l], d->gr[l + 1]};
+    for (void* p : q)
+      if (p) (void)hipFree(p);
+  }
+  delete d;
+  c->dn = nullptr;
+}
+
+extern "C" int swap_dn_add_pair(swap_ctx* c) {
+  if (!c) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  DnState* d = c->dn;
+  if (d->npairs >= d->cfg.max_pairs) return set_err(c, SWAP_ECAP, "dn_add_pair: pairs full");
+  if (!c->id && d->cfg.stride > 1)
+    return set_err(c, SWAP_ESTATE, "dn_add_pair needs SWAP_TRACK_IDS");
+  const size_t nn = (size_t)c->g.nn, rs = c->rsz;
+  // keep the context's np/up
+  HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_TMP2], c->fld[SWAP_F_NP], rs * nn,
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_TMP1], c->fld[SWAP_F_UP], 3 * rs * nn,
+                           hipMemcpyDeviceToDevice, c->stream));
+  const int64_t V = d->g3.V;
+  float* in = d->pin + (size_t)d->npairs * 4 * V;
+  float* tg = d->ptg + (size_t)d->npairs * 4 * V;
+  if (int e = dn_deposit(c, d->cfg.stride)) return e;
+  if (int e = dn_moments_in(c, in)) return e;
+  if (int e = dn_deposit(c, 1)) return e;
+  if (int e = dn_moments_in(c, tg)) return e;
+  HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_NP], c->fld[SWAP_F_TMP2], rs * nn,
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_UP], c->fld[SWAP_F_TMP1], 3 * rs * nn,
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d->npairs++;
+  return 0;
+}
+
+extern "C" int swap_dn_num_pairs(swap_ctx* c) { return c && c->dn ? c->dn->npairs : 0; }
+
+extern "C" int swap_dn_train(swap_ctx* c, int iters, double lr, int ntrain, double* loss) {
+  if (!c || iters < 0 || !(lr > 0)) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  DnState* d = c->dn;
+  if (ntrain < 1 || ntrain > d->npairs) return set_err(c, SWAP_EINVAL, "dn_train: ntrain");
+  const int64_t V = d->g3.V;
+  for (int it = 0; it < iters; it++) {
+    uint32_t r[4] = {(uint32_t)d->t, 0u, 0x60000u, 0u};
+    philox4x32_10(r, (uint32_t)d->cfg.seed, (uint32_t)(d->cfg.seed >> 32));
+    int pidx = (int)(r[0] % (uint32_t)ntrain);
+    HIPCHK(c, hipMemcpyAsync(d->xf, d->pin + (size_t)pidx * 4 * V, 16 * V,
+                             hipMemcpyDeviceToDevice, c->stream));
+    if (int e = dn_forward(c)) return e;
+    if (int e = dn_backward(c, d->ptg + (size_t)pidx * 4 * V)) return e;
+    d->t++;
+    float c1 = (float)(1.0 - pow(0.9, (double)d->t)), c2 = (float)(1.0 - pow(0.999, (double)d->t));
+    hipLaunchKernelGGL(k_adam, dim3(nblk(NPARAM, 256)), dim3(256), 0, c->stream, NPARAM,
+                       d->params, d->grads, d->am, d->av, (float)lr, c1, c2);
+    if (int e = post(c)) return e;
+    if (int e = dn_pack(c)) return e;
+  }
+  double h = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&h, d->dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss) *loss = h;
+  return 0;
+}
+
+extern "C" int swap_dn_eval(swap_ctx* c, int p0, int p1, double* err_raw, double* err_net) {
+  if (!c || !err_raw || !err_net) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  DnState* d = c->dn;
+  if (p0 < 0 || p1 > d->npairs || p0 >= p1) return set_err(c, SWAP_EINVAL, "dn_eval: range");
+  const int64_t V = d->g3.V;
+  double sr = 0.0, sn = 0.0;
+  for (int p = p0; p < p1; p++) {
+    const float* in = d->pin + (size_t)p * 4 * V;
+    const float* tg = d->ptg + (size_t)p * 4 * V;
+    double h;
+    HIPCHK(c, hipMemsetAsync(d->dloss, 0, sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, 4 * V, in, tg, nullptr,
+                       d->dloss);
+    HIPCHK(c, hipMemcpyAsync(&h, d->dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    sr += h;
+    HIPCHK(c, hipMemcpyAsync(d->xf, in, 16 * V, hipMemcpyDeviceToDevice, c->stream));
+    if (int e = dn_forward(c)) return e;
+    HIPCHK(c, hipMemsetAsync(d->dloss, 0, sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, 4 * V, d->out, tg, nullptr,
+                       d->dloss);
+    HIPCHK(c, hipMemcpyAsync(&h, d->dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    sn += h;
+  }
+  *err_raw = sr / (p1 - p0);
+  *err_net = sn / (p1 - p0);
+  d->gate = *err_net < *err_raw;
+  if (!d->gate) d->enabled = false;
+  return 0;
+}
+
+extern "C" int swap_dn_enable(swap_ctx* c, int on) {
+  if (!c) return SWAP_EINVAL;
+  if (!on) {
+    if (c->dn) c->dn->enabled = false;
+    return 0;
+  }
+  if (int e = dn_need(c)) return e;
+  if (!c->dn->gate)
+    return set_err(c, SWAP_ESTATE,
+                   "denoiser gate closed: the last swap_dn_eval did not beat plain deposition");
+  c->dn->enabled = true;
+  return 0;
+}
+
+extern "C" int swap_dn_grad(swap_ctx* c, const float* x, const float* target, double* loss,
+                            float* grads) {
+  if (!c || !x || !target || !loss || !grads) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  DnState* d = c->dn;
+  const int64_t V = d->g3.V;
+  float* tdev = nullptr;
+  HIPCHK(c, hipMalloc(&tdev, 16 * V));
+  int e = host_to_dev(c, target, tdev);
+  if (!e) e = host_to_dev(c, x, d->xf);
+  if (!e) e = dn_forward(c);
+  if (!e) e = dn_backward(c, tdev);
+  if (!e) {
+    HIPCHK(c, hipMemcpyAsync(grads, d->grads, 4 * NPARAM, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(loss, d->dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  (void)hipFree(tdev);
+  return e;
+}
+
+extern "C" int swap_dn_infer(swap_ctx* c, const float* x, float* y) {
+  if (!c || !x || !y) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  DnState* d = c->dn;
+  const int64_t V = d->g3.V;
+  if (int e = host_to_dev(c, x, d->xf)) return e;
+  if (int e = dn_forward(c)) return e;
+  std::vector<float> t(4 * V);
+  HIPCHK(c, hipMemcpyAsync(t.data(), d->out, 16 * V, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int64_t v = 0; v < V; v++)
+    for (int q = 0; q < 4; q++) y[q * V + v] = t[4 * v + q];
+  return 0;
+}
+
+extern "C" int swap_dn_get_params(swap_ctx* c, float* params) {
+  if (!c || !params) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  HIPCHK(c, hipMemcpyAsync(params, c->dn->params, 4 * NPARAM, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int swap_dn_set_params(swap_ctx* c, const float* params) {
+  if (!c || !params) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  HIPCHK(c, hipMemcpyAsync(c->dn->params, params, 4 * NPARAM, hipMemcpyHostToDevice, c->stream));
+  if (int e = dn_pack(c)) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int dn_apply_step(swap_ctx* c, bool with_np) {
+  DnState* d = c->dn;
+  if (!d || !d->enabled) return 0;
+  if (int e = dn_moments_in(c, d->xf)) return e;
+  if (int e = dn_forward(c)) return e;
+  if (c->prec == 64)
+    hipLaunchKernelGGL(k_dn_out<double>, dim3(nblk(c->g.nn, 256)), dim3(256), 0, c->stream,
+                       c->g, d->out, (float)d->cfg.n_scale, (float)d->cfg.v_scale,
+                       with_np ? 1 : 0, (double*)c->fld[SWAP_F_NP], (double*)c->fld[SWAP_F_UP]);
+  else
+    hipLaunchKernelGGL(k_dn_out<float>, dim3(nblk(c->g.nn, 256)), dim3(256), 0, c->stream,
+                       c->g, d->out, (float)d->cfg.n_scale, (float)d->cfg.v_scale,
+                       with_np ? 1 : 0, (float*)c->fld[SWAP_F_NP], (float*)c->fld[SWAP_F_UP]);
+  return post(c);
+}

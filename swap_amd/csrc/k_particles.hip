@@ -25,16 +25,20 @@ __global__ void k_weights(int64_t n, int64_t cap, PDev<R> P, int32_t* ijkp, R* w
 // update_np (mode 0) / update_up (mode 1: vp, 2: 1.5 vp - 0.5 vp1, 3: vp1),
 // exact (fixed-point) global atomics with the ghost fold applied to the node
 // index (gutsp.f:1480-1498, 1704-1805)
+// stride > 1 deposits only the particles with id % stride == 0, each with
+// stride times its weight (a low-ppc subsample of the same particles)
 template <typename R>
 __global__ void k_deposit_simple(Geo g, int64_t n, PDev<R> P, int mode,
-                                 typename AccOf<R>::T* acc) {
+                                 typename AccOf<R>::T* acc, const uint32_t* id, int stride) {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
+  if (stride > 1 && id[p] % (uint32_t)stride != 0u) return;
   uint32_t c = P.cell[p];
   int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
   R w[8];
   weights8(P.fx[p], P.fy[p], P.fz[p], w);
   R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
+  if (stride > 1) r *= (R)stride;
   R q[3] = {0, 0, 0};
   if (mode == 1) { q[0] = P.vx[p]; q[1] = P.vy[p]; q[2] = P.vz[p]; }
   else if (mode == 2) {
@@ -169,10 +173,11 @@ int pl_weights(swap_ctx* c) {
   PDISPATCH(c, PL((k_weights<R>), c->n, c->n, c->cap, pdev<R>(c), c->ijkp, (R*)c->wght));
   return post(c);
 }
-int pl_deposit_simple(swap_ctx* c, int mode) {
+int pl_deposit_simple(swap_ctx* c, int mode, int stride) {
   if (c->n == 0) return 0;
+  if (stride > 1 && !c->id) return set_err(c, SWAP_ESTATE, "subsampled deposit needs ids");
   PDISPATCH(c, PL((k_deposit_simple<R>), c->n, c->g, c->n, pdev<R>(c), mode,
-                  (typename AccOf<R>::T*)c->macc));
+                  (typename AccOf<R>::T*)c->macc, c->id, stride));
   return post(c);
 }
 int pl_get_Ep(swap_ctx* c) {

@@ -293,6 +293,7 @@ int swap_destroy(swap_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   prof_clear(c);
+  dn_destroy(c);
   void* ps[] = {c->b1, c->b12, c->b1p2, c->a, c->ct, c->btc0, c->cE, c->S, c->macc,
                 c->scratch, c->dred, c->dmin, c->dcnt, c->d_nitems, c->cell, c->cell2, c->bp,
                 c->bp2, c->mr, c->mr2, c->id, c->id2, c->Ep, c->vpl, c->vmi, c->wght, c->ijkp,
@@ -805,6 +806,7 @@ int swap_step(swap_ctx* c, int nsteps) {
       Stage t(c, SWAP_K_GLUE);
       TRY(fl_finalize_np(c));
       TRY(fl_finalize_up(c));
+      TRY(dn_apply_step(c, true));  // learned denoiser, when enabled
       // node sources: aj = curlB(bt,np), btc, ajc - upc (maind.f:395-398, gutsp.f:183-184)
       TRY(fl_curlB(c, c->b1, nullptr, c->fld[SWAP_F_AJ], nullptr));
       TRY(fl_e2c_B(c, c->b1, nullptr, c->fld[SWAP_F_BTC]));
@@ -820,6 +822,7 @@ int swap_step(swap_ctx* c, int nsteps) {
     {
       Stage t(c, SWAP_K_GLUE);
       TRY(fl_finalize_up(c));
+      TRY(dn_apply_step(c, false));
       TRY(fl_update_src(c, c->fld[SWAP_F_UP], c->fld[SWAP_F_AJC], c->S));
       TRY(zero_moments(c, 0, 4));
       TRY(pl_box_reset(c));
@@ -833,6 +836,7 @@ int swap_step(swap_ctx* c, int nsteps) {
       Stage t(c, SWAP_K_GLUE);
       TRY(fl_finalize_np(c));
       TRY(fl_finalize_up(c));
+      TRY(dn_apply_step(c, true));
     }
     // P3 reduced the cell boxes of x^{n+1} and the bounds of v^{n+1/2}
     std::swap(c->vb, c->vb_next);

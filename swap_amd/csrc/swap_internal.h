@@ -92,6 +92,8 @@ struct swap_ctx {
   void* scratch = nullptr;    // general device scratch
   size_t scratch_bytes = 0;
   int64_t step_count = 0;
+  // learned moment denoiser (k_denoise.hip); null until swap_dn_create
+  struct DnState* dn = nullptr;
   // stage timing (swap_profile): event pairs recorded on `stream`
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev[SWAP_K_COUNT];  // start, stop, start, stop ...
@@ -131,7 +133,8 @@ int fl_zero(swap_ctx* c, void* p, size_t bytes);
 
 // ---- particle launchers (k_particles.hip) ----
 int pl_weights(swap_ctx* c);
-int pl_deposit_simple(swap_ctx* c, int mode);  // 0: np, 1: up(vp) 2: up(ext) 3: up(vp1)
+// 0: np, 1: up(vp) 2: up(ext) 3: up(vp1); stride > 1: the id % stride == 0 subsample
+int pl_deposit_simple(swap_ctx* c, int mode, int stride = 1);
 int pl_get_Ep(swap_ctx* c);
 int pl_vplus_vminus(swap_ctx* c);
 int pl_improve_vp1(swap_ctx* c);
@@ -160,6 +163,11 @@ int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* inject
 int sr_diagnostics(swap_ctx* c, swap_diag* d);
 int sr_outflow_compact(swap_ctx* c);
 int sr_vbounds(swap_ctx* c);  // signed per-axis velocity bounds into c->vb
+
+// ---- denoiser (k_denoise.hip) ----
+// replace np (with_np) and up by the net's output when enabled; no-op otherwise
+int dn_apply_step(swap_ctx* c, bool with_np);
+void dn_destroy(swap_ctx* c);
 
 // helpers
 template <typename R>

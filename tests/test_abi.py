@@ -87,7 +87,8 @@ def c_layout(struct, fields):
 
 
 @pytest.mark.parametrize("cname, pyname", [("swap_config", "Config"), ("swap_diag", "Diag"),
-                                           ("swap_cloud", "Cloud")])
+                                           ("swap_cloud", "Cloud"),
+                                           ("swap_dn_config", "DnConfig")])
 def test_struct_layouts_match(cname, pyname):
     from swap_amd import api
     cls = getattr(api, pyname)
