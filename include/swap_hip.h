@@ -47,7 +47,11 @@ enum { SWAP_FP32 = 32, SWAP_FP64 = 64 };
 enum {
   SWAP_SPECIES = 1,        /* per-particle beta_p and mrat arrays exist */
   SWAP_OUTFLOW_X = 2,      /* x is inflow/outflow instead of periodic */
-  SWAP_TRACK_IDS = 4       /* keep a particle id through sorts (tests) */
+  SWAP_TRACK_IDS = 4,      /* keep a particle id through sorts (tests) */
+  SWAP_UNFUSED_FIELDS = 8  /* field subcycles of swap_step as one kernel per
+                              reference routine (nine launches) instead of the
+                              fused LDS-tiled predictor and corrector (two);
+                              the same arithmetic, kept for checking */
 };
 
 typedef struct {
@@ -239,10 +243,18 @@ typedef struct {
 int64_t swap_dn_num_params(void);
 int swap_dn_create(swap_ctx *ctx, const swap_dn_config *cfg);
 /* append a training pair from the current particles (needs SWAP_TRACK_IDS):
-   input = the stride subsample's moments, target = all particles' moments;
-   the context's np/up fields are left as they were */
-int swap_dn_add_pair(swap_ctx *ctx);
+   input = the stride subsample's moments, target = all particles' moments,
+   or with disjoint != 0 the moments of the other particles only
+   (id % stride != 0, weight x stride/(stride-1)), whose sampling noise is
+   independent of the input's; the context's np/up fields are left as they
+   were */
+int swap_dn_add_pair(swap_ctx *ctx, int disjoint);
 int swap_dn_num_pairs(swap_ctx *ctx);
+/* copy pair k out ([4][ncz][ncy][ncx] FP32 each, normalised), or append a
+   pair given on the host (e.g. held-out pairs of a high-ppc run, for the
+   gate of a low-ppc context) */
+int swap_dn_get_pair(swap_ctx *ctx, int k, float *x, float *target);
+int swap_dn_put_pair(swap_ctx *ctx, const float *x, const float *target);
 /* iters Adam steps (learning rate lr) on pairs drawn from [0, ntrain); *loss
    = mean squared error (normalised units) of the last step */
 int swap_dn_train(swap_ctx *ctx, int iters, double lr, int ntrain, double *loss);

@@ -27,7 +27,7 @@ P_XP, P_VP, P_VP1, P_EP, P_VPLUS, P_VMINUS, P_IJKP, P_WGHT, P_CELL, P_FRAC, P_BE
     P_ID = range(13)
 V_CUR, V_EXTRAP, V_VP1 = 0, 1, 2
 FP32, FP64 = 32, 64
-SPECIES, OUTFLOW_X, TRACK_IDS = 1, 2, 4
+SPECIES, OUTFLOW_X, TRACK_IDS, UNFUSED_FIELDS = 1, 2, 4, 8
 
 
 class SwapError(RuntimeError):
@@ -125,8 +125,10 @@ def load_library():
         "swap_profile_read": ([vp, vp, vp, i32], i32),
         "swap_dn_num_params": ([], i64),
         "swap_dn_create": ([vp, C.POINTER(DnConfig)], i32),
-        "swap_dn_add_pair": ([vp], i32),
+        "swap_dn_add_pair": ([vp, i32], i32),
         "swap_dn_num_pairs": ([vp], i32),
+        "swap_dn_get_pair": ([vp, i32, vp, vp], i32),
+        "swap_dn_put_pair": ([vp, vp, vp], i32),
         "swap_dn_train": ([vp, i32, dbl, i32, C.POINTER(dbl)], i32),
         "swap_dn_eval": ([vp, i32, i32, C.POINTER(dbl), C.POINTER(dbl)], i32),
         "swap_dn_enable": ([vp, i32], i32),
@@ -383,11 +385,22 @@ class Swap:
         cfg = DnConfig(n_scale, v_scale, max_pairs, stride, seed)
         self._ck(self.lib.swap_dn_create(self.h, C.byref(cfg)), "dn_create")
 
-    def dn_add_pair(self):
-        self._ck(self.lib.swap_dn_add_pair(self.h), "dn_add_pair")
+    def dn_add_pair(self, disjoint=False):
+        self._ck(self.lib.swap_dn_add_pair(self.h, 1 if disjoint else 0), "dn_add_pair")
 
     def dn_num_pairs(self):
         return self.lib.swap_dn_num_pairs(self.h)
+
+    def dn_get_pair(self, k):
+        x = np.zeros((4, self.nz - 2, self.ny - 2, self.nx - 2), np.float32)
+        t = np.zeros_like(x)
+        self._ck(self.lib.swap_dn_get_pair(self.h, k, x.ctypes.data, t.ctypes.data),
+                 "dn_get_pair")
+        return x, t
+
+    def dn_put_pair(self, x, target):
+        x, t = self._grid4(x), self._grid4(target)
+        self._ck(self.lib.swap_dn_put_pair(self.h, x.ctypes.data, t.ctypes.data), "dn_put_pair")
 
     def dn_train(self, iters, lr, ntrain):
         loss = C.c_double()

@@ -422,6 +422,17 @@ int host_to_dev(swap_ctx* c, const float* x, float* dst) {
   return 0;
 }
 
+// [V][4] device -> [4][V] host
+int dev_to_host(swap_ctx* c, const float* src, float* x) {
+  const int64_t V = c->dn->g3.V;
+  std::vector<float> t(4 * V);
+  HIPCHK(c, hipMemcpyAsync(t.data(), src, 16 * V, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int64_t v = 0; v < V; v++)
+    for (int q = 0; q < 4; q++) x[q * V + v] = t[4 * v + q];
+  return 0;
+}
+
 }  // namespace
 
 // ---------------- C-ABI ----------------
@@ -487,7 +498,7 @@ void dn_destroy(swap_ctx* c) {
   c->dn = nullptr;
 }
 
-extern "C" int swap_dn_add_pair(swap_ctx* c) {
+extern "C" int swap_dn_add_pair(swap_ctx* c, int disjoint) {
   if (!c) return SWAP_EINVAL;
   if (int e = dn_need(c)) return e;
   DnState* d = c->dn;
@@ -505,7 +516,9 @@ extern "C" int swap_dn_add_pair(swap_ctx* c) {
   float* tg = d->ptg + (size_t)d->npairs * 4 * V;
   if (int e = dn_deposit(c, d->cfg.stride)) return e;
   if (int e = dn_moments_in(c, in)) return e;
-  if (int e = dn_deposit(c, 1)) return e;
+  if (disjoint && d->cfg.stride < 2)
+    return set_err(c, SWAP_EINVAL, "dn_add_pair: a disjoint target needs stride > 1");
+  if (int e = dn_deposit(c, disjoint ? -d->cfg.stride : 1)) return e;
   if (int e = dn_moments_in(c, tg)) return e;
   HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_NP], c->fld[SWAP_F_TMP2], rs * nn,
                            hipMemcpyDeviceToDevice, c->stream));
@@ -517,6 +530,29 @@ extern "C" int swap_dn_add_pair(swap_ctx* c) {
 }
 
 extern "C" int swap_dn_num_pairs(swap_ctx* c) { return c && c->dn ? c->dn->npairs : 0; }
+
+extern "C" int swap_dn_get_pair(swap_ctx* c, int k, float* x, float* target) {
+  if (!c || !x || !target) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  DnState* d = c->dn;
+  if (k < 0 || k >= d->npairs) return set_err(c, SWAP_EINVAL, "dn_get_pair: k");
+  const int64_t V = d->g3.V;
+  if (int e = dev_to_host(c, d->pin + (size_t)k * 4 * V, x)) return e;
+  return dev_to_host(c, d->ptg + (size_t)k * 4 * V, target);
+}
+
+extern "C" int swap_dn_put_pair(swap_ctx* c, const float* x, const float* target) {
+  if (!c || !x || !target) return SWAP_EINVAL;
+  if (int e = dn_need(c)) return e;
+  DnState* d = c->dn;
+  if (d->npairs >= d->cfg.max_pairs) return set_err(c, SWAP_ECAP, "dn_put_pair: pairs full");
+  const int64_t V = d->g3.V;
+  if (int e = host_to_dev(c, x, d->pin + (size_t)d->npairs * 4 * V)) return e;
+  if (int e = host_to_dev(c, target, d->ptg + (size_t)d->npairs * 4 * V)) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d->npairs++;
+  return 0;
+}
 
 extern "C" int swap_dn_train(swap_ctx* c, int iters, double lr, int ntrain, double* loss) {
   if (!c || iters < 0 || !(lr > 0)) return SWAP_EINVAL;

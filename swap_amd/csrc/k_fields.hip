@@ -364,6 +364,332 @@ __global__ void k_halving(Geo g, const R* dtn, R dtsub0, int* out_h) {
   if (threadIdx.x == 0) *out_h = h;
 }
 
+// ---------------- fused, LDS-tiled field subcycle ----------------
+//
+// One subcycle (maind.f:420-476) in two launches instead of nine.
+//   predictor: aj = curlB(b1)/(alpha n), a = aj - up; ct = face_to_center(a)
+//     x edge_to_center(b1); E = edge average of ct + nu aj; b1p2 = b12 -
+//     2 dtsub curlE(E)                                   (gutsf.f:1115-1169)
+//   corrector: the same chain on bh = (b1p2 + b1)/2 with get_Ep1's smoothed
+//     B (edge_to_center, then node->face->centre averages); b = b1 - dtsub
+//     curlE(E)                                 (gutsf.f:1174-1304, 1309-1359)
+// A workgroup owns TX x TY columns of canonical nodes and a run of z planes.
+// It marches up z keeping a ring of planes of every intermediate in LDS, each
+// with the x/y halo its consumer's stencil needs, so the inputs are read once
+// per tile (halo re-reads are L2 hits) and no intermediate goes to HBM.  The
+// arithmetic is that of the per-routine kernels above, term by term, in the
+// same precisions.  Results are written to every storage image of a
+// canonical node (ghosts included), so the fields stay periodic.
+//
+// LDS plane layout: tile position (x, y) relative to the origin (X0, Y0),
+// x in [-2, TX+2], y in [-2, TY+2], at index (x+2) + PW (y+2); each position
+// holds the value of the canonical node it wraps to.
+
+// canonical index [1, n-2] of any (unwrapped) storage coordinate s
+__device__ __forceinline__ int wrapc(int s, int n) {
+  const int m = n - 2;
+  int r = (s - 1) % m;
+  if (r < 0) r += m;
+  return r + 1;
+}
+
+// storage indices that hold canonical index c along an axis of n nodes
+__device__ __forceinline__ int images(int c, int n, int s[2]) {
+  s[0] = c;
+  if (c == 1) { s[1] = n - 1; return 2; }
+  if (c == n - 2) { s[1] = 0; return 2; }
+  return 1;
+}
+
+template <typename T>
+__device__ __forceinline__ void put3(const Geo& g, T* dst, int x, int y, int z, T v0, T v1,
+                                     T v2) {
+  int xs[2], ys[2], zs[2];
+  const int ax = images(x, g.nx, xs), ay = images(y, g.ny, ys), az = images(z, g.nz, zs);
+  for (int c = 0; c < az; c++)
+    for (int b = 0; b < ay; b++)
+      for (int a = 0; a < ax; a++) {
+        const int64_t o = gidx(g, xs[a], ys[b], zs[c]);
+        dst[o] = v0;
+        dst[o + g.nn] = v1;
+        dst[o + 2 * g.nn] = v2;
+      }
+}
+
+template <int TX, int TY>
+struct FT {
+  static constexpr int PW = TX + 5, PH = TY + 5, PA = PW * PH, NT = TX * TY;
+};
+
+// 3-slot ring index of plane z (z may be negative)
+__device__ __forceinline__ int r3(int z) { return ((z % 3) + 3) % 3; }
+
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const double* __restrict__ b1,
+                                                        const double* __restrict__ b12,
+                                                        const R* __restrict__ np,
+                                                        const R* __restrict__ up,
+                                                        const R* __restrict__ nu, double coef,
+                                                        int tz, double* __restrict__ out) {
+  constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA, NT = TX * TY;
+  __shared__ double sb[2][3][PA];  // b1, planes s+1, s+2
+  __shared__ R sn[2][PA];          // np, planes s+2, s+3
+  __shared__ R sa[2][3][PA];       // a = aj - up
+  __shared__ R sj[2][3][PA];       // aj
+  __shared__ R sc[2][3][PA];       // ct
+  __shared__ R se[2][3][PA];       // E
+  __shared__ int tcx[PW], tcy[PH];
+  const int tid = threadIdx.x;
+  const int X0 = 1 + blockIdx.x * TX, Y0 = 1 + blockIdx.y * TY, Z0 = 1 + blockIdx.z * tz;
+  const int Z1 = min(Z0 + tz, g.ncz + 1);
+  if (Z0 > g.ncz) return;  // empty z chunk (whole workgroup)
+  for (int t = tid; t < PW; t += NT) tcx[t] = wrapc(X0 - 2 + t, g.nx);
+  for (int t = tid; t < PH; t += NT) tcy[t] = wrapc(Y0 - 2 + t, g.ny);
+  __syncthreads();
+  const int64_t nxy = (int64_t)g.nx * g.ny, nn = g.nn;
+  const R al = (R)f.alpha;
+  for (int s = Z0 - 4; s < Z1; s++) {
+    {  // b1 plane s+2, np plane s+3
+      const int64_t pb = wrapc(s + 2, g.nz) * nxy, pn = wrapc(s + 3, g.nz) * nxy;
+      const int rb = (s + 2) & 1, rn = (s + 3) & 1;
+      for (int p = tid; p < PA; p += NT) {
+        const int64_t q = tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
+        sb[rb][0][p] = b1[pb + q];
+        sb[rb][1][p] = b1[pb + q + nn];
+        sb[rb][2][p] = b1[pb + q + 2 * nn];
+        sn[rn][p] = np[pn + q];
+      }
+    }
+    __syncthreads();
+    if (s >= Z0 - 3) {  // aj, a at plane s+2 over x, y in [-1, T+1]
+      const int r = (s + 2) & 1, rm = r ^ 1;
+      const int64_t pz = wrapc(s + 2, g.nz) * nxy;
+      constexpr int W = TX + 3;
+      for (int t = tid; t < W * (TY + 3); t += NT) {
+        const int p = (t % W + 1) + PW * (t / W + 1);
+        const R n0 = sn[r][p];
+        const R nt1 = (R)0.5 * (n0 + sn[r][p + 1]);
+        const R nt2 = (R)0.5 * (n0 + sn[r][p + PW]);
+        const R nt3 = (R)0.5 * (n0 + sn[rm][p]);
+        const double c1 = (sb[r][2][p] - sb[r][2][p - PW]) / f.dy +
+                          (sb[rm][1][p] - sb[r][1][p]) / f.dz;
+        const double c2 = (sb[r][0][p] - sb[rm][0][p]) / f.dz +
+                          (sb[r][2][p - 1] - sb[r][2][p]) / f.dx;
+        const double c3 = (sb[r][1][p] - sb[r][1][p - 1]) / f.dx +
+                          (sb[r][0][p - PW] - sb[r][0][p]) / f.dy;
+        const R j1 = (R)c1 / (nt1 * al), j2 = (R)c2 / (nt2 * al), j3 = (R)c3 / (nt3 * al);
+        const int64_t oc = pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
+        sj[r][0][p] = j1;
+        sj[r][1][p] = j2;
+        sj[r][2][p] = j3;
+        sa[r][0][p] = j1 - up[oc];
+        sa[r][1][p] = j2 - up[oc + nn];
+        sa[r][2][p] = j3 - up[oc + 2 * nn];
+      }
+    }
+    __syncthreads();
+    if (s >= Z0 - 2) {  // ct at plane s+2 over x, y in [0, T+1]
+      const int r = (s + 2) & 1, rm = r ^ 1;
+      constexpr int W = TX + 2;
+      for (int t = tid; t < W * (TY + 2); t += NT) {
+        const int p = (t % W + 2) + PW * (t / W + 2);
+        const R aa0 = (R)f.rx * (sa[r][0][p] - sa[r][0][p - 1]) + sa[r][0][p - 1];
+        const R aa1 = (R)f.ry * (sa[r][1][p] - sa[r][1][p - PW]) + sa[r][1][p - PW];
+        const R aa2 = (R)f.rz * (sa[r][2][p] - sa[rm][2][p]) + sa[rm][2][p];
+        double hi, lo;
+        hi = sb[r][0][p - PW] + f.ry * (sb[r][0][p] - sb[r][0][p - PW]);
+        lo = sb[rm][0][p - PW] + f.ry * (sb[rm][0][p] - sb[rm][0][p - PW]);
+        const double bx = lo + f.rz * (hi - lo);
+        hi = sb[r][1][p - 1] + f.rx * (sb[r][1][p] - sb[r][1][p - 1]);
+        lo = sb[rm][1][p - 1] + f.rx * (sb[rm][1][p] - sb[rm][1][p - 1]);
+        const double by = lo + f.rz * (hi - lo);
+        hi = sb[r][2][p - PW] + f.ry * (sb[r][2][p] - sb[r][2][p - PW]);
+        lo = sb[r][2][p - 1 - PW] + f.ry * (sb[r][2][p - 1] - sb[r][2][p - 1 - PW]);
+        const double bz = lo + f.rx * (hi - lo);
+        const R Bx = (R)bx, By = (R)by, Bz = (R)bz;
+        sc[r][0][p] = aa1 * Bz - aa2 * By;
+        sc[r][1][p] = aa2 * Bx - aa0 * Bz;
+        sc[r][2][p] = aa0 * By - aa1 * Bx;
+      }
+    }
+    __syncthreads();
+    if (s >= Z0 - 1) {  // E at plane s+1 over x, y in [0, T]
+      const int r = (s + 1) & 1, rp = r ^ 1;
+      const int64_t pz = wrapc(s + 1, g.nz) * nxy;
+      constexpr int W = TX + 1;
+      for (int t = tid; t < W * (TY + 1); t += NT) {
+        const int p = (t % W + 2) + PW * (t / W + 2);
+        const R nuv = nu[pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW]];
+        se[r][0][p] = (R)0.5 * (sc[r][0][p] + sc[r][0][p + 1]) + nuv * sj[r][0][p];
+        se[r][1][p] = (R)0.5 * (sc[r][1][p] + sc[r][1][p + PW]) + nuv * sj[r][1][p];
+        se[r][2][p] = (R)0.5 * (sc[r][2][p] + sc[rp][2][p]) + nuv * sj[r][2][p];
+      }
+    }
+    __syncthreads();
+    if (s >= Z0) {  // b1p2 = b12 - coef curlE(E) at plane s
+      const int r = s & 1, rp = r ^ 1;
+      const int lx = tid % TX, ly = tid / TX, x = X0 + lx, y = Y0 + ly;
+      if (x <= g.ncx && y <= g.ncy) {
+        const int p = (lx + 2) + PW * (ly + 2);
+        const double c0 = ((double)se[r][2][p + PW] - (double)se[r][2][p]) / f.dy +
+                          ((double)se[r][1][p] - (double)se[rp][1][p]) / f.dz;
+        const double c1 = ((double)se[r][2][p] - (double)se[r][2][p + 1]) / f.dx +
+                          ((double)se[rp][0][p] - (double)se[r][0][p]) / f.dz;
+        const double c2 = ((double)se[r][0][p] - (double)se[r][0][p + PW]) / f.dy +
+                          ((double)se[r][1][p + 1] - (double)se[r][1][p]) / f.dx;
+        const int64_t oc = gidx(g, x, y, s);
+        put3<double>(g, out, x, y, s, b12[oc] - coef * c0, b12[oc + nn] - coef * c1,
+                     b12[oc + 2 * nn] - coef * c2);
+      }
+    }
+  }
+}
+
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const double* __restrict__ bp,
+                                                        const double* __restrict__ b1,
+                                                        const R* __restrict__ np,
+                                                        const R* __restrict__ up,
+                                                        const R* __restrict__ nu, double coef,
+                                                        int tz, double* __restrict__ out,
+                                                        R* __restrict__ Eout,
+                                                        R* __restrict__ ajout) {
+  constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA, NT = TX * TY;
+  __shared__ double sb[2][3][PA];  // bh = (b1p2 + b1)/2, planes s+2, s+3
+  __shared__ R sn[2][PA];          // np, planes s+3, s+4
+  __shared__ R sa[3][3][PA];       // a = aj - up, planes s+1 .. s+3
+  __shared__ R sj[3][3][PA];       // aj
+  __shared__ R sw[3][3][PA];       // btc0 = edge_to_center(bh)
+  __shared__ R sc[2][3][PA];       // ct
+  __shared__ R se[2][3][PA];       // E
+  __shared__ int tcx[PW], tcy[PH];
+  const int tid = threadIdx.x;
+  const int X0 = 1 + blockIdx.x * TX, Y0 = 1 + blockIdx.y * TY, Z0 = 1 + blockIdx.z * tz;
+  const int Z1 = min(Z0 + tz, g.ncz + 1);
+  if (Z0 > g.ncz) return;  // empty z chunk (whole workgroup)
+  for (int t = tid; t < PW; t += NT) tcx[t] = wrapc(X0 - 2 + t, g.nx);
+  for (int t = tid; t < PH; t += NT) tcy[t] = wrapc(Y0 - 2 + t, g.ny);
+  __syncthreads();
+  const int64_t nxy = (int64_t)g.nx * g.ny, nn = g.nn;
+  const R al = (R)f.alpha;
+  for (int s = Z0 - 5; s < Z1; s++) {
+    {  // bh plane s+3, np plane s+4
+      const int64_t pb = wrapc(s + 3, g.nz) * nxy, pn = wrapc(s + 4, g.nz) * nxy;
+      const int rb = (s + 3) & 1, rn = (s + 4) & 1;
+      for (int p = tid; p < PA; p += NT) {
+        const int64_t q = tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
+#pragma unroll
+        for (int m = 0; m < 3; m++) sb[rb][m][p] = 0.5 * (bp[pb + q + m * nn] + b1[pb + q + m * nn]);
+        sn[rn][p] = np[pn + q];
+      }
+    }
+    __syncthreads();
+    if (s >= Z0 - 4) {  // aj, a over [-1, T+1] and btc0 over [-1, T+2], plane s+3
+      const int r = (s + 3) & 1, rm = r ^ 1, w = r3(s + 3);
+      const int64_t pz = wrapc(s + 3, g.nz) * nxy;
+      constexpr int W = TX + 3;
+      for (int t = tid; t < W * (TY + 3); t += NT) {
+        const int p = (t % W + 1) + PW * (t / W + 1);
+        const R n0 = sn[r][p];
+        const R nt1 = (R)0.5 * (n0 + sn[r][p + 1]);
+        const R nt2 = (R)0.5 * (n0 + sn[r][p + PW]);
+        const R nt3 = (R)0.5 * (n0 + sn[rm][p]);
+        const double c1 = (sb[r][2][p] - sb[r][2][p - PW]) / f.dy +
+                          (sb[rm][1][p] - sb[r][1][p]) / f.dz;
+        const double c2 = (sb[r][0][p] - sb[rm][0][p]) / f.dz +
+                          (sb[r][2][p - 1] - sb[r][2][p]) / f.dx;
+        const double c3 = (sb[r][1][p] - sb[r][1][p - 1]) / f.dx +
+                          (sb[r][0][p - PW] - sb[r][0][p]) / f.dy;
+        const R j1 = (R)c1 / (nt1 * al), j2 = (R)c2 / (nt2 * al), j3 = (R)c3 / (nt3 * al);
+        const int64_t oc = pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
+        sj[w][0][p] = j1;
+        sj[w][1][p] = j2;
+        sj[w][2][p] = j3;
+        sa[w][0][p] = j1 - up[oc];
+        sa[w][1][p] = j2 - up[oc + nn];
+        sa[w][2][p] = j3 - up[oc + 2 * nn];
+      }
+      constexpr int W2 = TX + 4;
+      for (int t = tid; t < W2 * (TY + 4); t += NT) {
+        const int p = (t % W2 + 1) + PW * (t / W2 + 1);
+        double hi, lo;
+        hi = sb[r][0][p - PW] + f.ry * (sb[r][0][p] - sb[r][0][p - PW]);
+        lo = sb[rm][0][p - PW] + f.ry * (sb[rm][0][p] - sb[rm][0][p - PW]);
+        sw[w][0][p] = (R)(lo + f.rz * (hi - lo));
+        hi = sb[r][1][p - 1] + f.rx * (sb[r][1][p] - sb[r][1][p - 1]);
+        lo = sb[rm][1][p - 1] + f.rx * (sb[rm][1][p] - sb[rm][1][p - 1]);
+        sw[w][1][p] = (R)(lo + f.rz * (hi - lo));
+        hi = sb[r][2][p - PW] + f.ry * (sb[r][2][p] - sb[r][2][p - PW]);
+        lo = sb[r][2][p - 1 - PW] + f.ry * (sb[r][2][p - 1] - sb[r][2][p - 1 - PW]);
+        sw[w][2][p] = (R)(lo + f.rx * (hi - lo));
+      }
+    }
+    __syncthreads();
+    if (s >= Z0 - 2) {  // ct at plane s+2 over [0, T+1]: a x smoothed B
+      const int r = (s + 2) & 1, wc = r3(s + 2), wm = r3(s + 1), wp = r3(s + 3);
+      constexpr int W = TX + 2;
+      for (int t = tid; t < W * (TY + 2); t += NT) {
+        const int p = (t % W + 2) + PW * (t / W + 2);
+        const R aa0 = (R)f.rx * (sa[wc][0][p] - sa[wc][0][p - 1]) + sa[wc][0][p - 1];
+        const R aa1 = (R)f.ry * (sa[wc][1][p] - sa[wc][1][p - PW]) + sa[wc][1][p - PW];
+        const R aa2 = (R)f.rz * (sa[wc][2][p] - sa[wm][2][p]) + sa[wm][2][p];
+        const R m1c = (R)0.5 * (sw[wc][0][p] + sw[wc][0][p + 1]);
+        const R m1m = (R)0.5 * (sw[wc][0][p - 1] + sw[wc][0][p]);
+        const R m2c = (R)0.5 * (sw[wc][1][p] + sw[wc][1][p + PW]);
+        const R m2m = (R)0.5 * (sw[wc][1][p - PW] + sw[wc][1][p]);
+        const R m3c = (R)0.5 * (sw[wc][2][p] + sw[wp][2][p]);
+        const R m3m = (R)0.5 * (sw[wm][2][p] + sw[wc][2][p]);
+        const R bx = (R)f.rx * (m1c - m1m) + m1m;
+        const R by = (R)f.ry * (m2c - m2m) + m2m;
+        const R bz = (R)f.rz * (m3c - m3m) + m3m;
+        sc[r][0][p] = aa1 * bz - aa2 * by;
+        sc[r][1][p] = aa2 * bx - aa0 * bz;
+        sc[r][2][p] = aa0 * by - aa1 * bx;
+      }
+    }
+    __syncthreads();
+    if (s >= Z0 - 1) {  // E at plane s+1 over [0, T]
+      const int r = (s + 1) & 1, rp = r ^ 1, w = r3(s + 1);
+      const int ze = wrapc(s + 1, g.nz);
+      const int64_t pz = ze * nxy;
+      const bool store = Eout && s + 1 >= Z0 && s + 1 < Z1;
+      constexpr int W = TX + 1;
+      for (int t = tid; t < W * (TY + 1); t += NT) {
+        const int lx = t % W, ly = t / W;
+        const int p = (lx + 2) + PW * (ly + 2);
+        const R nuv = nu[pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW]];
+        const R e0 = (R)0.5 * (sc[r][0][p] + sc[r][0][p + 1]) + nuv * sj[w][0][p];
+        const R e1 = (R)0.5 * (sc[r][1][p] + sc[r][1][p + PW]) + nuv * sj[w][1][p];
+        const R e2 = (R)0.5 * (sc[r][2][p] + sc[rp][2][p]) + nuv * sj[w][2][p];
+        se[r][0][p] = e0;
+        se[r][1][p] = e1;
+        se[r][2][p] = e2;
+        if (store && lx < TX && ly < TY && X0 + lx <= g.ncx && Y0 + ly <= g.ncy) {
+          put3<R>(g, Eout, X0 + lx, Y0 + ly, ze, e0, e1, e2);
+          put3<R>(g, ajout, X0 + lx, Y0 + ly, ze, sj[w][0][p], sj[w][1][p], sj[w][2][p]);
+        }
+      }
+    }
+    __syncthreads();
+    if (s >= Z0) {  // b = b1 - coef curlE(E) at plane s
+      const int r = s & 1, rp = r ^ 1;
+      const int lx = tid % TX, ly = tid / TX, x = X0 + lx, y = Y0 + ly;
+      if (x <= g.ncx && y <= g.ncy) {
+        const int p = (lx + 2) + PW * (ly + 2);
+        const double c0 = ((double)se[r][2][p + PW] - (double)se[r][2][p]) / f.dy +
+                          ((double)se[r][1][p] - (double)se[rp][1][p]) / f.dz;
+        const double c1 = ((double)se[r][2][p] - (double)se[r][2][p + 1]) / f.dx +
+                          ((double)se[rp][0][p] - (double)se[r][0][p]) / f.dz;
+        const double c2 = ((double)se[r][0][p] - (double)se[r][0][p + PW]) / f.dy +
+                          ((double)se[r][1][p + 1] - (double)se[r][1][p]) / f.dx;
+        const int64_t oc = gidx(g, x, y, s);
+        put3<double>(g, out, x, y, s, b1[oc] - coef * c0, b1[oc + nn] - coef * c1,
+                     b1[oc + 2 * nn] - coef * c2);
+      }
+    }
+  }
+}
+
 template <typename R>
 R* F(swap_ctx* c, int id) {
   return (R*)c->fld[id];
@@ -518,6 +844,26 @@ int fl_copy_b(swap_ctx* c, const double* src, double* dst) {
   HIPCHK(c, hipMemcpyAsync(dst, src, sizeof(double) * 3 * c->g.nn, hipMemcpyDeviceToDevice,
                            c->stream));
   return 0;
+}
+// one fused subcycle: b1p2 = predictor(b1, b12); b12's buffer = corrector
+// (b1p2, b1); E and aj of the corrector are stored when last is set
+int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last) {
+  constexpr int TX = 32, TY = 4, TZ = 32;
+  FC f = fc_of(c);
+  const Geo& g = c->g;
+  const int nzc = (g.ncz + TZ - 1) / TZ, tz = (g.ncz + nzc - 1) / nzc;
+  const dim3 grid((g.ncx + TX - 1) / TX, (g.ncy + TY - 1) / TY, nzc), block(TX * TY);
+  DISPATCH(c, hipLaunchKernelGGL((k_sub_predict<R, TX, TY>), grid, block, 0, c->stream, g, f,
+                                 (const double*)c->b1, (const double*)c->b12,
+                                 (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
+                                 (const R*)F<R>(c, SWAP_F_NU), 2.0 * dtsub, tz, c->b1p2);
+           hipLaunchKernelGGL((k_sub_correct<R, TX, TY>), grid, block, 0, c->stream, g, f,
+                              (const double*)c->b1p2, (const double*)c->b1,
+                              (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
+                              (const R*)F<R>(c, SWAP_F_NU), dtsub, tz, c->b12,
+                              last ? F<R>(c, SWAP_F_E) : (R*)nullptr,
+                              last ? F<R>(c, SWAP_F_AJ) : (R*)nullptr));
+  return post(c);
 }
 int fl_zero(swap_ctx* c, void* p, size_t bytes) {
   HIPCHK(c, hipMemsetAsync(p, 0, bytes, c->stream));

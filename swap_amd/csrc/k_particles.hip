@@ -26,19 +26,23 @@ __global__ void k_weights(int64_t n, int64_t cap, PDev<R> P, int32_t* ijkp, R* w
 // exact (fixed-point) global atomics with the ghost fold applied to the node
 // index (gutsp.f:1480-1498, 1704-1805)
 // stride > 1 deposits only the particles with id % stride == 0, each with
-// stride times its weight (a low-ppc subsample of the same particles)
+// stride times its weight (a low-ppc subsample of the same particles);
+// stride < -1 deposits the complement, id % |stride| != 0, each with
+// |stride| / (|stride| - 1) times its weight (a sample disjoint from it)
 template <typename R>
 __global__ void k_deposit_simple(Geo g, int64_t n, PDev<R> P, int mode,
                                  typename AccOf<R>::T* acc, const uint32_t* id, int stride) {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
   if (stride > 1 && id[p] % (uint32_t)stride != 0u) return;
+  if (stride < -1 && id[p] % (uint32_t)(-stride) == 0u) return;
   uint32_t c = P.cell[p];
   int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
   R w[8];
   weights8(P.fx[p], P.fy[p], P.fz[p], w);
   R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
   if (stride > 1) r *= (R)stride;
+  if (stride < -1) r *= (R)(-stride) / (R)(-stride - 1);
   R q[3] = {0, 0, 0};
   if (mode == 1) { q[0] = P.vx[p]; q[1] = P.vy[p]; q[2] = P.vz[p]; }
   else if (mode == 2) {
@@ -175,7 +179,8 @@ int pl_weights(swap_ctx* c) {
 }
 int pl_deposit_simple(swap_ctx* c, int mode, int stride) {
   if (c->n == 0) return 0;
-  if (stride > 1 && !c->id) return set_err(c, SWAP_ESTATE, "subsampled deposit needs ids");
+  if ((stride > 1 || stride < -1) && !c->id)
+    return set_err(c, SWAP_ESTATE, "subsampled deposit needs ids");
   PDISPATCH(c, PL((k_deposit_simple<R>), c->n, c->g, c->n, pdev<R>(c), mode,
                   (typename AccOf<R>::T*)c->macc, c->id, stride));
   return post(c);

@@ -168,6 +168,16 @@ int subcycle_impl(swap_ctx* c, double dtsub) {
   // f_update_tlev (gutsf.f:10-35)
   return tlev_impl(c);
 }
+// the same subcycle as two LDS-tiled launches; the corrector writes into
+// b12's buffer, so f_update_tlev is the rotation b12 <- b1 <- corrected
+int subcycle_fused(swap_ctx* c, double dtsub, bool last) {
+  TRY(fl_subcycle_fused(c, dtsub, last));
+  double* corrected = c->b12;
+  c->b12 = c->b1;
+  c->b1 = corrected;
+  c->b1p2_alias = true;
+  return 0;
+}
 
 }  // namespace
 
@@ -760,7 +770,9 @@ int swap_face_to_center(swap_ctx* c, int in, int out) {
 
 int swap_field_subcycles(swap_ctx* c, int ntf, double dtsub) {
   if (!c || ntf < 0 || !(dtsub > 0)) return SWAP_EINVAL;
-  for (int q = 0; q < ntf; q++) TRY(subcycle_impl(c, dtsub));
+  const bool fused = !(c->cfg.flags & SWAP_UNFUSED_FIELDS);
+  for (int q = 0; q < ntf; q++)
+    TRY(fused ? subcycle_fused(c, dtsub, q == ntf - 1) : subcycle_impl(c, dtsub));
   return 0;
 }
 

@@ -129,6 +129,7 @@ int fl_finalize_np(swap_ctx* c);
 int fl_finalize_up(swap_ctx* c);
 int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub);
 int fl_copy_b(swap_ctx* c, const double* src, double* dst);
+int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last);
 int fl_zero(swap_ctx* c, void* p, size_t bytes);
 
 // ---- particle launchers (k_particles.hip) ----

@@ -48,6 +48,38 @@ def test_forward_and_gradients_match_cpu():
         o += nw + D.CH[l + 1]
 
 
+def test_disjoint_target_and_pair_transfer():
+    """the disjoint target deposits exactly the particles the input leaves
+    out: S * np(sub) + (S - 1) * np(rest) = S * np(all) at every node.  The
+    edge velocity up averages two nodal ratios (momentum / np), which no
+    such identity combines; its mean over the grid is the drift either way.
+    Pairs copied out and back in are unchanged."""
+    nc, S = (8, 6, 5), 4
+    p, s = _ctx(nc, ppc=16, capacity=8 * 6 * 5 * 16)
+    s.load_maxwellian(p["npart"], p["vsw"], p["vth"])
+    s.dn_create(p["n0"], p["vsw"], max_pairs=4, stride=S)
+    s.prime()
+    s.step(1)
+    s.dn_add_pair()               # target: all particles
+    s.dn_add_pair(disjoint=True)  # target: id % S != 0
+    x0, t_all = s.dn_get_pair(0)
+    x1, t_rest = s.dn_get_pair(1)
+    assert np.array_equal(x0, x1)
+    n_sub, n_rest, n_all = x0[0] + 1.0, t_rest[0] + 1.0, t_all[0] + 1.0  # np / n0
+    assert np.allclose(n_sub + (S - 1) * n_rest, S * n_all, rtol=2e-5, atol=2e-5 * S)
+    # the drift (up_x = vsw, normalised 1) with thermal noise 30/400/sqrt(N)
+    assert abs(t_rest[1].mean() - t_all[1].mean()) < 5e-3
+    assert abs(t_rest[1].mean() - 1.0) < 2e-2
+    # a disjoint target differs from the full one: the subsample is not in it
+    assert not np.allclose(t_all, t_rest)
+    s.dn_put_pair(x0, t_rest)
+    x2, t2 = s.dn_get_pair(2)
+    assert np.array_equal(x2, x0) and np.array_equal(t2, t_rest)
+    raw_a, _ = s.dn_eval(1, 2)
+    raw_b, _ = s.dn_eval(2, 3)
+    assert raw_a == raw_b
+
+
 def test_training_beats_plain_low_ppc_and_gate():
     """pairs from the code's own particles: a 4-ppc subsample (stride 16) of
     a 64-ppc plasma with a pickup cloud; train on 16 pairs, judge on 8"""

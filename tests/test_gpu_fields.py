@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 from helpers import Pair, rel
+from swap_amd import api, physics
 
 pytestmark = pytest.mark.gpu
 
@@ -137,3 +138,46 @@ def test_check_time_step_halving():
     d_o, n_o = o.dtsub()
     assert ntf == int(n_o) and ntf > 10
     assert dtsub == pytest.approx(d_o, rel=1e-6)
+
+
+def _periodic_np(a):
+    """ghost layers of a storage array [..., nz, ny, nx] (boundary.f:9-52)"""
+    for ax in (-1, -2, -3):
+        n = a.shape[ax]
+        d = [slice(None)] * a.ndim
+        s = [slice(None)] * a.ndim
+        d[ax], s[ax] = 0, n - 2
+        a[tuple(d)] = a[tuple(s)]
+        d[ax], s[ax] = n - 1, 1
+        a[tuple(d)] = a[tuple(s)]
+    return a
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("nc", [(16, 8, 8), (20, 6, 37), (2, 3, 70), (40, 9, 33)])
+def test_fused_subcycles_equal_per_routine_kernels(precision, nc):
+    """the LDS-tiled predictor/corrector pair (two launches per subcycle) gives
+    bit for bit what the nine per-routine kernels give: B levels, and the E
+    and aj the last corrector leaves.  Shapes off the 32x4 tile and the z
+    chunking, and a 2-cell axis (every halo wraps more than once)."""
+    p = physics.workload(*nc, 2)
+    out = []
+    for flags in (0, api.UNFUSED_FIELDS):
+        s = physics.make_swap(p, precision=precision, flags=flags, capacity=16)
+        rng = np.random.default_rng(5)
+        shp = (s.nz, s.ny, s.nx)
+        b = rng.standard_normal((3,) + shp) * 0.1 * p["b0z"]
+        b[2] += p["b0z"]
+        b12 = b + rng.standard_normal((3,) + shp) * 0.01 * p["b0z"]
+        npf = p["n0"] * (1 + 0.3 * rng.uniform(-1, 1, shp))
+        up = rng.standard_normal((3,) + shp) * 30
+        up[0] += 400
+        for nm, a in (("b1", b), ("b12", b12), ("np", npf), ("up", up)):
+            s.set_field(nm, _periodic_np(a))
+        s.field_subcycles(3, p["dt"] / 10)
+        out.append({f: s.get_field(f) for f in ("b1", "b12", "b1p2", "E", "aj")})
+        s.close()
+    for f in out[0]:
+        assert np.array_equal(out[0][f], out[1][f]), f
+    # and the subcycles did move B
+    assert not np.array_equal(out[0]["b1"], out[0]["b12"])
