@@ -424,6 +424,20 @@ struct FT {
 // 3-slot ring index of plane z (z may be negative)
 __device__ __forceinline__ int r3(int z) { return ((z % 3) + 3) % 3; }
 
+// the positions a thread owns in one phase: t = tid + k NT over an N-position
+// region W wide whose first position sits at LDS (ox, oy).  p[k] = LDS index
+// (-1: none), q[k] = offset of the canonical node within its z plane.
+template <int K, int NT, int PW>
+__device__ __forceinline__ void own(int tid, int W, int N, int ox, int oy, const int* tcx,
+                                    const int* tcy, int nx, int p[K], int q[K]) {
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int t = tid + k * NT;
+    p[k] = t < N ? (t % W + ox) + PW * (t / W + oy) : -1;
+    q[k] = t < N ? tcx[p[k] % PW] + nx * tcy[p[k] / PW] : 0;
+  }
+}
+
 template <typename R, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const double* __restrict__ b1,
                                                         const double* __restrict__ b12,
@@ -448,25 +462,76 @@ __global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const doubl
   __syncthreads();
   const int64_t nxy = (int64_t)g.nx * g.ny, nn = g.nn;
   const R al = (R)f.alpha;
-  for (int s = Z0 - 4; s < Z1; s++) {
-    {  // b1 plane s+2, np plane s+3
-      const int64_t pb = wrapc(s + 2, g.nz) * nxy, pn = wrapc(s + 3, g.nz) * nxy;
-      const int rb = (s + 2) & 1, rn = (s + 3) & 1;
-      for (int p = tid; p < PA; p += NT) {
-        const int64_t q = tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
-        sb[rb][0][p] = b1[pb + q];
-        sb[rb][1][p] = b1[pb + q + nn];
-        sb[rb][2][p] = b1[pb + q + 2 * nn];
-        sn[rn][p] = np[pn + q];
+  // positions per phase: the whole plane (staging), curl [-1, T+1], E [0, T]
+  constexpr int AW = TX + 3, AN = AW * (TY + 3), EW = TX + 1, EN = EW * (TY + 1);
+  constexpr int KL = (PA + NT - 1) / NT, KA = (AN + NT - 1) / NT, KE = (EN + NT - 1) / NT;
+  int lp[KL], lq[KL], ap[KA], aq[KA], ep[KE], eq[KE];
+  own<KL, NT, PW>(tid, PW, PA, 0, 0, tcx, tcy, g.nx, lp, lq);
+  own<KA, NT, PW>(tid, AW, AN, 1, 1, tcx, tcy, g.nx, ap, aq);
+  own<KE, NT, PW>(tid, EW, EN, 2, 2, tcx, tcy, g.nx, ep, eq);
+  const int olx = tid % TX, oly = tid / TX, ox = X0 + olx, oy = Y0 + oly;
+  const bool oown = ox <= g.ncx && oy <= g.ncy;
+  // the global data of step s, fetched one step ahead so that its latency
+  // hides behind the previous step's phases: b1 plane s+2 and np plane s+3
+  // (staged in LDS), up plane s+2 at the curl positions, nu plane s+1 at the
+  // E positions, b12 at the output node of plane s
+  double fb[KL][3] = {}, fbase[3] = {};
+  R fn[KL] = {}, fu[KA][3] = {}, fnu[KE] = {};
+  auto fetch = [&](int s) {
+    const int64_t pb = wrapc(s + 2, g.nz) * nxy, pn = wrapc(s + 3, g.nz) * nxy,
+                  pe = wrapc(s + 1, g.nz) * nxy;
+#pragma unroll
+    for (int k = 0; k < KL; k++)
+      if (lp[k] >= 0) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) fb[k][m] = b1[pb + lq[k] + m * nn];
+        fn[k] = np[pn + lq[k]];
       }
+#pragma unroll
+    for (int k = 0; k < KA; k++)
+      if (ap[k] >= 0)
+#pragma unroll
+        for (int m = 0; m < 3; m++) fu[k][m] = up[pb + aq[k] + m * nn];
+#pragma unroll
+    for (int k = 0; k < KE; k++)
+      if (ep[k] >= 0) fnu[k] = nu[pe + eq[k]];
+    if (oown && s >= Z0) {
+      const int64_t oc = gidx(g, ox, oy, s);
+#pragma unroll
+      for (int m = 0; m < 3; m++) fbase[m] = b12[oc + m * nn];
+    }
+  };
+  fetch(Z0 - 4);
+  for (int s = Z0 - 4; s < Z1; s++) {
+    R cu[KA][3], cnu[KE];
+    double cbase[3];
+    {  // stage b1 plane s+2 and np plane s+3, keep this step's up, nu, b12,
+       // start the next step's loads
+      const int rb = (s + 2) & 1, rn = (s + 3) & 1;
+#pragma unroll
+      for (int k = 0; k < KL; k++)
+        if (lp[k] >= 0) {
+#pragma unroll
+          for (int m = 0; m < 3; m++) sb[rb][m][lp[k]] = fb[k][m];
+          sn[rn][lp[k]] = fn[k];
+        }
+#pragma unroll
+      for (int k = 0; k < KA; k++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) cu[k][m] = fu[k][m];
+#pragma unroll
+      for (int k = 0; k < KE; k++) cnu[k] = fnu[k];
+#pragma unroll
+      for (int m = 0; m < 3; m++) cbase[m] = fbase[m];
+      if (s + 1 < Z1) fetch(s + 1);
     }
     __syncthreads();
     if (s >= Z0 - 3) {  // aj, a at plane s+2 over x, y in [-1, T+1]
       const int r = (s + 2) & 1, rm = r ^ 1;
-      const int64_t pz = wrapc(s + 2, g.nz) * nxy;
-      constexpr int W = TX + 3;
-      for (int t = tid; t < W * (TY + 3); t += NT) {
-        const int p = (t % W + 1) + PW * (t / W + 1);
+#pragma unroll
+      for (int k = 0; k < KA; k++) {
+        const int p = ap[k];
+        if (p < 0) continue;
         const R n0 = sn[r][p];
         const R nt1 = (R)0.5 * (n0 + sn[r][p + 1]);
         const R nt2 = (R)0.5 * (n0 + sn[r][p + PW]);
@@ -478,13 +543,12 @@ __global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const doubl
         const double c3 = (sb[r][1][p] - sb[r][1][p - 1]) / f.dx +
                           (sb[r][0][p - PW] - sb[r][0][p]) / f.dy;
         const R j1 = (R)c1 / (nt1 * al), j2 = (R)c2 / (nt2 * al), j3 = (R)c3 / (nt3 * al);
-        const int64_t oc = pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
         sj[r][0][p] = j1;
         sj[r][1][p] = j2;
         sj[r][2][p] = j3;
-        sa[r][0][p] = j1 - up[oc];
-        sa[r][1][p] = j2 - up[oc + nn];
-        sa[r][2][p] = j3 - up[oc + 2 * nn];
+        sa[r][0][p] = j1 - cu[k][0];
+        sa[r][1][p] = j2 - cu[k][1];
+        sa[r][2][p] = j3 - cu[k][2];
       }
     }
     __syncthreads();
@@ -515,32 +579,28 @@ __global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const doubl
     __syncthreads();
     if (s >= Z0 - 1) {  // E at plane s+1 over x, y in [0, T]
       const int r = (s + 1) & 1, rp = r ^ 1;
-      const int64_t pz = wrapc(s + 1, g.nz) * nxy;
-      constexpr int W = TX + 1;
-      for (int t = tid; t < W * (TY + 1); t += NT) {
-        const int p = (t % W + 2) + PW * (t / W + 2);
-        const R nuv = nu[pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW]];
+#pragma unroll
+      for (int k = 0; k < KE; k++) {
+        const int p = ep[k];
+        if (p < 0) continue;
+        const R nuv = cnu[k];
         se[r][0][p] = (R)0.5 * (sc[r][0][p] + sc[r][0][p + 1]) + nuv * sj[r][0][p];
         se[r][1][p] = (R)0.5 * (sc[r][1][p] + sc[r][1][p + PW]) + nuv * sj[r][1][p];
         se[r][2][p] = (R)0.5 * (sc[r][2][p] + sc[rp][2][p]) + nuv * sj[r][2][p];
       }
     }
     __syncthreads();
-    if (s >= Z0) {  // b1p2 = b12 - coef curlE(E) at plane s
+    if (s >= Z0 && oown) {  // b1p2 = b12 - coef curlE(E) at plane s
       const int r = s & 1, rp = r ^ 1;
-      const int lx = tid % TX, ly = tid / TX, x = X0 + lx, y = Y0 + ly;
-      if (x <= g.ncx && y <= g.ncy) {
-        const int p = (lx + 2) + PW * (ly + 2);
-        const double c0 = ((double)se[r][2][p + PW] - (double)se[r][2][p]) / f.dy +
-                          ((double)se[r][1][p] - (double)se[rp][1][p]) / f.dz;
-        const double c1 = ((double)se[r][2][p] - (double)se[r][2][p + 1]) / f.dx +
-                          ((double)se[rp][0][p] - (double)se[r][0][p]) / f.dz;
-        const double c2 = ((double)se[r][0][p] - (double)se[r][0][p + PW]) / f.dy +
-                          ((double)se[r][1][p + 1] - (double)se[r][1][p]) / f.dx;
-        const int64_t oc = gidx(g, x, y, s);
-        put3<double>(g, out, x, y, s, b12[oc] - coef * c0, b12[oc + nn] - coef * c1,
-                     b12[oc + 2 * nn] - coef * c2);
-      }
+      const int p = (olx + 2) + PW * (oly + 2);
+      const double c0 = ((double)se[r][2][p + PW] - (double)se[r][2][p]) / f.dy +
+                        ((double)se[r][1][p] - (double)se[rp][1][p]) / f.dz;
+      const double c1 = ((double)se[r][2][p] - (double)se[r][2][p + 1]) / f.dx +
+                        ((double)se[rp][0][p] - (double)se[r][0][p]) / f.dz;
+      const double c2 = ((double)se[r][0][p] - (double)se[r][0][p + PW]) / f.dy +
+                        ((double)se[r][1][p + 1] - (double)se[r][1][p]) / f.dx;
+      put3<double>(g, out, ox, oy, s, cbase[0] - coef * c0, cbase[1] - coef * c1,
+                   cbase[2] - coef * c2);
     }
   }
 }
@@ -572,24 +632,77 @@ __global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const doubl
   __syncthreads();
   const int64_t nxy = (int64_t)g.nx * g.ny, nn = g.nn;
   const R al = (R)f.alpha;
-  for (int s = Z0 - 5; s < Z1; s++) {
-    {  // bh plane s+3, np plane s+4
-      const int64_t pb = wrapc(s + 3, g.nz) * nxy, pn = wrapc(s + 4, g.nz) * nxy;
-      const int rb = (s + 3) & 1, rn = (s + 4) & 1;
-      for (int p = tid; p < PA; p += NT) {
-        const int64_t q = tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
+  constexpr int AW = TX + 3, AN = AW * (TY + 3), EW = TX + 1, EN = EW * (TY + 1);
+  constexpr int KL = (PA + NT - 1) / NT, KA = (AN + NT - 1) / NT, KE = (EN + NT - 1) / NT;
+  int lp[KL], lq[KL], ap[KA], aq[KA], ep[KE], eq[KE];
+  own<KL, NT, PW>(tid, PW, PA, 0, 0, tcx, tcy, g.nx, lp, lq);
+  own<KA, NT, PW>(tid, AW, AN, 1, 1, tcx, tcy, g.nx, ap, aq);
+  own<KE, NT, PW>(tid, EW, EN, 2, 2, tcx, tcy, g.nx, ep, eq);
+  const int olx = tid % TX, oly = tid / TX, ox = X0 + olx, oy = Y0 + oly;
+  const bool oown = ox <= g.ncx && oy <= g.ncy;
+  // step s's global data, fetched one step ahead: b1p2 and b1 plane s+3 and
+  // np plane s+4 (staged), up plane s+3 at the curl positions, nu plane s+1
+  // at the E positions, b1 at the output node of plane s
+  double fp[KL][3] = {}, fq[KL][3] = {}, fbase[3] = {};
+  R fn[KL] = {}, fu[KA][3] = {}, fnu[KE] = {};
+  auto fetch = [&](int s) {
+    const int64_t pb = wrapc(s + 3, g.nz) * nxy, pn = wrapc(s + 4, g.nz) * nxy,
+                  pe = wrapc(s + 1, g.nz) * nxy;
 #pragma unroll
-        for (int m = 0; m < 3; m++) sb[rb][m][p] = 0.5 * (bp[pb + q + m * nn] + b1[pb + q + m * nn]);
-        sn[rn][p] = np[pn + q];
+    for (int k = 0; k < KL; k++)
+      if (lp[k] >= 0) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          fp[k][m] = bp[pb + lq[k] + m * nn];
+          fq[k][m] = b1[pb + lq[k] + m * nn];
+        }
+        fn[k] = np[pn + lq[k]];
       }
+#pragma unroll
+    for (int k = 0; k < KA; k++)
+      if (ap[k] >= 0)
+#pragma unroll
+        for (int m = 0; m < 3; m++) fu[k][m] = up[pb + aq[k] + m * nn];
+#pragma unroll
+    for (int k = 0; k < KE; k++)
+      if (ep[k] >= 0) fnu[k] = nu[pe + eq[k]];
+    if (oown && s >= Z0) {
+      const int64_t oc = gidx(g, ox, oy, s);
+#pragma unroll
+      for (int m = 0; m < 3; m++) fbase[m] = b1[oc + m * nn];
+    }
+  };
+  fetch(Z0 - 5);
+  for (int s = Z0 - 5; s < Z1; s++) {
+    R cu[KA][3], cnu[KE];
+    double cbase[3];
+    {  // stage bh plane s+3 and np plane s+4, keep this step's up, nu, b1,
+       // start the next step's loads
+      const int rb = (s + 3) & 1, rn = (s + 4) & 1;
+#pragma unroll
+      for (int k = 0; k < KL; k++)
+        if (lp[k] >= 0) {
+#pragma unroll
+          for (int m = 0; m < 3; m++) sb[rb][m][lp[k]] = 0.5 * (fp[k][m] + fq[k][m]);
+          sn[rn][lp[k]] = fn[k];
+        }
+#pragma unroll
+      for (int k = 0; k < KA; k++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) cu[k][m] = fu[k][m];
+#pragma unroll
+      for (int k = 0; k < KE; k++) cnu[k] = fnu[k];
+#pragma unroll
+      for (int m = 0; m < 3; m++) cbase[m] = fbase[m];
+      if (s + 1 < Z1) fetch(s + 1);
     }
     __syncthreads();
     if (s >= Z0 - 4) {  // aj, a over [-1, T+1] and btc0 over [-1, T+2], plane s+3
       const int r = (s + 3) & 1, rm = r ^ 1, w = r3(s + 3);
-      const int64_t pz = wrapc(s + 3, g.nz) * nxy;
-      constexpr int W = TX + 3;
-      for (int t = tid; t < W * (TY + 3); t += NT) {
-        const int p = (t % W + 1) + PW * (t / W + 1);
+#pragma unroll
+      for (int k = 0; k < KA; k++) {
+        const int p = ap[k];
+        if (p < 0) continue;
         const R n0 = sn[r][p];
         const R nt1 = (R)0.5 * (n0 + sn[r][p + 1]);
         const R nt2 = (R)0.5 * (n0 + sn[r][p + PW]);
@@ -601,13 +714,12 @@ __global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const doubl
         const double c3 = (sb[r][1][p] - sb[r][1][p - 1]) / f.dx +
                           (sb[r][0][p - PW] - sb[r][0][p]) / f.dy;
         const R j1 = (R)c1 / (nt1 * al), j2 = (R)c2 / (nt2 * al), j3 = (R)c3 / (nt3 * al);
-        const int64_t oc = pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW];
         sj[w][0][p] = j1;
         sj[w][1][p] = j2;
         sj[w][2][p] = j3;
-        sa[w][0][p] = j1 - up[oc];
-        sa[w][1][p] = j2 - up[oc + nn];
-        sa[w][2][p] = j3 - up[oc + 2 * nn];
+        sa[w][0][p] = j1 - cu[k][0];
+        sa[w][1][p] = j2 - cu[k][1];
+        sa[w][2][p] = j3 - cu[k][2];
       }
       constexpr int W2 = TX + 4;
       for (int t = tid; t < W2 * (TY + 4); t += NT) {
@@ -651,13 +763,13 @@ __global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const doubl
     if (s >= Z0 - 1) {  // E at plane s+1 over [0, T]
       const int r = (s + 1) & 1, rp = r ^ 1, w = r3(s + 1);
       const int ze = wrapc(s + 1, g.nz);
-      const int64_t pz = ze * nxy;
       const bool store = Eout && s + 1 >= Z0 && s + 1 < Z1;
-      constexpr int W = TX + 1;
-      for (int t = tid; t < W * (TY + 1); t += NT) {
-        const int lx = t % W, ly = t / W;
-        const int p = (lx + 2) + PW * (ly + 2);
-        const R nuv = nu[pz + tcx[p % PW] + (int64_t)g.nx * tcy[p / PW]];
+#pragma unroll
+      for (int k = 0; k < KE; k++) {
+        const int p = ep[k];
+        if (p < 0) continue;
+        const int lx = p % PW - 2, ly = p / PW - 2;
+        const R nuv = cnu[k];
         const R e0 = (R)0.5 * (sc[r][0][p] + sc[r][0][p + 1]) + nuv * sj[w][0][p];
         const R e1 = (R)0.5 * (sc[r][1][p] + sc[r][1][p + PW]) + nuv * sj[w][1][p];
         const R e2 = (R)0.5 * (sc[r][2][p] + sc[rp][2][p]) + nuv * sj[w][2][p];
@@ -671,21 +783,17 @@ __global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const doubl
       }
     }
     __syncthreads();
-    if (s >= Z0) {  // b = b1 - coef curlE(E) at plane s
+    if (s >= Z0 && oown) {  // b = b1 - coef curlE(E) at plane s
       const int r = s & 1, rp = r ^ 1;
-      const int lx = tid % TX, ly = tid / TX, x = X0 + lx, y = Y0 + ly;
-      if (x <= g.ncx && y <= g.ncy) {
-        const int p = (lx + 2) + PW * (ly + 2);
-        const double c0 = ((double)se[r][2][p + PW] - (double)se[r][2][p]) / f.dy +
-                          ((double)se[r][1][p] - (double)se[rp][1][p]) / f.dz;
-        const double c1 = ((double)se[r][2][p] - (double)se[r][2][p + 1]) / f.dx +
-                          ((double)se[rp][0][p] - (double)se[r][0][p]) / f.dz;
-        const double c2 = ((double)se[r][0][p] - (double)se[r][0][p + PW]) / f.dy +
-                          ((double)se[r][1][p + 1] - (double)se[r][1][p]) / f.dx;
-        const int64_t oc = gidx(g, x, y, s);
-        put3<double>(g, out, x, y, s, b1[oc] - coef * c0, b1[oc + nn] - coef * c1,
-                     b1[oc + 2 * nn] - coef * c2);
-      }
+      const int p = (olx + 2) + PW * (oly + 2);
+      const double c0 = ((double)se[r][2][p + PW] - (double)se[r][2][p]) / f.dy +
+                        ((double)se[r][1][p] - (double)se[rp][1][p]) / f.dz;
+      const double c1 = ((double)se[r][2][p] - (double)se[r][2][p + 1]) / f.dx +
+                        ((double)se[rp][0][p] - (double)se[r][0][p]) / f.dz;
+      const double c2 = ((double)se[r][0][p] - (double)se[r][0][p + PW]) / f.dy +
+                        ((double)se[r][1][p + 1] - (double)se[r][1][p]) / f.dx;
+      put3<double>(g, out, ox, oy, s, cbase[0] - coef * c0, cbase[1] - coef * c1,
+                   cbase[2] - coef * c2);
     }
   }
 }
@@ -847,13 +955,25 @@ int fl_copy_b(swap_ctx* c, const double* src, double* dst) {
 }
 // one fused subcycle: b1p2 = predictor(b1, b12); b12's buffer = corrector
 // (b1p2, b1); E and aj of the corrector are stored when last is set
+// tile of canonical columns per workgroup and z planes per chunk (FP32:
+// 32x8 measured best among 16..64 x 4..12 and 16..64 planes at 256x128x128
+// cells); the FP64 build's LDS rings take twice the bytes per position
+template <typename R>
+struct FTile {
+  static constexpr int TX = 32, TY = 8, TZ = 32;
+};
+template <>
+struct FTile<double> {
+  static constexpr int TX = 32, TY = 4, TZ = 32;
+};
+
 int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last) {
-  constexpr int TX = 32, TY = 4, TZ = 32;
   FC f = fc_of(c);
   const Geo& g = c->g;
-  const int nzc = (g.ncz + TZ - 1) / TZ, tz = (g.ncz + nzc - 1) / nzc;
-  const dim3 grid((g.ncx + TX - 1) / TX, (g.ncy + TY - 1) / TY, nzc), block(TX * TY);
-  DISPATCH(c, hipLaunchKernelGGL((k_sub_predict<R, TX, TY>), grid, block, 0, c->stream, g, f,
+  DISPATCH(c, constexpr int TX = FTile<R>::TX, TY = FTile<R>::TY, TZ = FTile<R>::TZ;
+           const int nzc = (g.ncz + TZ - 1) / TZ, tz = (g.ncz + nzc - 1) / nzc;
+           const dim3 grid((g.ncx + TX - 1) / TX, (g.ncy + TY - 1) / TY, nzc), block(TX * TY);
+           hipLaunchKernelGGL((k_sub_predict<R, TX, TY>), grid, block, 0, c->stream, g, f,
                                  (const double*)c->b1, (const double*)c->b12,
                                  (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
                                  (const R*)F<R>(c, SWAP_F_NU), 2.0 * dtsub, tz, c->b1p2);
