@@ -6,10 +6,10 @@
 // conv4(...)).  Activations are channels-last BF16 [voxel][channel] over the
 // interior nodes (i fastest).  Every convolution is an implicit GEMM on MFMA
 // (v_mfma_f32_16x16x16_bf16, FP32 accumulation): M = voxels, N = output
-// channels, K = 27 taps x input channels.  A wavefront owns 16 voxels and all
-// output channels; lane l supplies voxel l%16 and the four K entries
-// 4(l/16)..4(l/16)+3 of the A operand, output channel l%16 of the B operand,
-// and receives outputs (voxel 4(l/16)+e, channel l%16).
+// channels, K = 27 taps x input channels.  An M-tile is 16 voxels along x;
+// lane l supplies voxel l%16 and the four K entries 4(l/16)..4(l/16)+3 of the
+// A operand, output channel l%16 of the B operand, and receives outputs
+// (voxel 4(l/16)+e, channel l%16).
 //
 // Backward: the loss gradient is propagated with the same kernel through the
 // tap-flipped, transposed weights (the adjoint of a periodic convolution),
@@ -55,55 +55,95 @@ __device__ __forceinline__ int64_t nbr(const G3& g, int x, int y, int z, int t) 
   return (int64_t)a + (int64_t)g.nx * ((int64_t)b + (int64_t)g.ny * c);
 }
 
-// Y = epilogue(conv(X, W)); Wt is [r16(COUT)][k16(CIN)], zero padded
+// Y = epilogue(conv(X, W)); Wt is [r16(COUT)][k16(CIN)], zero padded.
+// A workgroup computes a CTX x CTY x CTZ block of output voxels.  Its input
+// halo (one voxel on each side, periodic, all channels) is staged in LDS
+// once; wavefront w runs the four 16-voxel x-runs at y = y0 + w (one per z
+// plane) against all output channels, k ascending (tap-major, then channel).
+constexpr int CTX = 16, CTY = 4, CTZ = 4;
+constexpr int CHX = CTX + 2, CHY = CTY + 2, CHZ = CTZ + 2, CHV = CHX * CHY * CHZ;
+
+unsigned ceil_div(int a, int b) { return (unsigned)((a + b - 1) / b); }
+
+__device__ __forceinline__ int wrapi(int s, int n) {
+  int r = s % n;
+  return r < 0 ? r + n : r;
+}
+
 template <int CIN, int COUT, int EPI>
-__global__ __launch_bounds__(256) void k_conv(G3 g, const bf16* __restrict__ X,
-                                              const bf16* __restrict__ Wt,
-                                              const float* __restrict__ bias,
-                                              const bf16* __restrict__ mask,
-                                              const float* __restrict__ resid,
-                                              bf16* __restrict__ Yb, float* __restrict__ Yf) {
-  constexpr int K = 27 * CIN, KP = k16(CIN), NT = r16(COUT) / 16;
-  const int lane = threadIdx.x & 63;
-  const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-  if (v0 >= g.V) return;
-  const int m = lane & 15, kq = (lane >> 4) * 4;
-  const int64_t vm = v0 + m;
-  const bool vok = vm < g.V;
-  const int x = (int)(vm % g.nx), y = (int)((vm / g.nx) % g.ny), z = (int)(vm / ((int64_t)g.nx * g.ny));
-  f4 acc[NT];
+__global__ __launch_bounds__(256) void k_conv_t(G3 g, const bf16* __restrict__ X,
+                                                const bf16* __restrict__ Wt,
+                                                const float* __restrict__ bias,
+                                                const bf16* __restrict__ mask,
+                                                const float* __restrict__ resid,
+                                                bf16* __restrict__ Yb, float* __restrict__ Yf) {
+  static_assert(CIN == 4 || CIN == 32, "input channels");
+  // LDS stride per voxel in BF16: 32 channels padded to 36 (72 B), so the 16
+  // voxels one lane group reads fall in distinct banks
+  constexpr int VS = CIN == 32 ? 36 : CIN;
+  constexpr int K = 27 * CIN, KP = k16(CIN), NT = r16(COUT) / 16, CPV = CIN / 4;
+  __shared__ __align__(16) bf16 sx[CHV * VS];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int x0 = blockIdx.x * CTX, y0 = blockIdx.y * CTY, z0 = blockIdx.z * CTZ;
+  for (int q = tid; q < CHV * CPV; q += 256) {
+    const int hv = q / CPV, ch = q % CPV;
+    const int hx = hv % CHX, hy = (hv / CHX) % CHY, hz = hv / (CHX * CHY);
+    const int64_t gv = wrapi(x0 - 1 + hx, g.nx) +
+                       (int64_t)g.nx * (wrapi(y0 - 1 + hy, g.ny) +
+                                        (int64_t)g.ny * wrapi(z0 - 1 + hz, g.nz));
+    *(s4*)(sx + hv * VS + 4 * ch) = *(const s4*)(X + gv * CIN + 4 * ch);
+  }
+  __syncthreads();
+  const int m = lane & 15, gq = lane >> 4;
+  f4 acc[CTZ][NT];
 #pragma unroll
-  for (int nt = 0; nt < NT; nt++) acc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int mt = 0; mt < CTZ; mt++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
   for (int kk = 0; kk < KP; kk += 16) {
-    const int k0 = kk + kq;
-    s4 a = s4{0, 0, 0, 0};
-    if (vok && k0 < K) {
-      const int t = k0 / CIN, ci = k0 % CIN;
-      a = *(const s4*)(X + nbr(g, x, y, z, t) * CIN + ci);
+    const int k0 = kk + 4 * gq;
+    const bool kok = k0 < K;  // the zero-padded tail of K (taps past 26)
+    const int t = !kok ? 0 : (CIN == 32 ? kk / 32 : k0 / 4);
+    const int ci = CIN == 32 ? kk % 32 + 4 * gq : 0;
+    const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
+    s4 a[CTZ];
+#pragma unroll
+    for (int mt = 0; mt < CTZ; mt++) {
+      const int hv = (m + dx) + CHX * ((w + dy) + CHY * (mt + dz));
+      a[mt] = kok ? *(const s4*)(sx + hv * VS + ci) : s4{0, 0, 0, 0};
     }
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) {
       const s4 b = *(const s4*)(Wt + (int64_t)(nt * 16 + m) * KP + k0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+#pragma unroll
+      for (int mt = 0; mt < CTZ; mt++)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mt], b, acc[mt][nt], 0, 0, 0);
     }
   }
+  const int y = y0 + w;
+  if (y >= g.ny) return;
 #pragma unroll
-  for (int nt = 0; nt < NT; nt++) {
-    const int co = nt * 16 + m;
-    if (co >= COUT) continue;
+  for (int mt = 0; mt < CTZ; mt++) {
+    const int z = z0 + mt;
+    if (z >= g.nz) continue;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int64_t v = v0 + kq + e;
-      if (v >= g.V) continue;
-      float val = acc[nt][e];
-      const int64_t o = v * COUT + co;
-      if constexpr (EPI == EPI_RELU) {
-        val += bias[co];
-        Yb[o] = f2bf(val > 0.f ? val : 0.f);
-      } else if constexpr (EPI == EPI_RESID) {
-        Yf[o] = val + bias[co] + resid[o];
-      } else {
-        Yb[o] = f2bf(bf2f(mask[o]) > 0.f ? val : 0.f);
+    for (int nt = 0; nt < NT; nt++) {
+      const int co = nt * 16 + m;
+      if (co >= COUT) continue;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int x = x0 + 4 * gq + e;
+        if (x >= g.nx) continue;
+        const int64_t o = (x + (int64_t)g.nx * (y + (int64_t)g.ny * z)) * COUT + co;
+        float val = acc[mt][nt][e];
+        if constexpr (EPI == EPI_RELU) {
+          val += bias[co];
+          Yb[o] = f2bf(val > 0.f ? val : 0.f);
+        } else if constexpr (EPI == EPI_RESID) {
+          Yf[o] = val + bias[co] + resid[o];
+        } else {
+          Yb[o] = f2bf(bf2f(mask[o]) > 0.f ? val : 0.f);
+        }
       }
     }
   }
@@ -322,18 +362,18 @@ int dn_forward(swap_ctx* c) {
   DnState* d = c->dn;
   const G3 g = d->g3;
   const int64_t V = g.V;
-  dim3 grid((unsigned)((V + 63) / 64)), blk(256);
+  const dim3 grid(ceil_div(g.nx, CTX), ceil_div(g.ny, CTY), ceil_div(g.nz, CTZ)), blk(256);
   hipLaunchKernelGGL(k_f2bf, dim3(nblk(4 * V, 256)), dim3(256), 0, c->stream, 4 * V, d->xf,
                      d->act[0]);
   const float* b[NL];
   for (int l = 0; l < NL; l++) b[l] = d->params + param_off(l) + wcount(l);
-  hipLaunchKernelGGL((k_conv<4, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[0], d->Wt[0],
+  hipLaunchKernelGGL((k_conv_t<4, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[0], d->Wt[0],
                      b[0], nullptr, nullptr, d->act[1], nullptr);
-  hipLaunchKernelGGL((k_conv<32, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[1],
+  hipLaunchKernelGGL((k_conv_t<32, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[1],
                      d->Wt[1], b[1], nullptr, nullptr, d->act[2], nullptr);
-  hipLaunchKernelGGL((k_conv<32, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[2],
+  hipLaunchKernelGGL((k_conv_t<32, 32, EPI_RELU>), grid, blk, 0, c->stream, g, d->act[2],
                      d->Wt[2], b[2], nullptr, nullptr, d->act[3], nullptr);
-  hipLaunchKernelGGL((k_conv<32, 4, EPI_RESID>), grid, blk, 0, c->stream, g, d->act[3],
+  hipLaunchKernelGGL((k_conv_t<32, 4, EPI_RESID>), grid, blk, 0, c->stream, g, d->act[3],
                      d->Wt[3], b[3], nullptr, d->xf, nullptr, d->out);
   return post(c);
 }
@@ -343,7 +383,7 @@ int dn_backward(swap_ctx* c, const float* tgt) {
   DnState* d = c->dn;
   const G3 g = d->g3;
   const int64_t V = g.V;
-  dim3 grid((unsigned)((V + 63) / 64)), blk(256);
+  const dim3 grid(ceil_div(g.nx, CTX), ceil_div(g.ny, CTY), ceil_div(g.nz, CTZ)), blk(256);
   HIPCHK(c, hipMemsetAsync(d->grads, 0, sizeof(float) * NPARAM, c->stream));
   HIPCHK(c, hipMemsetAsync(d->dloss, 0, sizeof(double), c->stream));
   hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, 4 * V, d->out, tgt, d->gr[NL],
@@ -360,14 +400,14 @@ int dn_backward(swap_ctx* c, const float* tgt) {
   hipLaunchKernelGGL((k_wgrad<32, 4>), dim3(nch, k16(32) / 16, 1), dim3(64), 0, c->stream, g,
                      d->act[3], d->gr[4], gw[3], chunk);
   hipLaunchKernelGGL(k_bgrad<4>, dim3(256), dim3(256), 0, c->stream, V, d->gr[4], gb[3]);
-  hipLaunchKernelGGL((k_conv<4, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[4], d->Wb[3],
+  hipLaunchKernelGGL((k_conv_t<4, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[4], d->Wb[3],
                      nullptr, d->act[3], nullptr, d->gr[3], nullptr);
   // layers 3 and 2 (32 -> 32)
   for (int l = 2; l >= 1; l--) {
     hipLaunchKernelGGL((k_wgrad<32, 32>), dim3(nch, k16(32) / 16, 2), dim3(64), 0, c->stream, g,
                        d->act[l], d->gr[l + 1], gw[l], chunk);
     hipLaunchKernelGGL(k_bgrad<32>, dim3(256), dim3(256), 0, c->stream, V, d->gr[l + 1], gb[l]);
-    hipLaunchKernelGGL((k_conv<32, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[l + 1],
+    hipLaunchKernelGGL((k_conv_t<32, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[l + 1],
                        d->Wb[l], nullptr, d->act[l], nullptr, d->gr[l], nullptr);
   }
   // layer 1 (4 -> 32); no gradient to the input

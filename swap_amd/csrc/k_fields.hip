@@ -308,22 +308,31 @@ __global__ void k_finalize_up(Geo g, const typename AccOf<R>::T* a, const R* np,
 }
 
 // check_time_step's per-node whistler bound (gutsf.f:1782-1792)
+// The grid minimum goes through one atomic per wavefront (a 64-lane min
+// first); non-negative floats order as their bit patterns.
 template <typename R>
 __global__ void k_deltat(Geo g, FC f, const double* b, const R* np, R* dtn, float* dmin) {
-  GRID_LOOP_HEAD
-  R b1 = (R)b[o], b2 = (R)b[o + g.nn], b3 = (R)b[o + 2 * g.nn];
-  R ak = (R)2. / (R)f.dx;
-  R btot = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
-  R al = (R)f.alpha;
-  R a1 = ak * ak * btot / (al * np[o]);
-  R a2 = (ak * btot) * (ak * btot) / (al * np[o]);
-  R womega = (R)0.5 * (a1 + sqrt(a1 * a1 + (R)4 * a2));
-  R phi = womega / ak;
-  R dt = (R)f.dx / phi;
-  dtn[o] = dt;
-  float d = (float)dt;
-  if (!(d >= 0.f)) d = 0.f;  // NaN or negative: treat as failing
-  atomicMin((unsigned int*)dmin, __float_as_uint(d));
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  unsigned int u = 0xFFFFFFFFu;
+  if (i < g.nx && j < g.ny) {
+    const int64_t o = gidx(g, i, j, blockIdx.z);
+    R b1 = (R)b[o], b2 = (R)b[o + g.nn], b3 = (R)b[o + 2 * g.nn];
+    R ak = (R)2. / (R)f.dx;
+    R btot = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+    R al = (R)f.alpha;
+    R a1 = ak * ak * btot / (al * np[o]);
+    R a2 = (ak * btot) * (ak * btot) / (al * np[o]);
+    R womega = (R)0.5 * (a1 + sqrt(a1 * a1 + (R)4 * a2));
+    R phi = womega / ak;
+    R dt = (R)f.dx / phi;
+    dtn[o] = dt;
+    float d = (float)dt;
+    if (!(d >= 0.f)) d = 0.f;  // NaN or negative: treat as failing
+    u = __float_as_uint(d);
+  }
+  for (int off = 32; off > 0; off >>= 1) u = min(u, (unsigned int)__shfl_xor((int)u, off));
+  if ((threadIdx.x & 63) == 0) atomicMin((unsigned int*)dmin, u);
 }
 
 // the reference halves dtsub once per failing node, in its loop order
@@ -438,14 +447,15 @@ __device__ __forceinline__ void own(int tid, int W, int N, int ox, int oy, const
   }
 }
 
-template <typename R, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const double* __restrict__ b1,
+template <typename R, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* __restrict__ b1,
                                                         const double* __restrict__ b12,
                                                         const R* __restrict__ np,
                                                         const R* __restrict__ up,
                                                         const R* __restrict__ nu, double coef,
                                                         int tz, double* __restrict__ out) {
-  constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA, NT = TX * TY;
+  constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
+  static_assert(NT >= TX * TY, "one thread per output column at least");
   __shared__ double sb[2][3][PA];  // b1, planes s+1, s+2
   __shared__ R sn[2][PA];          // np, planes s+2, s+3
   __shared__ R sa[2][3][PA];       // a = aj - up
@@ -470,7 +480,7 @@ __global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const doubl
   own<KA, NT, PW>(tid, AW, AN, 1, 1, tcx, tcy, g.nx, ap, aq);
   own<KE, NT, PW>(tid, EW, EN, 2, 2, tcx, tcy, g.nx, ep, eq);
   const int olx = tid % TX, oly = tid / TX, ox = X0 + olx, oy = Y0 + oly;
-  const bool oown = ox <= g.ncx && oy <= g.ncy;
+  const bool oown = tid < TX * TY && ox <= g.ncx && oy <= g.ncy;
   // the global data of step s, fetched one step ahead so that its latency
   // hides behind the previous step's phases: b1 plane s+2 and np plane s+3
   // (staged in LDS), up plane s+2 at the curl positions, nu plane s+1 at the
@@ -605,8 +615,8 @@ __global__ __launch_bounds__(TX* TY) void k_sub_predict(Geo g, FC f, const doubl
   }
 }
 
-template <typename R, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const double* __restrict__ bp,
+template <typename R, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* __restrict__ bp,
                                                         const double* __restrict__ b1,
                                                         const R* __restrict__ np,
                                                         const R* __restrict__ up,
@@ -614,7 +624,8 @@ __global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const doubl
                                                         int tz, double* __restrict__ out,
                                                         R* __restrict__ Eout,
                                                         R* __restrict__ ajout) {
-  constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA, NT = TX * TY;
+  constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
+  static_assert(NT >= TX * TY, "one thread per output column at least");
   __shared__ double sb[2][3][PA];  // bh = (b1p2 + b1)/2, planes s+2, s+3
   __shared__ R sn[2][PA];          // np, planes s+3, s+4
   __shared__ R sa[3][3][PA];       // a = aj - up, planes s+1 .. s+3
@@ -639,7 +650,7 @@ __global__ __launch_bounds__(TX* TY) void k_sub_correct(Geo g, FC f, const doubl
   own<KA, NT, PW>(tid, AW, AN, 1, 1, tcx, tcy, g.nx, ap, aq);
   own<KE, NT, PW>(tid, EW, EN, 2, 2, tcx, tcy, g.nx, ep, eq);
   const int olx = tid % TX, oly = tid / TX, ox = X0 + olx, oy = Y0 + oly;
-  const bool oown = ox <= g.ncx && oy <= g.ncy;
+  const bool oown = tid < TX * TY && ox <= g.ncx && oy <= g.ncy;
   // step s's global data, fetched one step ahead: b1p2 and b1 plane s+3 and
   // np plane s+4 (staged), up plane s+3 at the curl positions, nu plane s+1
   // at the E positions, b1 at the output node of plane s
@@ -960,24 +971,25 @@ int fl_copy_b(swap_ctx* c, const double* src, double* dst) {
 // cells); the FP64 build's LDS rings take twice the bytes per position
 template <typename R>
 struct FTile {
-  static constexpr int TX = 32, TY = 8, TZ = 32;
+  static constexpr int TX = 32, TY = 8, TZ = 32, NP = 256, NC = 256;
 };
 template <>
 struct FTile<double> {
-  static constexpr int TX = 32, TY = 4, TZ = 32;
+  static constexpr int TX = 32, TY = 4, TZ = 32, NP = 128, NC = 128;
 };
 
 int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last) {
   FC f = fc_of(c);
   const Geo& g = c->g;
   DISPATCH(c, constexpr int TX = FTile<R>::TX, TY = FTile<R>::TY, TZ = FTile<R>::TZ;
+           constexpr int NP = FTile<R>::NP, NC = FTile<R>::NC;
            const int nzc = (g.ncz + TZ - 1) / TZ, tz = (g.ncz + nzc - 1) / nzc;
-           const dim3 grid((g.ncx + TX - 1) / TX, (g.ncy + TY - 1) / TY, nzc), block(TX * TY);
-           hipLaunchKernelGGL((k_sub_predict<R, TX, TY>), grid, block, 0, c->stream, g, f,
+           const dim3 grid((g.ncx + TX - 1) / TX, (g.ncy + TY - 1) / TY, nzc);
+           hipLaunchKernelGGL((k_sub_predict<R, TX, TY, NP>), grid, dim3(NP), 0, c->stream, g, f,
                                  (const double*)c->b1, (const double*)c->b12,
                                  (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
                                  (const R*)F<R>(c, SWAP_F_NU), 2.0 * dtsub, tz, c->b1p2);
-           hipLaunchKernelGGL((k_sub_correct<R, TX, TY>), grid, block, 0, c->stream, g, f,
+           hipLaunchKernelGGL((k_sub_correct<R, TX, TY, NC>), grid, dim3(NC), 0, c->stream, g, f,
                               (const double*)c->b1p2, (const double*)c->b1,
                               (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
                               (const R*)F<R>(c, SWAP_F_NU), dtsub, tz, c->b12,
