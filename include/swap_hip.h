@@ -126,6 +126,9 @@ int swap_destroy(swap_ctx *ctx);
 const char *swap_last_error(const swap_ctx *ctx);
 int swap_sync(swap_ctx *ctx);
 int64_t swap_num_particles(const swap_ctx *ctx);
+/* the Philox key of later loads and injections (swap_config.seed at
+   creation): a fresh seeded realisation in the same context */
+int swap_set_seed(swap_ctx *ctx, uint64_t seed);
 /* geometry: qx(i) etc. as the device uses them (grd7, initial.f:147-301) */
 int swap_get_coords(swap_ctx *ctx, int axis, double *q /* n_axis+2 */);
 
@@ -243,12 +246,18 @@ typedef struct {
 int64_t swap_dn_num_params(void);
 int swap_dn_create(swap_ctx *ctx, const swap_dn_config *cfg);
 /* append a training pair from the current particles (needs SWAP_TRACK_IDS):
-   input = the stride subsample's moments, target = all particles' moments,
-   or with disjoint != 0 the moments of the other particles only
-   (id % stride != 0, weight x stride/(stride-1)), whose sampling noise is
-   independent of the input's; the context's np/up fields are left as they
-   were */
-int swap_dn_add_pair(swap_ctx *ctx, int disjoint);
+   input = the stride subsample's moments, target (mode):
+     SWAP_DN_ALL       all particles' moments;
+     SWAP_DN_DISJOINT  the moments of the other particles only (id % stride
+                       != 0, weight x stride/(stride-1)), whose sampling noise
+                       is independent of the input's;
+     SWAP_DN_DISJOINT_STEP  the same target from the sums of the last
+                       swap_step's final deposit minus the subsample's, with
+                       no pass over the other particles; refused (SWAP_ESTATE)
+                       if particles or moment sums changed since that step.
+   The context's np/up fields are left as they were. */
+enum { SWAP_DN_ALL = 0, SWAP_DN_DISJOINT = 1, SWAP_DN_DISJOINT_STEP = 2 };
+int swap_dn_add_pair(swap_ctx *ctx, int mode);
 int swap_dn_num_pairs(swap_ctx *ctx);
 /* copy pair k out ([4][ncz][ncy][ncx] FP32 each, normalised), or append a
    pair given on the host (e.g. held-out pairs of a high-ppc run, for the

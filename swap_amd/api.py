@@ -28,6 +28,7 @@ P_XP, P_VP, P_VP1, P_EP, P_VPLUS, P_VMINUS, P_IJKP, P_WGHT, P_CELL, P_FRAC, P_BE
 V_CUR, V_EXTRAP, V_VP1 = 0, 1, 2
 FP32, FP64 = 32, 64
 SPECIES, OUTFLOW_X, TRACK_IDS, UNFUSED_FIELDS = 1, 2, 4, 8
+DN_ALL, DN_DISJOINT, DN_DISJOINT_STEP = 0, 1, 2
 
 
 class SwapError(RuntimeError):
@@ -123,6 +124,7 @@ def load_library():
         "swap_diagnostics": ([vp, C.POINTER(Diag)], i32),
         "swap_profile": ([vp, i32], i32),
         "swap_profile_read": ([vp, vp, vp, i32], i32),
+        "swap_set_seed": ([vp, C.c_uint64], i32),
         "swap_dn_num_params": ([], i64),
         "swap_dn_create": ([vp, C.POINTER(DnConfig)], i32),
         "swap_dn_add_pair": ([vp, i32], i32),
@@ -275,6 +277,9 @@ class Swap:
         self._ck(self.lib.swap_get_coords(self.h, axis, q.ctypes.data), "get_coords")
         return q
 
+    def set_seed(self, seed):
+        self._ck(self.lib.swap_set_seed(self.h, seed), "set_seed")
+
     def sync(self):
         self._ck(self.lib.swap_sync(self.h), "sync")
 
@@ -385,8 +390,8 @@ class Swap:
         cfg = DnConfig(n_scale, v_scale, max_pairs, stride, seed)
         self._ck(self.lib.swap_dn_create(self.h, C.byref(cfg)), "dn_create")
 
-    def dn_add_pair(self, disjoint=False):
-        self._ck(self.lib.swap_dn_add_pair(self.h, 1 if disjoint else 0), "dn_add_pair")
+    def dn_add_pair(self, mode=DN_ALL):
+        self._ck(self.lib.swap_dn_add_pair(self.h, int(mode)), "dn_add_pair")
 
     def dn_num_pairs(self):
         return self.lib.swap_dn_num_pairs(self.h)

@@ -16,6 +16,8 @@
 // masked by the ReLU of the layer below; weight gradients are a second GEMM
 // (M = K entries, N = output channels, reduction over voxels) accumulated
 // into FP32 with atomics.  FP32 master weights, Adam.
+#include <utility>
+
 #include "swap_internal.h"
 
 namespace {
@@ -262,6 +264,20 @@ __global__ void k_adam(int64_t n, float* p, const float* g, float* m, float* v, 
   p[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + 1e-8f);
 }
 
+// io = (S A - io) / (S - 1) per moment sum: fixed-point sums (two's
+// complement in 64 bits, rounded back to the grid) or doubles
+template <typename T>
+__global__ void k_rest(int64_t n, const T* __restrict__ A, T* __restrict__ io, int S) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  if constexpr (sizeof(T) == 8 && !__is_same(T, double)) {
+    const long long dlt = (long long)S * (long long)A[q] - (long long)io[q];
+    io[q] = (T)(long long)llrint((double)dlt / (double)(S - 1));
+  } else {
+    io[q] = ((T)S * A[q] - io[q]) / (T)(S - 1);
+  }
+}
+
 __global__ void k_f2bf(int64_t n, const float* __restrict__ x, bf16* __restrict__ y) {
   int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q < n) y[q] = f2bf(x[q]);
@@ -337,6 +353,7 @@ struct DnState {
   double* dloss = nullptr;
   float* pin = nullptr;     // pairs: inputs [max_pairs][V][4]
   float* ptg = nullptr;     // pairs: targets
+  void* sacc = nullptr;     // moment sums of a subsample [4][nn] (stride > 1)
   int npairs = 0;
   int64_t t = 0;            // Adam step
   bool gate = false, enabled = false;
@@ -431,6 +448,7 @@ int dn_moments_in(swap_ctx* c, float* X) {
 }
 
 int zero_acc(swap_ctx* c, int m0, int m1) {
+  c->mgen++;
   size_t off = 8 * (size_t)c->g.nn * m0, len = 8 * (size_t)c->g.nn * (m1 - m0);
   HIPCHK(c, hipMemsetAsync((char*)c->macc + off, 0, len, c->stream));
   return 0;
@@ -512,6 +530,7 @@ extern "C" int swap_dn_create(swap_ctx* c, const swap_dn_config* cfg) {
   DA(d->dloss, sizeof(double));
   DA(d->pin, 16 * (size_t)V * (size_t)cfg->max_pairs);
   DA(d->ptg, 16 * (size_t)V * (size_t)cfg->max_pairs);
+  if (cfg->stride > 1) DA(d->sacc, 32 * (size_t)c->g.nn);
 #undef DA
   HIPCHK(c, hipMemsetAsync(d->am, 0, 4 * NPARAM, c->stream));
   HIPCHK(c, hipMemsetAsync(d->av, 0, 4 * NPARAM, c->stream));
@@ -526,7 +545,8 @@ extern "C" int swap_dn_create(swap_ctx* c, const swap_dn_config* cfg) {
 void dn_destroy(swap_ctx* c) {
   DnState* d = c->dn;
   if (!d) return;
-  void* ps[] = {d->params, d->grads, d->am, d->av, d->xf, d->out, d->dloss, d->pin, d->ptg};
+  void* ps[] = {d->params, d->grads, d->am, d->av, d->xf,
+                d->out,    d->dloss, d->pin, d->ptg, d->sacc};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (int l = 0; l < NL; l++) {
@@ -538,13 +558,51 @@ void dn_destroy(swap_ctx* c) {
   c->dn = nullptr;
 }
 
-extern "C" int swap_dn_add_pair(swap_ctx* c, int disjoint) {
-  if (!c) return SWAP_EINVAL;
+// the input and target moments of SWAP_DN_DISJOINT_STEP: macc holds the
+// step's sums A over all particles at x^{n+1/2} (before the final half move)
+// and stays untouched; the subsample (weights x S) is deposited at its
+// x^{n+1/2}, half a move back, into the scratch sums, and the target is
+// (S A - scratch) / (S - 1) = the other particles with weights x S/(S-1)
+static int dn_pair_from_step(swap_ctx* c, float* in, float* tg) {
+  DnState* d = c->dn;
+  const int S = d->cfg.stride;
+  const int64_t n4 = 4 * c->g.nn;
+  struct Swap {  // macc <-> scratch for the finalize kernels, restored on exit
+    swap_ctx* c;
+    ~Swap() { std::swap(c->macc, c->dn->sacc); }
+  };
+  std::swap(c->macc, d->sacc);
+  Swap guard{c};
+  HIPCHK(c, hipMemsetAsync(c->macc, 0, 8 * (size_t)n4, c->stream));
+  if (int e = pl_deposit_simple(c, 0, S, true)) return e;
+  if (int e = fl_finalize_np(c)) return e;
+  if (int e = pl_deposit_simple(c, 1, S, true)) return e;
+  if (int e = fl_finalize_up(c)) return e;
+  if (int e = dn_moments_in(c, in)) return e;
+  if (c->prec == 64)
+    hipLaunchKernelGGL(k_rest<double>, dim3(nblk(n4, 256)), dim3(256), 0, c->stream, n4,
+                       (const double*)d->sacc, (double*)c->macc, S);
+  else
+    hipLaunchKernelGGL(k_rest<unsigned long long>, dim3(nblk(n4, 256)), dim3(256), 0, c->stream,
+                       n4, (const unsigned long long*)d->sacc, (unsigned long long*)c->macc, S);
+  if (int e = post(c)) return e;
+  if (int e = fl_finalize_np(c)) return e;
+  if (int e = fl_finalize_up(c)) return e;
+  return dn_moments_in(c, tg);
+}
+
+extern "C" int swap_dn_add_pair(swap_ctx* c, int mode) {
+  if (!c || mode < SWAP_DN_ALL || mode > SWAP_DN_DISJOINT_STEP) return SWAP_EINVAL;
   if (int e = dn_need(c)) return e;
   DnState* d = c->dn;
   if (d->npairs >= d->cfg.max_pairs) return set_err(c, SWAP_ECAP, "dn_add_pair: pairs full");
   if (!c->id && d->cfg.stride > 1)
     return set_err(c, SWAP_ESTATE, "dn_add_pair needs SWAP_TRACK_IDS");
+  if (mode != SWAP_DN_ALL && d->cfg.stride < 2)
+    return set_err(c, SWAP_EINVAL, "dn_add_pair: a disjoint target needs stride > 1");
+  if (mode == SWAP_DN_DISJOINT_STEP && c->mgen != c->step_gen)
+    return set_err(c, SWAP_ESTATE,
+                   "dn_add_pair: particles or moment sums changed since the last swap_step");
   const size_t nn = (size_t)c->g.nn, rs = c->rsz;
   // keep the context's np/up
   HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_TMP2], c->fld[SWAP_F_NP], rs * nn,
@@ -554,12 +612,14 @@ extern "C" int swap_dn_add_pair(swap_ctx* c, int disjoint) {
   const int64_t V = d->g3.V;
   float* in = d->pin + (size_t)d->npairs * 4 * V;
   float* tg = d->ptg + (size_t)d->npairs * 4 * V;
-  if (int e = dn_deposit(c, d->cfg.stride)) return e;
-  if (int e = dn_moments_in(c, in)) return e;
-  if (disjoint && d->cfg.stride < 2)
-    return set_err(c, SWAP_EINVAL, "dn_add_pair: a disjoint target needs stride > 1");
-  if (int e = dn_deposit(c, disjoint ? -d->cfg.stride : 1)) return e;
-  if (int e = dn_moments_in(c, tg)) return e;
+  if (mode == SWAP_DN_DISJOINT_STEP) {
+    if (int e = dn_pair_from_step(c, in, tg)) return e;
+  } else {
+    if (int e = dn_deposit(c, d->cfg.stride)) return e;
+    if (int e = dn_moments_in(c, in)) return e;
+    if (int e = dn_deposit(c, mode == SWAP_DN_DISJOINT ? -d->cfg.stride : 1)) return e;
+    if (int e = dn_moments_in(c, tg)) return e;
+  }
   HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_NP], c->fld[SWAP_F_TMP2], rs * nn,
                            hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->fld[SWAP_F_UP], c->fld[SWAP_F_TMP1], 3 * rs * nn,

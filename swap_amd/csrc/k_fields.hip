@@ -966,12 +966,15 @@ int fl_copy_b(swap_ctx* c, const double* src, double* dst) {
 }
 // one fused subcycle: b1p2 = predictor(b1, b12); b12's buffer = corrector
 // (b1p2, b1); E and aj of the corrector are stored when last is set
-// tile of canonical columns per workgroup and z planes per chunk (FP32:
-// 32x8 measured best among 16..64 x 4..12 and 16..64 planes at 256x128x128
-// cells); the FP64 build's LDS rings take twice the bytes per position
+// tile of canonical columns per workgroup, z planes per chunk, threads of
+// the predictor (NP) and the corrector (NC).  FP32: 32x8 columns, 32 planes
+// and 512 threads measured best among 16..64 x 4..12 columns, 16..64 planes
+// and 256..1024 threads at 256x128x128 cells (more threads than columns hide
+// the latency of each z step's phases).  The FP64 build's LDS rings take
+// twice the bytes per position.
 template <typename R>
 struct FTile {
-  static constexpr int TX = 32, TY = 8, TZ = 32, NP = 256, NC = 256;
+  static constexpr int TX = 32, TY = 8, TZ = 32, NP = 512, NC = 512;
 };
 template <>
 struct FTile<double> {

@@ -28,18 +28,27 @@ __global__ void k_weights(int64_t n, int64_t cap, PDev<R> P, int32_t* ijkp, R* w
 // stride > 1 deposits only the particles with id % stride == 0, each with
 // stride times its weight (a low-ppc subsample of the same particles);
 // stride < -1 deposits the complement, id % |stride| != 0, each with
-// |stride| / (|stride| - 1) times its weight (a sample disjoint from it)
+// |stride| / (|stride| - 1) times its weight (a sample disjoint from it).
+// back != 0 deposits at x - (dt/2) v, the position before the last half move
+// (where swap_step's final deposit took place).
 template <typename R>
 __global__ void k_deposit_simple(Geo g, int64_t n, PDev<R> P, int mode,
-                                 typename AccOf<R>::T* acc, const uint32_t* id, int stride) {
+                                 typename AccOf<R>::T* acc, const uint32_t* id, int stride,
+                                 PCoef<R> K, int back) {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
   if (stride > 1 && id[p] % (uint32_t)stride != 0u) return;
   if (stride < -1 && id[p] % (uint32_t)(-stride) == 0u) return;
   uint32_t c = P.cell[p];
   int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
+  R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
+  if (back) {
+    ci = wrapc(ci + half_move(f0, -P.vx[p], K.s[0]), K.nc[0]);
+    cj = wrapc(cj + half_move(f1, -P.vy[p], K.s[1]), K.nc[1]);
+    ck = wrapc(ck + half_move(f2, -P.vz[p], K.s[2]), K.nc[2]);
+  }
   R w[8];
-  weights8(P.fx[p], P.fy[p], P.fz[p], w);
+  weights8(f0, f1, f2, w);
   R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
   if (stride > 1) r *= (R)stride;
   if (stride < -1) r *= (R)(-stride) / (R)(-stride - 1);
@@ -177,12 +186,12 @@ int pl_weights(swap_ctx* c) {
   PDISPATCH(c, PL((k_weights<R>), c->n, c->n, c->cap, pdev<R>(c), c->ijkp, (R*)c->wght));
   return post(c);
 }
-int pl_deposit_simple(swap_ctx* c, int mode, int stride) {
+int pl_deposit_simple(swap_ctx* c, int mode, int stride, bool back) {
   if (c->n == 0) return 0;
   if ((stride > 1 || stride < -1) && !c->id)
     return set_err(c, SWAP_ESTATE, "subsampled deposit needs ids");
   PDISPATCH(c, PL((k_deposit_simple<R>), c->n, c->g, c->n, pdev<R>(c), mode,
-                  (typename AccOf<R>::T*)c->macc, c->id, stride));
+                  (typename AccOf<R>::T*)c->macc, c->id, stride, pcoef<R>(c), back ? 1 : 0));
   return post(c);
 }
 int pl_get_Ep(swap_ctx* c) {

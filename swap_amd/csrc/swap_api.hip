@@ -95,6 +95,7 @@ int host_to_real(swap_ctx* c, void* d, size_t n, const double* hd, const float* 
 }
 
 int zero_moments(swap_ctx* c, int m0, int m1) {
+  c->mgen++;
   size_t off = 8 * (size_t)c->g.nn * m0, len = 8 * (size_t)c->g.nn * (m1 - m0);
   HIPCHK(c, hipMemsetAsync((char*)c->macc + off, 0, len, c->stream));
   return 0;
@@ -330,6 +331,12 @@ int swap_sync(swap_ctx* c) {
   return 0;
 }
 
+int swap_set_seed(swap_ctx* c, uint64_t seed) {
+  if (!c) return SWAP_EINVAL;
+  c->cfg.seed = seed;
+  return 0;
+}
+
 int64_t swap_num_particles(const swap_ctx* c) { return c ? c->n : -1; }
 
 int swap_get_coords(swap_ctx* c, int axis, double* q) {
@@ -345,6 +352,7 @@ int swap_get_coords(swap_ctx* c, int axis, double* q) {
 int swap_load_particles(swap_ctx* c, int64_t n, const double* xp, const double* vp,
                         const double* vp1, const double* beta_p, const double* mrat) {
   if (!c || n < 0 || (n > 0 && (!xp || !vp || !vp1))) return SWAP_EINVAL;
+  c->mgen++;
   if (n > c->cap) return set_err(c, SWAP_ECAP, "load: %lld > capacity", (long long)n);
   if ((beta_p || mrat) && !(c->cfg.flags & SWAP_SPECIES))
     return set_err(c, SWAP_EINVAL, "load: beta_p/mrat need SWAP_SPECIES");
@@ -382,6 +390,7 @@ int swap_load_particles_split(swap_ctx* c, int64_t n, const uint32_t* cell, cons
                               const void* vp, const void* vp1, const float* beta_p,
                               const float* mrat) {
   if (!c || n < 0 || (n > 0 && (!cell || !frac || !vp || !vp1))) return SWAP_EINVAL;
+  c->mgen++;
   if (n > c->cap) return set_err(c, SWAP_ECAP, "load: %lld > capacity", (long long)n);
   if ((beta_p || mrat) && !(c->cfg.flags & SWAP_SPECIES))
     return set_err(c, SWAP_EINVAL, "load: beta_p/mrat need SWAP_SPECIES");
@@ -427,6 +436,7 @@ int swap_load_particles_split(swap_ctx* c, int64_t n, const uint32_t* cell, cons
 int swap_load_maxwellian(swap_ctx* c, int64_t n, double vsw, double vth, double beta_p,
                          double mrat) {
   if (!c || n < 0) return SWAP_EINVAL;
+  c->mgen++;
   if (n > c->cap) return set_err(c, SWAP_ECAP, "load: %lld > capacity", (long long)n);
   if ((beta_p != 1.0 || mrat != 1.0) && !(c->cfg.flags & SWAP_SPECIES))
     return set_err(c, SWAP_EINVAL, "load: beta_p/mrat != 1 need SWAP_SPECIES");
@@ -504,6 +514,7 @@ int swap_get_particle_array(swap_ctx* c, int which, void* host) {
 
 int swap_set_particle_array(swap_ctx* c, int which, const void* host) {
   if (!c || !host) return SWAP_EINVAL;
+  c->mgen++;
   size_t n = (size_t)c->n, rn = c->rsz * n;
   if (!n) return 0;
   switch (which) {
@@ -669,12 +680,14 @@ int swap_improve_up(swap_ctx* c) {
 int swap_get_vp_final(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
   if (!c->vpl) return set_err(c, SWAP_ESTATE, "get_vp_final before get_vplus_vminus");
+  c->mgen++;
   return pl_vp_final(c);
 }
 
 int swap_move_ion_half(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
   c->n_sorted = -1;  // boxes of the work items no longer describe the particles
+  c->mgen++;
   return pl_move_half(c);
 }
 
@@ -866,6 +879,7 @@ int swap_step(swap_ctx* c, int nsteps) {
       TRY(swap_field_subcycles(c, ntf, dtsub));
     }
     c->step_count++;
+    c->step_gen = c->mgen;  // macc: the final deposit of this step
   }
   return 0;
 }
@@ -873,12 +887,14 @@ int swap_step(swap_ctx* c, int nsteps) {
 int swap_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
                    int64_t* injected) {
   if (!c || !injected) return SWAP_EINVAL;
+  c->mgen++;
   return sr_ionize_sw(c, step, ndot, vsw, beta_pu, injected);
 }
 
 int swap_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cloud, int64_t* injected,
                       int64_t* chex) {
   if (!c || !cloud || !injected || !chex) return SWAP_EINVAL;
+  c->mgen++;
   return sr_ionize_cloud(c, step, cloud, injected, chex);
 }
 

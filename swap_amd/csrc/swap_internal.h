@@ -81,6 +81,10 @@ struct swap_ctx {
   int64_t max_items = 0;
   int64_t n_sorted = -1;      // particles covered by the item list (-1: none)
   int steps_since_sort = 0;
+  // mutation count of the particles and the moment sums (macc); step_gen is
+  // its value when swap_step finished, so mgen == step_gen means macc still
+  // holds the sums of the last step's final deposit over today's particles
+  uint64_t mgen = 0, step_gen = ~0ull;
   // cloud source scratch (allocated on first use): per interior node
   uint32_t *ccnt = nullptr, *coff = nullptr;
   float* cbp = nullptr;
@@ -135,7 +139,7 @@ int fl_zero(swap_ctx* c, void* p, size_t bytes);
 // ---- particle launchers (k_particles.hip) ----
 int pl_weights(swap_ctx* c);
 // 0: np, 1: up(vp) 2: up(ext) 3: up(vp1); stride > 1: the id % stride == 0 subsample
-int pl_deposit_simple(swap_ctx* c, int mode, int stride = 1);
+int pl_deposit_simple(swap_ctx* c, int mode, int stride = 1, bool back = false);
 int pl_get_Ep(swap_ctx* c);
 int pl_vplus_vminus(swap_ctx* c);
 int pl_improve_vp1(swap_ctx* c);

@@ -4,8 +4,9 @@ of each.
 
     python -m swap_amd.denoise_study --cells 128 128 128 --ppc 256 --stride 32
 
-1. Pairs.  A high-ppc FP32 run (drifting Maxwellian plus a pickup-ion cloud,
-   so the moments have structure) writes a pair every --pair-every steps:
+1. Pairs.  High-ppc FP32 runs (drifting Maxwellian plus a pickup-ion cloud,
+   so the moments have structure; --segments seeded realisations) write a
+   pair every --pair-every steps from the step's own final deposit:
    input x = moments of the particles with id % S == 0 (weights x S, a
    ppc/S subsample), target t = moments of the other particles (weights
    x S/(S-1)).  The two samples are disjoint, so their sampling noises are
@@ -31,6 +32,8 @@ import json
 import math
 import sys
 import time
+
+import numpy as np
 
 from . import api, physics
 
@@ -83,43 +86,62 @@ def main(argv=None):
     ap.add_argument("--ppc", type=int, default=256)
     ap.add_argument("--stride", type=int, default=32)
     ap.add_argument("--spinup", type=int, default=20, help="steps before the first pair")
-    ap.add_argument("--pairs", type=int, default=160)
-    ap.add_argument("--pair-every", type=int, default=2)
-    ap.add_argument("--test-pairs", type=int, default=16)
-    ap.add_argument("--iters", type=int, default=2000)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--pairs", type=int, default=500)
+    ap.add_argument("--segments", type=int, default=10,
+                    help="seeded realisations the pairs are drawn from")
+    ap.add_argument("--pair-every", type=int, default=1)
+    ap.add_argument("--test-pairs", type=int, default=50,
+                    help="held out: the last pairs, from the last segment(s)")
+    ap.add_argument("--iters", type=int, default=6000)
+    ap.add_argument("--lr", type=float, default=1e-3,
+                    help="Adam step for the first half; 0.3x for the next 30%%, 0.1x after")
+    ap.add_argument("--n0-rate", type=float, default=1.0,
+                    help="cloud ions born per node and step at r0 (structure in the moments)")
+    ap.add_argument("--region", type=float, default=16.0,
+                    help="radius (cells) of the cloud region reported on its own")
     ap.add_argument("--time-steps", type=int, default=5)
     ap.add_argument("--no-timing", action="store_true")
     args = ap.parse_args(argv)
     nc = tuple(args.cells)
     S = args.stride
     p = physics.workload(*nc, args.ppc)
-    cl = physics.cloud(p, N0_rate=4.0, r0_cells=3.0, H_cells=12.0)
-    nsteps = args.spinup + args.pairs * args.pair_every
+    cl = physics.cloud(p, N0_rate=args.n0_rate, r0_cells=3.0, H_cells=12.0)
 
-    # 1. pairs from the high-ppc FP32 run
+    # 1. pairs from the high-ppc FP32 run: --segments seeded realisations,
+    # each a fresh load with B = B0, spun up, then a pair every --pair-every
+    # steps (short segments keep the cloud from driving the plasma into the
+    # low-density, high-B state whose subcycling blows up the step time)
     t0 = time.perf_counter()
-    s = physics.make_swap(p, capacity=_capacity(p, nsteps),
+    per_seg = -(-args.pairs // args.segments)
+    s = physics.make_swap(p, capacity=_capacity(p, args.spinup + per_seg * args.pair_every),
                           flags=api.TRACK_IDS | api.SPECIES)
-    s.load_maxwellian(p["npart"], p["vsw"], p["vth"])
     s.dn_create(p["n0"], p["vsw"], max_pairs=args.pairs, stride=S, seed=5)
-    s.prime()
-    k = 0
-    for k in range(args.spinup):
-        s.ionize_cloud(k, cl)
-        s.step(1)
-    tp = 0.0
-    for j in range(args.pairs):
-        for _ in range(args.pair_every):
-            k += 1
+    b0 = np.zeros((3, s.nz, s.ny, s.nx))
+    b0[2] = p["b0z"]
+    tp, j = 0.0, 0
+    for seg in range(args.segments):
+        s.set_seed(20261203 + 7919 * seg)
+        s.load_maxwellian(p["npart"], p["vsw"], p["vth"])
+        s.set_field("b1", b0)
+        s.set_field("b12", b0)
+        s.prime()
+        k = 0
+        for k in range(args.spinup):
             s.ionize_cloud(k, cl)
             s.step(1)
-        s.sync()
-        t1 = time.perf_counter()
-        s.dn_add_pair(disjoint=True)
-        tp += time.perf_counter() - t1
-        if (j + 1) % 20 == 0:
-            _log("pair %d/%d, %.1f s" % (j + 1, args.pairs, time.perf_counter() - t0))
+        for _ in range(per_seg):
+            if j == args.pairs:
+                break
+            for _ in range(args.pair_every):
+                k += 1
+                s.ionize_cloud(k, cl)
+                s.step(1)
+            s.sync()
+            t1 = time.perf_counter()
+            s.dn_add_pair(api.DN_DISJOINT_STEP)
+            tp += time.perf_counter() - t1
+            j += 1
+        _log("segment %d: %d pairs, %.1f s" % (seg, j, time.perf_counter() - t0))
     n_high = s.n
     gen_s = time.perf_counter() - t0
 
@@ -130,13 +152,29 @@ def main(argv=None):
     done, loss = 0, float("nan")
     while done < args.iters:
         n = min(250, args.iters - done)
-        loss = s.dn_train(n, args.lr, ntr)
+        lr = args.lr * (1.0 if done < 0.5 * args.iters else 0.3 if done < 0.8 * args.iters else 0.1)
+        loss = s.dn_train(n, lr, ntr)
         done += n
-        _log("iter %d loss %.4g" % (done, loss))
+        _log("iter %d lr %.2g loss %.4g" % (done, lr, loss))
     s.sync()
     train_s = time.perf_counter() - t0
     raw, net = s.dn_eval(ntr, args.pairs)
     held = [s.dn_get_pair(i) for i in range(ntr, args.pairs)]
+    # where the structure is: the same decomposition restricted to the
+    # voxels within --region cells of the cloud centre, and the variance of
+    # the truth about its mean (signal) next to the noise, on held-out pairs
+    zz, yy, xx = np.meshgrid(*(np.arange(n) + 1.0 for n in nc[::-1]), indexing="ij")
+    rc = np.sqrt((xx - cl["cx"] / p["dx"]) ** 2 + (yy - cl["cy"] / p["dx"]) ** 2 +
+                 (zz - cl["cz"] / p["dx"]) ** 2)
+    inside = rc <= args.region
+    acc = {"raw_in": 0.0, "net_in": 0.0, "tvar": 0.0, "tvar_in": 0.0}
+    nh = min(len(held), 10)
+    for x, t in held[:nh]:
+        y = s.dn_infer(x)
+        acc["raw_in"] += float(np.mean((x - t)[:, inside] ** 2)) / nh
+        acc["net_in"] += float(np.mean((y - t)[:, inside] ** 2)) / nh
+        acc["tvar"] += float(np.mean(t.var(axis=(1, 2, 3)))) / nh
+        acc["tvar_in"] += float(np.mean(t[:, inside].var(axis=1))) / nh
     params = s.dn_params()
     s.close()
     s2 = raw * (S - 1) / S ** 2
@@ -146,6 +184,7 @@ def main(argv=None):
     out = {
         "study": "learned moment denoiser",
         "cells": list(nc), "ppc_high": args.ppc, "ppc_low": args.ppc // S,
+        "cloud_n0_rate": args.n0_rate, "segments": args.segments,
         "pairs_train": ntr, "pairs_test": args.test_pairs, "pair_every": args.pair_every,
         "particles_high": n_high, "pair_gen_s": gen_s, "pair_deposit_s": tp,
         "iters": args.iters, "lr": args.lr, "train_s": train_s, "last_loss": loss,
@@ -156,6 +195,16 @@ def main(argv=None):
         "reaches_high_ppc_error": bool(e_net <= s2),
         "units": "mean squared error over (np/n0 - 1, up/vsw), per value",
     }
+    s2_in = acc["raw_in"] * (S - 1) / S ** 2
+    e_in = acc["net_in"] - acc["raw_in"] / S
+    out.update({
+        "region_cells": args.region, "region_voxels": int(inside.sum()),
+        "signal_var": acc["tvar"] - S / (S - 1) * s2,
+        "signal_var_region": acc["tvar_in"] - S / (S - 1) * s2_in,
+        "mse_plain_high_ppc_region": s2_in, "mse_denoised_low_ppc_region": e_in,
+        "ppc_equivalent_region": args.ppc * s2_in / e_in if e_in > 0 else float("inf"),
+        "reaches_high_ppc_error_region": bool(e_in <= s2_in),
+    })
     _log(json.dumps(out))
 
     # 3. wall time per step

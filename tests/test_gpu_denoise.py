@@ -61,7 +61,7 @@ def test_disjoint_target_and_pair_transfer():
     s.prime()
     s.step(1)
     s.dn_add_pair()               # target: all particles
-    s.dn_add_pair(disjoint=True)  # target: id % S != 0
+    s.dn_add_pair(api.DN_DISJOINT)  # target: id % S != 0
     x0, t_all = s.dn_get_pair(0)
     x1, t_rest = s.dn_get_pair(1)
     assert np.array_equal(x0, x1)
@@ -111,3 +111,55 @@ def test_training_beats_plain_low_ppc_and_gate():
     assert np.isfinite(s.get_field("np")).all() and not np.array_equal(np_before,
                                                                        s.get_field("np"))
     s.dn_enable(False)
+
+
+def test_disjoint_target_from_the_step_sums():
+    """SWAP_DN_DISJOINT_STEP: the input is the subsample deposited where the
+    step's final deposit took place (x^{n+1/2}, half a move back), checked
+    against the FP64 oracle's update_np of those positions; the target is
+    the step's sums minus the subsample's, so S np(sub) + (S-1) np(rest) =
+    S np of the step; the mode is refused before a step and once particles
+    or moment sums changed after it"""
+    import oracle as O
+    from helpers import rel
+    nc, S = (12, 8, 6), 4
+    p = physics.workload(*nc, 16)
+    s = physics.make_swap(p, flags=api.TRACK_IDS | api.SPECIES, capacity=p["npart"] + 20000)
+    s.load_maxwellian(p["npart"], p["vsw"], p["vth"])
+    s.dn_create(p["n0"], p["vsw"], max_pairs=4, stride=S)
+    with pytest.raises(api.SwapError):
+        s.dn_add_pair(api.DN_DISJOINT_STEP)
+    s.prime()
+    s.step(1)
+    np_step = s.get_field("np")
+    s.dn_add_pair(api.DN_DISJOINT_STEP)
+    s.dn_add_pair(api.DN_DISJOINT_STEP)  # nothing changed in between
+    x0, t0 = s.dn_get_pair(0)
+    x1, t1 = s.dn_get_pair(1)
+    assert np.array_equal(x0, x1) and np.array_equal(t0, t1)
+    assert np.array_equal(s.get_field("np"), np_step)  # the context keeps its np
+    inner = (slice(1, -1),) * 3
+    n_sub, n_rest = x0[0] + 1.0, t0[0] + 1.0
+    assert np.allclose(n_sub + (S - 1) * n_rest, S * np_step[inner] / p["n0"], rtol=2e-5,
+                       atol=2e-5 * S)
+    # the subsample at x^{n+1/2}, deposited by the oracle with weights x S
+    sel = s.particle_array(api.P_ID) % S == 0
+    xp = np.ascontiguousarray(s.xp()[:, sel])
+    back = np.ascontiguousarray(-s.vp()[:, sel].astype(np.float64))
+    n = xp.shape[1]
+    o = O.Oracle(s.nx, s.ny, s.nz, p["dx"], p["dx"], p["dx"], p["dt"], p["ntsub"], p["alpha"],
+                 p["beta"] / S)
+    o.move_ion_half(n, n, xp, back)
+    ijkp, wght, npo = np.zeros((3, n), np.int32), o.parts(n, 8), o.grid(1)
+    o.get_interp_weights(n, n, xp, ijkp, wght)
+    o.update_np(n, n, ijkp, wght, None, npo)
+    assert rel(n_sub * p["n0"], npo[inner]) < 1e-5
+    # refused once the sums or the particles changed
+    s.update_np()
+    with pytest.raises(api.SwapError):
+        s.dn_add_pair(api.DN_DISJOINT_STEP)
+    s.step(1)
+    s.dn_add_pair(api.DN_DISJOINT_STEP)
+    s.ionize_cloud(3, physics.cloud(p, N0_rate=1.0, r0_cells=2.0, H_cells=4.0))
+    with pytest.raises(api.SwapError):
+        s.dn_add_pair(api.DN_DISJOINT_STEP)
