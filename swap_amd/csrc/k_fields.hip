@@ -6,6 +6,8 @@
 // each storage node, so ghost layers come out equal to what boundary.f's
 // periodic() would copy (boundary.f:9-52) and no separate ghost pass runs.
 // B is held and advanced in FP64; everything else in the context precision R.
+#include <algorithm>
+
 #include "swap_internal.h"
 
 namespace {
@@ -308,15 +310,17 @@ __global__ void k_finalize_up(Geo g, const typename AccOf<R>::T* a, const R* np,
 }
 
 // check_time_step's per-node whistler bound (gutsf.f:1782-1792)
-// The grid minimum goes through one atomic per wavefront (a 64-lane min
-// first); non-negative floats order as their bit patterns.
+// A grid-stride loop over all storage nodes with a bounded number of
+// workgroups; the grid minimum takes one atomic per workgroup (atomics on
+// one address serialise).  Non-negative floats order as their bit patterns.
+constexpr int DT_BLOCKS = 1024;
 template <typename R>
-__global__ void k_deltat(Geo g, FC f, const double* b, const R* np, R* dtn, float* dmin) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+__global__ __launch_bounds__(256) void k_deltat(Geo g, FC f, const double* b, const R* np,
+                                                R* dtn, float* dmin) {
+  __shared__ unsigned int wmin[4];
   unsigned int u = 0xFFFFFFFFu;
-  if (i < g.nx && j < g.ny) {
-    const int64_t o = gidx(g, i, j, blockIdx.z);
+  for (int64_t o = blockIdx.x * (int64_t)256 + threadIdx.x; o < g.nn;
+       o += (int64_t)gridDim.x * 256) {
     R b1 = (R)b[o], b2 = (R)b[o + g.nn], b3 = (R)b[o + 2 * g.nn];
     R ak = (R)2. / (R)f.dx;
     R btot = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
@@ -329,10 +333,13 @@ __global__ void k_deltat(Geo g, FC f, const double* b, const R* np, R* dtn, floa
     dtn[o] = dt;
     float d = (float)dt;
     if (!(d >= 0.f)) d = 0.f;  // NaN or negative: treat as failing
-    u = __float_as_uint(d);
+    u = min(u, __float_as_uint(d));
   }
   for (int off = 32; off > 0; off >>= 1) u = min(u, (unsigned int)__shfl_xor((int)u, off));
-  if ((threadIdx.x & 63) == 0) atomicMin((unsigned int*)dmin, u);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = u;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMin((unsigned int*)dmin, min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3])));
 }
 
 // the reference halves dtsub once per failing node, in its loop order
@@ -931,7 +938,10 @@ int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub) {
   double dtsub0 = c->cfg.dt / (double)c->cfg.ntsub;
   float big = 3.0e38f;
   HIPCHK(c, hipMemcpyAsync(c->dmin, &big, sizeof(float), hipMemcpyHostToDevice, c->stream));
-  DISPATCH(c, LAUNCH((k_deltat<R>), c->g, f, c->b1, F<R>(c, SWAP_F_NP), (R*)c->scratch, c->dmin));
+  const unsigned nb = (unsigned)std::min<int64_t>(DT_BLOCKS, (c->g.nn + 255) / 256);
+  DISPATCH(c, hipLaunchKernelGGL((k_deltat<R>), dim3(nb), dim3(256), 0, c->stream, c->g, f,
+                                 (const double*)c->b1, (const R*)F<R>(c, SWAP_F_NP),
+                                 (R*)c->scratch, c->dmin));
   HIPCHK(c, hipGetLastError());
   float mn = 0;
   HIPCHK(c, hipMemcpyAsync(&mn, c->dmin, sizeof(float), hipMemcpyDeviceToHost, c->stream));
