@@ -228,7 +228,7 @@ int swap_diagnostics(swap_ctx *ctx, swap_diag *out);
 
 /* ---- learned moment denoiser (build-only; no reference counterpart) ----
    A 3-D convolutional net, 4 -> 32 -> 32 -> 32 -> 4 channels, 3x3x3 kernels
-   with periodic padding, ReLU between layers and a residual output, that maps
+   with periodic padding, leaky ReLU between layers and a residual output, that maps
    the moments (np, up) of a low-ppc deposit to those of a high-ppc one.
    Convolutions run on MFMA with BF16 operands and FP32 accumulation; master
    weights are FP32 and trained with Adam by the library's own forward and

@@ -3,8 +3,9 @@
 TEST INFRASTRUCTURE ONLY: the checker of the device's forward and backward
 kernels.  The net has no counterpart in the reference; this is its
 mathematical definition: 4 -> 32 -> 32 -> 32 -> 4 channels, 3x3x3 kernels
-with periodic padding (tap = dx+1 + 3(dy+1) + 9(dz+1)), ReLU after layers
-1-3, residual output, mean squared error.  With bf16=True every tensor the
+with periodic padding (tap = dx+1 + 3(dy+1) + 9(dz+1)), leaky ReLU (slope
+LEAK = 1/16 below zero) after layers 1-3, residual output, mean squared
+error.  With bf16=True every tensor the
 device stores in BF16 (activations, weights, gradients passed between
 layers) is rounded the same way (round to nearest even) and every sum is
 taken in float64; with bf16=False it is the exact float64 network.
@@ -16,6 +17,7 @@ import numpy as np
 
 CH = (4, 32, 32, 32, 4)
 NL = 4
+LEAK = 0.0625
 
 
 def n_params():
@@ -79,7 +81,7 @@ def forward_backward(params, x, target, shape, bf=True):
     Ws = [r(W) for W, _ in layers]
     for l in range(NL - 1):
         z = conv(acts[-1], Ws[l], nbr) + layers[l][1]
-        acts.append(r(np.maximum(z, 0.0)))
+        acts.append(r(np.where(z > 0, z, LEAK * z)))
     out = conv(acts[-1], Ws[NL - 1], nbr) + layers[NL - 1][1] + x
     n = out.size
     d = out - target
@@ -89,6 +91,7 @@ def forward_backward(params, x, target, shape, bf=True):
     for l in range(NL - 1, -1, -1):
         grads[l] = (wgrad(acts[l], g, nbr), g.sum(0))
         if l > 0:
-            g = r(np.where(acts[l] > 0, conv_adj(g, Ws[l], nbr), 0.0))
+            a = conv_adj(g, Ws[l], nbr)
+            g = r(np.where(acts[l] > 0, a, LEAK * a))
     flat = np.concatenate([np.concatenate([gw.ravel(), gb]) for gw, gb in grads])
     return loss, out, flat

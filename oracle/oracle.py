@@ -71,7 +71,7 @@ def _load(width):
         "or_cloud_counts": ([vp, dbl, dbl, dbl, dbl, dbl, vp, i32, C.c_uint64, C.c_uint32, vp,
                              vp], None),
         "or_cloud_ions": ([vp, vp, C.c_uint64, C.c_uint32, vp, vp, i64], None),
-        "or_chex": ([i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, C.c_uint64,
+        "or_chex": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, C.c_uint64,
                      C.c_uint32, vp], None),
     }
     for name, (args, res) in sig.items():
@@ -234,17 +234,20 @@ def cloud_ions(nc, cnt, seed, step):
     return cell, frac
 
 
-def chex(cell, f, v, d, dt, cloud, seed, step):
-    """charge-exchange decision (uint8) per ion; f, v float32 [3, n]"""
+def chex(cell, f, v, d, dt, cloud, seed, step, ids=None):
+    """charge-exchange decision (uint8) per ion; f, v float32 [3, n]; the draw
+    of an ion is keyed by its id when ids are given, else by its index"""
     lib, _ = _load(64)
     cell = np.ascontiguousarray(cell, np.uint32)
     f = [np.ascontiguousarray(a, np.float32) for a in f]
     v = [np.ascontiguousarray(a, np.float32) for a in v]
+    idp = None if ids is None else np.ascontiguousarray(ids, np.uint32)
     cl = np.array([cloud[k] for k in CLOUD_KEYS], np.float64)
     hit = np.zeros(cell.size, np.uint8)
     lib.or_chex(cell.size, cell.ctypes.data, f[0].ctypes.data, f[1].ctypes.data,
-                f[2].ctypes.data, v[0].ctypes.data, v[1].ctypes.data, v[2].ctypes.data, d[0],
-                d[1], d[2], dt, cl.ctypes.data, seed, step, hit.ctypes.data)
+                f[2].ctypes.data, v[0].ctypes.data, v[1].ctypes.data, v[2].ctypes.data,
+                None if idp is None else idp.ctypes.data, d[0], d[1], d[2], dt, cl.ctypes.data,
+                seed, step, hit.ctypes.data)
     return hit
 
 

@@ -877,8 +877,9 @@ void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t
 }
 
 void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, const float *fz,
-             const float *vx, const float *vy, const float *vz, double dx, double dy, double dz,
-             double dt, const double *cl, uint64_t seed, uint32_t step, uint8_t *hit) {
+             const float *vx, const float *vy, const float *vz, const uint32_t *id, double dx,
+             double dy, double dz, double dt, const double *cl, uint64_t seed, uint32_t step,
+             uint8_t *hit) {
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   for (int64_t p = 0; p < n; p++) {
     uint32_t c = cell[p];
@@ -887,7 +888,8 @@ void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, 
     double z = ((double)(c >> 21) + 1.0 + (double)fz[p]) * dz;
     double a = vx[p], b = vy[p], e = vz[p];
     double prob = dt * or_cloud_nn(cl, x, y, z) * cl[8] * sqrt(a * a + b * b + e * e);
-    uint32_t ctr[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), 0x40000u + step, 0u}, r[4];
+    const uint64_t who = id ? (uint64_t)id[p] : (uint64_t)p; /* the ion's identity */
+    uint32_t ctr[4] = {(uint32_t)who, (uint32_t)(who >> 32), 0x40000u + step, 0u}, r[4];
     or_philox4x32_10(ctr, key, r);
     hit[p] = or_u24(r[0]) < prob ? 1 : 0;
   }

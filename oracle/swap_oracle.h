@@ -136,8 +136,10 @@ void or_cloud_counts(const int nc[3], double dx, double dy, double dz, double be
                      uint32_t *cnt, float *bp);
 void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t step,
                    uint32_t *cell, double *frac, int64_t m);
+/* id: the ions' ids (the Philox counter; NULL: the array index) */
 void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, const float *fz,
-             const float *vx, const float *vy, const float *vz, double dx, double dy, double dz,
-             double dt, const double *cl, uint64_t seed, uint32_t step, uint8_t *hit);
+             const float *vx, const float *vy, const float *vz, const uint32_t *id, double dx,
+             double dy, double dz, double dt, const double *cl, uint64_t seed, uint32_t step,
+             uint8_t *hit);
 
 #endif

@@ -279,6 +279,7 @@ class Swap:
 
     def set_seed(self, seed):
         self._ck(self.lib.swap_set_seed(self.h, seed), "set_seed")
+        self.cfg.seed = seed
 
     def sync(self):
         self._ck(self.lib.swap_sync(self.h), "sync")

@@ -2,7 +2,8 @@
 // and run on the device with the library's own kernels.
 //
 // Net: 4 -> 32 -> 32 -> 32 -> 4 channels, 3x3x3 kernels with periodic
-// padding, ReLU after the first three layers, residual output (out = in +
+// padding, leaky ReLU (slope 1/16 below zero, so no unit can die) after the
+// first three layers, residual output (out = in +
 // conv4(...)).  Activations are channels-last BF16 [voxel][channel] over the
 // interior nodes (i fastest).  Every convolution is an implicit GEMM on MFMA
 // (v_mfma_f32_16x16x16_bf16, FP32 accumulation): M = voxels, N = output
@@ -13,7 +14,8 @@
 //
 // Backward: the loss gradient is propagated with the same kernel through the
 // tap-flipped, transposed weights (the adjoint of a periodic convolution),
-// masked by the ReLU of the layer below; weight gradients are a second GEMM
+// scaled by the leaky ReLU's slope of the layer below (1 or 1/16, by the sign
+// of its stored activation); weight gradients are a second GEMM
 // (M = K entries, N = output channels, reduction over voxels) accumulated
 // into FP32 with atomics.  FP32 master weights, Adam.
 #include <utility>
@@ -44,6 +46,7 @@ constexpr int k16(int cin) { return (27 * cin + 15) / 16 * 16; }
 constexpr int r16(int c) { return (c + 15) / 16 * 16; }
 
 enum { EPI_RELU = 0, EPI_RESID = 1, EPI_MASK = 2 };
+constexpr float LEAK = 0.0625f;  // leaky ReLU slope below zero
 
 struct G3 {
   int nx, ny, nz;
@@ -140,11 +143,11 @@ __global__ __launch_bounds__(256) void k_conv_t(G3 g, const bf16* __restrict__ X
         float val = acc[mt][nt][e];
         if constexpr (EPI == EPI_RELU) {
           val += bias[co];
-          Yb[o] = f2bf(val > 0.f ? val : 0.f);
+          Yb[o] = f2bf(val > 0.f ? val : LEAK * val);
         } else if constexpr (EPI == EPI_RESID) {
           Yf[o] = val + bias[co] + resid[o];
         } else {
-          Yb[o] = f2bf(bf2f(mask[o]) > 0.f ? val : 0.f);
+          Yb[o] = f2bf(bf2f(mask[o]) > 0.f ? val : LEAK * val);
         }
       }
     }
@@ -346,8 +349,8 @@ struct DnState {
   float* av = nullptr;
   bf16* Wt[NL] = {};
   bf16* Wb[NL] = {};
-  bf16* act[NL] = {};       // act[0] input (4 ch), act[1..3] post-ReLU (32 ch)
-  bf16* gr[NL + 1] = {};    // gradients w.r.t. the inputs of layer l (pre-ReLU of l-1); gr[NL] = dOut
+  bf16* act[NL] = {};       // act[0] input (4 ch), act[1..3] after the leaky ReLU (32 ch)
+  bf16* gr[NL + 1] = {};    // gradients w.r.t. the inputs of layer l (pre-activation of l-1); gr[NL] = dOut
   float* xf = nullptr;      // normalised input [V][4]
   float* out = nullptr;     // output [V][4]
   double* dloss = nullptr;
@@ -413,7 +416,7 @@ int dn_backward(swap_ctx* c, const float* tgt) {
     gw[l] = d->grads + param_off(l);
     gb[l] = gw[l] + wcount(l);
   }
-  // layer 4 (32 -> 4, no ReLU)
+  // layer 4 (32 -> 4, no activation)
   hipLaunchKernelGGL((k_wgrad<32, 4>), dim3(nch, k16(32) / 16, 1), dim3(64), 0, c->stream, g,
                      d->act[3], d->gr[4], gw[3], chunk);
   hipLaunchKernelGGL(k_bgrad<4>, dim3(256), dim3(256), 0, c->stream, V, d->gr[4], gb[3]);

@@ -393,9 +393,9 @@ __global__ void k_cloud_fill(Geo g, int64_t n0, PDev<R> P, CloudDev cl, uint32_t
 // probability dt n_n sigma |v|; a hit stops the ion (v = 0) and makes it a
 // cloud ion (mrat = 1/m_pu)
 template <typename R>
-__global__ void k_chex(int64_t n, PDev<R> P, double dx, double dy, double dz, double dt,
-                       CloudDev cl, uint32_t k0, uint32_t k1, uint32_t stream, float* mr,
-                       unsigned long long* count) {
+__global__ void k_chex(int64_t n, PDev<R> P, const uint32_t* id, double dx, double dy,
+                       double dz, double dt, CloudDev cl, uint32_t k0, uint32_t k1,
+                       uint32_t stream, float* mr, unsigned long long* count) {
 #pragma clang fp contract(off)
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -405,7 +405,10 @@ __global__ void k_chex(int64_t n, PDev<R> P, double dx, double dy, double dz, do
   double z = ((double)cell_k(c) + 1.0 + (double)P.fz[p]) * dz;
   double vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
   double prob = dt * cloud_nn(cl, x, y, z) * cl.sigma * sqrt(vx * vx + vy * vy + vz * vz);
-  uint32_t r[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};
+  // the draw belongs to the ion: keyed by its id when ids are kept, so it
+  // does not depend on where the sort has put the ion
+  const uint64_t who = id ? (uint64_t)id[p] : (uint64_t)p;
+  uint32_t r[4] = {(uint32_t)who, (uint32_t)(who >> 32), stream, 0u};
   philox4x32_10(r, k0, k1);
   if (u24(r[0]) < prob) {
     P.vx[p] = (R)0; P.vy[p] = (R)0; P.vz[p] = (R)0;
@@ -445,7 +448,8 @@ int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* inject
   if (c->n > 0 && cl->sigma_chex > 0) {
     HIPCHK(c, hipMemsetAsync(c->dcnt + 3, 0, sizeof(unsigned long long), c->stream));
     SDISPATCH(c, hipLaunchKernelGGL((k_chex<R>), dim3(nblk(c->n, 256)), dim3(256), 0, c->stream,
-                                    c->n, pdev<R>(c), c->cfg.dx, c->cfg.dy, c->cfg.dz,
+                                    c->n, pdev<R>(c), (const uint32_t*)c->id, c->cfg.dx,
+                                    c->cfg.dy, c->cfg.dz,
                                     c->cfg.dt, d, k0, k1, 0x40000u + (uint32_t)step, c->mr,
                                     c->dcnt + 3));
     if (int e = post(c)) return e;

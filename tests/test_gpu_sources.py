@@ -30,6 +30,9 @@ def test_ionize_cloud_matches_oracle():
     # a cross-section that makes charge exchange frequent near the core
     nn_r0 = cl["N0"] * math.exp(-cl["r0"] / cl["H"])
     cl["sigma_chex"] = 0.05 / (p["dt"] * nn_r0 * 400.0)
+    s.sort()  # ions no longer sit at their id: the chex draw follows the id
+    ids0 = s.particle_array(api.P_ID)
+    assert not np.array_equal(ids0, np.arange(n, dtype=np.uint32))
     cell0 = s.particle_array(api.P_CELL)
     frac0 = s.particle_array(api.P_FRAC)
     v0 = s.vp()
@@ -38,7 +41,7 @@ def test_ionize_cloud_matches_oracle():
     inj, nchex = s.ionize_cloud(step, cl)
 
     d = (p["dx"],) * 3
-    hit = O.chex(cell0, frac0, v0, d, p["dt"], cl, SEED, step).astype(bool)
+    hit = O.chex(cell0, frac0, v0, d, p["dt"], cl, SEED, step, ids=ids0).astype(bool)
     assert nchex == int(hit.sum()) and nchex > 0
     v1 = s.vp()
     mr1 = s.particle_array(api.P_MRAT)

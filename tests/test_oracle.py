@@ -658,3 +658,26 @@ def test_step_fp32_tracks_fp64_and_conserves_charge():
     beta = n / (np.prod(NC) * PAR["dx"] ** 3 * 0.1e15)
     total = a["np"][1:-1, 1:-1, 1:-1].sum() * PAR["dx"] ** 3 * beta
     assert total == pytest.approx(n, rel=1e-12)
+
+
+def test_chex_draw_follows_the_ion_id():
+    """keyed by id, a permutation of the ions permutes the decisions (the
+    sort cannot change who is hit); keyed by index it does not"""
+    p, cl = _cloud_setup()
+    x = np.array([9.5, 5.5, 5.5]) * p["dx"]
+    r = max(np.linalg.norm(x - np.array([cl["cx"], cl["cy"], cl["cz"]])), cl["rmin"])
+    nn = cl["N0"] * (cl["r0"] / r) ** 2 * math.exp(-r / cl["H"])
+    cl = dict(cl, sigma_chex=0.3 / (p["dt"] * nn * 400.0))
+    n = 5000
+    cell = np.full(n, 8 | (4 << 11) | (4 << 21), np.uint32)
+    f = np.full((3, n), 0.5, np.float32)
+    v = np.zeros((3, n), np.float32)
+    v[0] = 400.0
+    ids = np.arange(n, dtype=np.uint32) * 7 + 3
+    perm = np.random.default_rng(1).permutation(n)
+    d = (p["dx"],) * 3
+    h = O.chex(cell, f, v, d, p["dt"], cl, 9, 2, ids=ids)
+    hp = O.chex(cell[perm], f[:, perm], v[:, perm], d, p["dt"], cl, 9, 2, ids=ids[perm])
+    assert np.array_equal(hp, h[perm]) and 0 < h.sum() < n
+    hi = O.chex(cell[perm], f[:, perm], v[:, perm], d, p["dt"], cl, 9, 2)
+    assert not np.array_equal(hi, h[perm])
