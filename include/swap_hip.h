@@ -46,7 +46,18 @@ enum { SWAP_FP32 = 32, SWAP_FP64 = 64 };
 /* config flags */
 enum {
   SWAP_SPECIES = 1,        /* per-particle beta_p and mrat arrays exist */
-  SWAP_OUTFLOW_X = 2,      /* x is inflow/outflow instead of periodic */
+  SWAP_OUTFLOW_X = 2,      /* x is inflow/outflow instead of periodic
+                              (build-defined; the reference is periodic,
+                              gutsp.f:490-510, boundary.f:9-52): ghosts in x
+                              are zero-gradient copies, the density of the
+                              face nodes 1 and ncx is divided by their share
+                              of the particle domain (1.5 and 0.5 cells),
+                              ions that leave through either x face during
+                              swap_step are dropped by a sort at the end of
+                              the step (swap_diag.escaped counts them), and
+                              swap_set_inflow feeds the x = 0 face.
+                              Needs ncx >= 4 and sort_every >= 1; the field
+                              subcycles run unfused; no denoiser. */
   SWAP_TRACK_IDS = 4,      /* keep a particle id through sorts (tests) */
   SWAP_UNFUSED_FIELDS = 8  /* field subcycles of swap_step as one kernel per
                               reference routine (nine launches) instead of the
@@ -222,6 +233,20 @@ typedef struct {
 } swap_cloud;
 int swap_ionize_cloud(swap_ctx *ctx, int step, const swap_cloud *cloud, int64_t *injected,
                       int64_t *chex);
+
+/* Inflow through the x = 0 face of a SWAP_OUTFLOW_X context (build-defined;
+   the reference's nearest routine refills a boundary layer to the upstream
+   density, check_min_den_boundary gutsp.f:575-816).  Once set, every
+   swap_step adds the ions of an upstream drifting Maxwellian (ppc ions per
+   cell, bulk vsw along x, thermal vth as swap_load_maxwellian draws it)
+   that cross the face during the step (the exact kinetic flux): candidates
+   fill a slab of L = ceil((|vsw| + 4.08 vth) dt/dx) cells in front of the
+   face at the start of the step and make the step's two half moves; one
+   inside after the first deposits into the step's moments at x^{n+1/2},
+   one inside after the second is appended with vp1 = vp, mrat = 1 and
+   beta_p as given, its id continuing the id counter.  The step count keys
+   the draws.  ppc = 0 switches it off. */
+int swap_set_inflow(swap_ctx *ctx, double ppc, double vsw, double vth, double beta_p);
 
 /* ---- diagnostics (FP64 reductions; Energy_diag part_init.f:14-139) ---- */
 int swap_diagnostics(swap_ctx *ctx, swap_diag *out);

@@ -64,13 +64,17 @@ def _load(width):
         "or_get_vp_final": ([vp, i32, i32, vp, vp, vp, vp], None),
         "or_move_ion_half": ([vp, i32, i32, vp, vp], None),
         "or_step": ([vp, vp], None),
+        "or_set_open_x": ([vp, i32], None),
+        "or_inflow": ([C.c_uint64, C.c_uint32, vp, dbl, vp, dbl, dbl, dbl, vp, vp, vp, vp,
+                       i64], i64),
+        "or_set_inflow": ([vp, C.c_uint64, dbl, dbl, dbl, dbl, C.c_uint32], None),
         "or_philox4x32_10": ([vp, vp, vp], None),
         "or_split_move": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
         "or_stable_sort": ([i64, vp, C.c_uint32, vp], None),
         "or_load_maxwellian": ([i64, C.c_uint64, C.c_uint32, vp, dbl, dbl, vp, vp, vp], None),
         "or_cloud_counts": ([vp, dbl, dbl, dbl, dbl, dbl, vp, i32, C.c_uint64, C.c_uint32, vp,
                              vp], None),
-        "or_cloud_ions": ([vp, vp, C.c_uint64, C.c_uint32, vp, vp, i64], None),
+        "or_cloud_ions": ([vp, vp, C.c_uint64, C.c_uint32, i32, vp, vp, i64], None),
         "or_chex": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, C.c_uint64,
                      C.c_uint32, vp], None),
     }
@@ -89,7 +93,8 @@ class OrState(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "b1", "b12", "b1p2", "bt", "btc", "np", "up", "aj", "E", "nu",
         "xp", "vp", "vp1", "vplus", "vminus", "Ep", "wght", "beta_p", "mrat",
-        "ijkp")] + [("Ni_tot", C.c_int), ("Ni_max", C.c_int)]
+        "ijkp", "id")] + [("Ni_tot", C.c_int), ("Ni_max", C.c_int), ("escaped", C.c_int),
+                          ("injected", C.c_int)]
 
 
 class Oracle:
@@ -161,11 +166,23 @@ class Oracle:
     def step(self, st):
         """st: dict of numpy arrays (see OrState); advances one step in place."""
         s = OrState()
-        for n, _ in OrState._fields_[:-2]:
+        for n, _ in OrState._fields_[:-4]:
             setattr(s, n, _p(st.get(n)))
         s.Ni_tot = st["Ni_tot"]
         s.Ni_max = st["xp"].shape[1]
         self.lib.or_step(self.h, C.byref(s))
+        # an open x removes escaped ions: the live ones are the first Ni_tot
+        st["Ni_tot"] = s.Ni_tot
+        st["escaped"] = s.escaped
+        st["injected"] = s.injected
+
+    def set_open_x(self, on=True):
+        """build-defined inflow/outflow x instead of periodic"""
+        self.lib.or_set_open_x(self.h, 1 if on else 0)
+
+    def set_inflow(self, seed, ppc, vsw, vth, beta_p=1.0, next_id=0):
+        """inflow through the open x face in every later step"""
+        self.lib.or_set_inflow(self.h, seed, ppc, vsw, vth, beta_p, next_id)
 
 
 def initparameters(b0_init=0.2e-9, ion_amu=1, nf_init=0.1e15, dt_frac=0.05,
@@ -221,7 +238,7 @@ def cloud_counts(nc, d, beta, dt, cloud, seed, step):
     return cnt, bp
 
 
-def cloud_ions(nc, cnt, seed, step):
+def cloud_ions(nc, cnt, seed, step, open_x=False):
     """(cell uint32[m], frac float64[3,m]) of the new ions in node order"""
     lib, _ = _load(64)
     nc = np.asarray(nc, np.int32)
@@ -229,9 +246,26 @@ def cloud_ions(nc, cnt, seed, step):
     m = int(cnt.sum())
     cell = np.zeros(m, np.uint32)
     frac = np.zeros((3, m))
-    lib.or_cloud_ions(nc.ctypes.data, cnt.ctypes.data, seed, step, cell.ctypes.data,
-                      frac.ctypes.data, m)
+    lib.or_cloud_ions(nc.ctypes.data, cnt.ctypes.data, seed, step, 1 if open_x else 0,
+                      cell.ctypes.data, frac.ctypes.data, m)
     return cell, frac
+
+
+def inflow(seed, step, nc, dt, d, ppc, vsw, vth, cap=1 << 22):
+    """the inflow candidates of one step that deposit or are appended, in
+    slot order: (th float64[3,m], te float64[3,m], v float64[3,m],
+    flag uint8[m]); th, te are cell coordinates after the first and second
+    half move, flag bit 0 = deposits at x^{n+1/2}, bit 1 = appended"""
+    lib, _ = _load(64)
+    nc = np.asarray(nc, np.int32)
+    d = np.asarray(d, np.float64)
+    th, te, v = np.zeros((3, cap)), np.zeros((3, cap)), np.zeros((3, cap))
+    flag = np.zeros(cap, np.uint8)
+    m = lib.or_inflow(seed, step, nc.ctypes.data, dt, d.ctypes.data, ppc, vsw, vth,
+                      th.ctypes.data, te.ctypes.data, v.ctypes.data, flag.ctypes.data, cap)
+    if m < 0 or m > cap:
+        raise ValueError("or_inflow: bad parameters or more than %d candidates" % cap)
+    return th[:, :m].copy(), te[:, :m].copy(), v[:, :m].copy(), flag[:m].copy()
 
 
 def chex(cell, f, v, d, dt, cloud, seed, step, ids=None):

@@ -26,6 +26,15 @@ struct orc {
   real *dx_cell, *dy_cell, *dz_cell, *xrat, *yrat, *zrat;
   /* module scratch of global.f:59 (a, c, ct) */
   real *a, *c, *ct;
+  /* open x (build-defined inflow/outflow; the reference is periodic) and
+     its inflow (or_set_inflow): seed, upstream ions per cell, drift,
+     thermal speed, beta_p, the id of the next ion; step counts or_step
+     calls (it keys the inflow draws) */
+  int ox;
+  uint64_t in_seed;
+  double in_ppc, in_vsw, in_vth, in_bp;
+  uint32_t next_id;
+  int64_t step;
 };
 
 /* Fortran (i,j,k) / (i,j,k,m), 1-based, column-major */
@@ -128,6 +137,7 @@ void or_get_coord(const orc *h, int which, real *out) {
 }
 
 void or_set_dtsub(orc *h, double dtsub, double ntf) { h->dtsub = (real)dtsub; h->ntf = (real)ntf; }
+void or_set_open_x(orc *h, int on) { h->ox = on ? 1 : 0; }
 void or_get_dtsub(const orc *h, double *dtsub, double *ntf) { *dtsub = h->dtsub; *ntf = h->ntf; }
 
 /* initparameters, inputs.f:125-172 (constants inputs.f:19-67) */
@@ -157,14 +167,17 @@ void or_grd6_b0(const orc *h, double b0_init, double mion, real *b0) {
   }
 }
 
-/* periodic, boundary.f:9-52: x, then y, then z ghost copies */
+/* periodic, boundary.f:9-52: x, then y, then z ghost copies.  An open x
+   (build-defined) copies the nearest interior layer instead: ghost 1 takes
+   layer 2, ghost nx takes layer nx-1 (zero gradient). */
 void or_periodic(const orc *h, real *b) {
   int nx = h->nx, ny = h->ny, nz = h->nz;
+  int sx1 = h->ox ? 2 : nx - 1, sxn = h->ox ? nx - 1 : 2;
   for (int m = 1; m <= 3; m++) {
     for (int k = 1; k <= nz; k++)
       for (int j = 1; j <= ny; j++) {
-        b[F4(1, j, k, m)] = b[F4(nx - 1, j, k, m)];
-        b[F4(nx, j, k, m)] = b[F4(2, j, k, m)];
+        b[F4(1, j, k, m)] = b[F4(sx1, j, k, m)];
+        b[F4(nx, j, k, m)] = b[F4(sxn, j, k, m)];
       }
   }
   for (int m = 1; m <= 3; m++)
@@ -184,10 +197,11 @@ void or_periodic(const orc *h, real *b) {
 /* periodic_scalar, boundary.f:167-206 */
 void or_periodic_scalar(const orc *h, real *b) {
   int nx = h->nx, ny = h->ny, nz = h->nz;
+  int sx1 = h->ox ? 2 : nx - 1, sxn = h->ox ? nx - 1 : 2;
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++) {
-      b[F3(1, j, k)] = b[F3(nx - 1, j, k)];
-      b[F3(nx, j, k)] = b[F3(2, j, k)];
+      b[F3(1, j, k)] = b[F3(sx1, j, k)];
+      b[F3(nx, j, k)] = b[F3(sxn, j, k)];
     }
   for (int k = 1; k <= nz; k++)
     for (int i = 1; i <= nx; i++) {
@@ -507,13 +521,25 @@ static inline void corner(int c, int i, int j, int k, int *ci, int *cj, int *ck)
   *ci = i + di[c - 1]; *cj = j + dj[c - 1]; *ck = k + dk[c - 1];
 }
 
-/* fold ghost layer 1 into n-1 per axis (gutsp.f:1496-1498, 1803-1805) */
+/* fold ghost layer 1 into n-1 per axis (gutsp.f:1496-1498, 1803-1805).
+   An open x folds ghost layer 1 onto layer 2 instead and then divides the
+   two face layers by their share of the particle domain: layer 2 collects
+   the whole first cell and half the second (1.5 cells), layer nx-1 half the
+   last cell (0.5 cells). */
 static void fold(const orc *h, real *g, int ncomp) {
   int nx = h->nx, ny = h->ny, nz = h->nz;
+  int tx = h->ox ? 2 : nx - 1;
   for (int m = 1; m <= ncomp; m++) {
     real *s = g + NN * (size_t)(m - 1);
     for (int k = 1; k <= nz; k++)
-      for (int j = 1; j <= ny; j++) s[F3(nx - 1, j, k)] += s[F3(1, j, k)];
+      for (int j = 1; j <= ny; j++) {
+        s[F3(tx, j, k)] += s[F3(1, j, k)];
+        if (h->ox) {
+          s[F3(1, j, k)] = 0;
+          s[F3(2, j, k)] *= R(2.0) / R(3.0);
+          s[F3(nx - 1, j, k)] *= R(2.0);
+        }
+      }
     for (int k = 1; k <= nz; k++)
       for (int i = 1; i <= nx; i++) s[F3(i, ny - 1, k)] += s[F3(i, 1, k)];
     for (int j = 1; j <= ny; j++)
@@ -672,7 +698,8 @@ void or_get_vp_final(const orc *h, int Ni_tot, int Ni_max, const real *Ep, real 
 }
 
 /* move_ion_half, gutsp.f:454-570: x += dt/2 v, one periodic wrap per axis
-   into (q(1), q(n-1)] */
+   into (q(1), q(n-1)]; an open x is not wrapped (or_remove_escaped then
+   drops the ions outside) */
 void or_move_ion_half(const orc *h, int Ni_tot, int Ni_max, real *xp, const real *vp) {
   real dth = h->dt / R(2.);
   const real *q[3] = {h->qx, h->qy, h->qz};
@@ -681,16 +708,66 @@ void or_move_ion_half(const orc *h, int Ni_tot, int Ni_max, real *xp, const real
     for (int m = 1; m <= 3; m++) {
       real x = P(xp, l, m) + dth * P(vp, l, m);
       real q1 = q[m - 1][0], qn = q[m - 1][n[m - 1] - 2];
-      if (x > qn) x = q1 + (x - qn);
-      else if (x <= q1) x = qn - (q1 - x);
+      if (!(m == 1 && h->ox)) {
+        if (x > qn) x = q1 + (x - qn);
+        else if (x <= q1) x = qn - (q1 - x);
+      }
       P(xp, l, m) = x;
     }
 }
+
+/* an open x (build-defined): drop the ions outside (q(1), q(nx-1)] in x,
+   keeping the order of the rest (remove_ion, gutsp.f:12-46, one ion at a
+   time there); returns the count removed */
+int or_remove_escaped(const orc *h, or_state *s) {
+  const int N = s->Ni_tot, M = s->Ni_max;
+  const real q1 = h->qx[0], qn = h->qx[h->nx - 2];
+  int w = 0;
+  for (int l = 0; l < N; l++) {
+    real x = s->xp[l];
+    if (x <= q1 || x > qn) continue;
+    for (int m = 0; m < 3; m++) {
+      s->xp[(size_t)m * M + w] = s->xp[(size_t)m * M + l];
+      s->vp[(size_t)m * M + w] = s->vp[(size_t)m * M + l];
+      s->vp1[(size_t)m * M + w] = s->vp1[(size_t)m * M + l];
+    }
+    if (s->beta_p) s->beta_p[w] = s->beta_p[l];
+    if (s->mrat) s->mrat[w] = s->mrat[l];
+    if (s->id) s->id[w] = s->id[l];
+    w++;
+  }
+  s->Ni_tot = w;
+  return N - w;
+}
+
+static void inflow_append(const orc *h, or_state *s, const double *t, const double *v,
+                          int64_t cap, int64_t c, uint32_t id);
 
 /* one step in the order of maind.f:385-479 (ionization and the energy
    diagnostic of maind.f:379-393 left out) */
 void or_step(orc *h, or_state *s) {
   int N = s->Ni_tot, M = s->Ni_max;
+  s->escaped = 0;
+  s->injected = 0;
+  /* an open x with inflow: this step's candidates (swap_set_inflow) */
+  int64_t nin = 0, cap = 0;
+  double *th = NULL, *te = NULL, *vin = NULL;
+  uint8_t *flag = NULL;
+  uint32_t *ids = NULL;
+  if (h->ox && h->in_ppc > 0) {
+    const int nc[3] = {h->nx - 2, h->ny - 2, h->nz - 2};
+    const double d[3] = {h->dx, h->dy, h->delz};
+    cap = (int64_t)(h->in_ppc * (h->nx - 2) * (h->ny - 2) * (h->nz - 2)) + 1024;
+    th = (double *)malloc(sizeof(double) * 3 * cap);
+    te = (double *)malloc(sizeof(double) * 3 * cap);
+    vin = (double *)malloc(sizeof(double) * 3 * cap);
+    flag = (uint8_t *)malloc((size_t)cap);
+    ids = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+    nin = or_inflow(h->in_seed, (uint32_t)h->step, nc, h->dt, d, h->in_ppc, h->in_vsw, h->in_vth,
+                    th, te, vin, flag, cap);
+    if (nin > cap) nin = cap;
+    for (int64_t c = 0; c < nin; c++) ids[c] = (flag[c] & 2) ? h->next_id++ : 0xFFFFFFFFu;
+  }
   or_get_interp_weights(h, N, M, s->xp, s->ijkp, s->wght);
   or_update_np(h, N, M, s->ijkp, s->wght, s->beta_p, s->np);
   or_update_up(h, N, M, s->ijkp, s->wght, s->beta_p, s->vp, s->np, s->up);
@@ -704,6 +781,13 @@ void or_step(orc *h, or_state *s) {
   or_get_vplus_vminus(h, N, M, s->ijkp, s->wght, s->mrat, s->Ep, s->btc, s->vp, s->vplus, s->vminus);
   or_get_vp_final(h, N, M, s->Ep, s->vp, s->vp1, s->vplus);
   or_move_ion_half(h, N, M, s->xp, s->vp);
+  if (h->ox) {
+    s->escaped += or_remove_escaped(h, s);
+    /* candidates inside at x^{n+1/2} join before the deposit */
+    for (int64_t c = 0; c < nin; c++)
+      if (flag[c] & 1) inflow_append(h, s, th, vin, cap, c, ids[c]);
+    N = s->Ni_tot;
+  }
   or_get_interp_weights(h, N, M, s->xp, s->ijkp, s->wght);
   or_update_np(h, N, M, s->ijkp, s->wght, s->beta_p, s->np);
   or_update_up(h, N, M, s->ijkp, s->wght, s->beta_p, s->vp, s->np, s->up);
@@ -717,6 +801,18 @@ void or_step(orc *h, or_state *s) {
     or_f_update_tlev(h, s->b1, s->b12, s->b1p2, s->bt);
   }
   or_move_ion_half(h, N, M, s->xp, s->vp);
+  if (h->ox) {
+    /* candidates that entered in the first half and left in the second go
+       with the escapes but are neither injected nor escaped ions */
+    int passing = 0;
+    for (int64_t c = 0; c < nin; c++) passing += flag[c] == 1;
+    s->escaped += or_remove_escaped(h, s) - passing;
+    for (int64_t c = 0; c < nin; c++)
+      if ((flag[c] & 2) && !(flag[c] & 1)) inflow_append(h, s, te, vin, cap, c, ids[c]);
+    for (int64_t c = 0; c < nin; c++) s->injected += (flag[c] & 2) ? 1 : 0;
+  }
+  free(th); free(te); free(vin); free(flag); free(ids);
+  h->step++;
 }
 
 /* ---------------------------------------------------------------- */
@@ -853,7 +949,7 @@ void or_cloud_counts(const int nc[3], double dx, double dy, double dz, double be
 }
 
 void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t step,
-                   uint32_t *cell, double *frac, int64_t m) {
+                   int open_x, uint32_t *cell, double *frac, int64_t m) {
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   int64_t q = 0, p = 0;
   for (int k = 1; k <= nc[2]; k++)
@@ -867,7 +963,9 @@ void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t
           int ijk[3];
           for (int a = 0; a < 3; a++) {
             double t = (double)s[a] + or_u24(r[a]) - 0.5; /* chem_rates.f:363-365 */
-            if (t > (double)nc[a]) t -= (double)nc[a];
+            /* past the last face: the periodic image, or the mirror image
+               on an open x */
+            if (t > (double)nc[a]) t = a == 0 && open_x ? 2.0 * nc[a] - t : t - (double)nc[a];
             double ci = ceil(t) - 1.0;
             ijk[a] = (int)ci;
             frac[a * m + p] = t - ci;
@@ -893,4 +991,106 @@ void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, 
     or_philox4x32_10(ctr, key, r);
     hit[p] = or_u24(r[0]) < prob ? 1 : 0;
   }
+}
+
+/* the inflow candidates of one step (build-defined inflow through the open
+   x face; cf. check_min_den_boundary gutsp.f:575-816, which refills a
+   boundary layer instead): slab depth L = max(1, ceil((|vsw| + 4.08 vth)
+   dt/dx)) cells, nslot = round(ppc L ncy ncz) candidates; slot s draws
+   Philox counter (s, step, 0x50000, d) d = 0..2 keyed by the seed, start
+   position x = (1-u) L - L, y = (1-u) ncy, z = (1-u) ncz (cell units),
+   velocity as the loader, then two half moves t += v dt/(2 d) (explicit fma;
+   y and z wrapped into (0, nc]).  flag bit 0: inside after the first half
+   move (deposits at x^{n+1/2}); bit 1: inside after the second and not past
+   the far face after the first (appended).  Writes th, te, v ([3][cap]) and
+   flag of the flagged slots in slot order; returns their count (at most cap
+   written), or -1 for bad parameters. */
+static double or_wrap_open(double t, double n) {
+  if (t > n || t <= 0.0) {
+    t -= (ceil(t / n) - 1.0) * n;
+    if (t <= 0.0) t += n;
+    else if (t > n) t -= n;
+  }
+  return t;
+}
+int64_t or_inflow(uint64_t seed, uint32_t step, const int nc[3], double dt, const double d[3],
+                  double ppc, double vsw, double vth, double *th, double *te, double *v,
+                  uint8_t *flag, int64_t cap) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  double sh[3] = {0.5 * dt / d[0], 0.5 * dt / d[1], 0.5 * dt / d[2]};
+  double reach = (fabs(vsw) + 4.08 * vth) * dt / d[0];
+  if (!(reach < 1048576.0)) return -1;
+  int L = (int)ceil(reach);
+  if (L < 1) L = 1;
+  double ns = round(ppc * (double)L * (double)nc[1] * (double)nc[2]);
+  if (!(ns < 4.0e9)) return -1;
+  uint32_t nslot = (uint32_t)ns;
+  int64_t m = 0;
+  for (uint32_t q = 0; q < nslot; q++) {
+    uint32_t r[3][4];
+    for (int w = 0; w < 3; w++) {
+      uint32_t ctr[4] = {q, step, 0x50000u, (uint32_t)w};
+      or_philox4x32_10(ctr, key, r[w]);
+    }
+    const uint32_t *ur[6] = {&r[0][3], &r[1][0], &r[1][1], &r[1][2], &r[1][3], &r[2][0]};
+    double vv[3], t0[3], a1[3], a2[3];
+    for (int a = 0; a < 3; a++) {
+      double u1 = 1.0 - or_u24(*ur[2 * a]);
+      double u2 = or_u24(*ur[2 * a + 1]);
+      vv[a] = (a == 0 ? vsw : 0.0) + vth * sqrt(-log(u1)) * cos(3.14159 * u2);
+    }
+    t0[0] = (1.0 - or_u24(r[0][0])) * (double)L - (double)L;
+    t0[1] = (1.0 - or_u24(r[0][1])) * (double)nc[1];
+    t0[2] = (1.0 - or_u24(r[0][2])) * (double)nc[2];
+    for (int a = 0; a < 3; a++) {
+      a1[a] = fma(vv[a], sh[a], t0[a]);
+      a2[a] = fma(vv[a], sh[a], a1[a]);
+    }
+    for (int a = 1; a < 3; a++) {
+      a1[a] = or_wrap_open(a1[a], (double)nc[a]);
+      a2[a] = or_wrap_open(a2[a], (double)nc[a]);
+    }
+    double ncx = (double)nc[0];
+    int dep = a1[0] > 0.0 && a1[0] <= ncx;
+    int keep = a2[0] > 0.0 && a2[0] <= ncx && a1[0] <= ncx;
+    if (!dep && !keep) continue;
+    if (m < cap) {
+      for (int a = 0; a < 3; a++) {
+        th[a * cap + m] = a1[a];
+        te[a * cap + m] = a2[a];
+        v[a * cap + m] = vv[a];
+      }
+      flag[m] = (uint8_t)(dep | (keep << 1));
+    }
+    m++;
+  }
+  return m;
+}
+
+void or_set_inflow(orc *h, uint64_t seed, double ppc, double vsw, double vth, double beta_p,
+                   uint32_t next_id) {
+  h->in_seed = seed;
+  h->in_ppc = ppc;
+  h->in_vsw = vsw;
+  h->in_vth = vth;
+  h->in_bp = beta_p;
+  h->next_id = next_id;
+}
+
+/* append candidate c of the inflow list at cell coordinates t (th or te) */
+static void inflow_append(const orc *h, or_state *s, const double *t, const double *v,
+                          int64_t cap, int64_t c, uint32_t id) {
+  if (s->Ni_tot >= s->Ni_max) return;
+  const int M = s->Ni_max, l = s->Ni_tot;
+  const real d[3] = {h->dx, h->dy, h->delz};
+  const real *q[3] = {h->qx, h->qy, h->qz};
+  for (int a = 0; a < 3; a++) {
+    s->xp[(size_t)a * M + l] = q[a][0] + (real)t[a * cap + c] * d[a];
+    s->vp[(size_t)a * M + l] = (real)v[a * cap + c];
+    s->vp1[(size_t)a * M + l] = (real)v[a * cap + c];
+  }
+  if (s->beta_p) s->beta_p[l] = (real)h->in_bp;
+  if (s->mrat) s->mrat[l] = R(1.0);
+  if (s->id) s->id[l] = id;
+  s->Ni_tot = l + 1;
 }

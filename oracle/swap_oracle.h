@@ -104,9 +104,23 @@ typedef struct {
   real *b1, *b12, *b1p2, *bt, *btc, *np, *up, *aj, *E, *nu;
   real *xp, *vp, *vp1, *vplus, *vminus, *Ep, *wght, *beta_p, *mrat;
   int32_t *ijkp;
+  uint32_t *id;    /* optional ion ids, carried through or_remove_escaped */
   int Ni_tot, Ni_max;
+  int escaped;     /* ions or_step removed through an open x */
+  int injected;    /* ions its inflow appended */
 } or_state;
 void or_step(orc *h, or_state *s);
+/* --- open x (build-defined inflow/outflow; the reference is periodic):
+       zero-gradient x ghosts, the x ghost layer folded onto layer 2, face
+       layers divided by their share of the particle domain, no x wrap in
+       move_ion_half, ions outside (q(1), q(nx-1)] removed after each half
+       move --- */
+void or_set_open_x(orc *h, int on);
+int or_remove_escaped(const orc *h, or_state *s);
+/* inflow through the open x face in every later or_step (ppc = 0: none);
+   next_id: the id of the first ion it appends */
+void or_set_inflow(orc *h, uint64_t seed, double ppc, double vsw, double vth, double beta_p,
+                   uint32_t next_id);
 
 /* --- build-only pieces (no reference counterpart; FP32/integer mirror of
        the device arithmetic so integer bookkeeping can be bit-compared) --- */
@@ -135,7 +149,11 @@ void or_cloud_counts(const int nc[3], double dx, double dy, double dz, double be
                      const double *cl, int max_per_cell, uint64_t seed, uint32_t step,
                      uint32_t *cnt, float *bp);
 void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t step,
-                   uint32_t *cell, double *frac, int64_t m);
+                   int open_x, uint32_t *cell, double *frac, int64_t m);
+/* the inflow candidates of one step (build-defined): see swap_oracle.c */
+int64_t or_inflow(uint64_t seed, uint32_t step, const int nc[3], double dt, const double d[3],
+                  double ppc, double vsw, double vth, double *th, double *te, double *v,
+                  uint8_t *flag, int64_t cap);
 /* id: the ions' ids (the Philox counter; NULL: the array index) */
 void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, const float *fz,
              const float *vx, const float *vy, const float *vz, const uint32_t *id, double dx,

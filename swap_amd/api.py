@@ -120,6 +120,7 @@ def load_library():
         "swap_prime": ([vp], i32),
         "swap_step": ([vp, i32], i32),
         "swap_ionize_sw": ([vp, i32, dbl, dbl, dbl, C.POINTER(i64)], i32),
+        "swap_set_inflow": ([vp, dbl, dbl, dbl, dbl], i32),
         "swap_ionize_cloud": ([vp, i32, C.POINTER(Cloud), C.POINTER(i64), C.POINTER(i64)], i32),
         "swap_diagnostics": ([vp, C.POINTER(Diag)], i32),
         "swap_profile": ([vp, i32], i32),
@@ -375,6 +376,12 @@ class Swap:
         self._ck(self.lib.swap_ionize_sw(self.h, step, ndot, vsw, beta_pu, C.byref(n)),
                  "ionize_sw")
         return n.value
+
+    def set_inflow(self, ppc, vsw, vth, beta_p=1.0):
+        """every later step of an OUTFLOW_X context adds the ions of an
+        upstream drifting Maxwellian (ppc per cell) that cross x = 0 during
+        the step; ppc = 0 switches it off"""
+        self._ck(self.lib.swap_set_inflow(self.h, ppc, vsw, vth, beta_p), "set_inflow")
 
     def ionize_cloud(self, step, cloud):
         """photoionization + charge exchange of a neutral cloud; cloud is a

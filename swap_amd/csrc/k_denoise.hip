@@ -504,6 +504,8 @@ extern "C" int swap_dn_create(swap_ctx* c, const swap_dn_config* cfg) {
   if (!c || !cfg) return SWAP_EINVAL;
   if (!(cfg->n_scale > 0) || !(cfg->v_scale > 0) || cfg->max_pairs < 0 || cfg->stride < 1)
     return set_err(c, SWAP_EINVAL, "dn_create: bad config");
+  if (c->g.ox)  // the convolutions pad periodically in every axis
+    return set_err(c, SWAP_EINVAL, "dn_create: the denoiser needs a periodic x");
   dn_destroy(c);
   DnState* d = new DnState();
   c->dn = d;

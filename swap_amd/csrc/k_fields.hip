@@ -31,7 +31,7 @@ FC fc_of(swap_ctx* c) {
   int j = blockIdx.y * blockDim.y + threadIdx.y;                           \
   int k = blockIdx.z;                                                      \
   if (i >= g.nx || j >= g.ny) return;                                      \
-  int ci = canon(i, g.nx), cj = canon(j, g.ny), ck = canon(k, g.nz);       \
+  int ci = canon_x(g, i), cj = canon(j, g.ny), ck = canon(k, g.nz);        \
   const int64_t o = gidx(g, i, j, k);                                      \
   (void)o;                                                                 \
   (void)ci; (void)cj; (void)ck;
@@ -144,7 +144,9 @@ __global__ void k_cross_smooth(Geo g, FC f, const R* a, const R* btc0, R* ct) {
   GRID_LOOP_HEAD
   R aa[3];
   f2c3(g, f, a, ci, cj, ck, aa);
-  int im = canon(ci - 1, g.nx), jm = canon(cj - 1, g.ny), km = canon(ck - 1, g.nz);
+  // the face average below node 1 is the average's own ghost: periodic, or
+  // on an open x a zero-gradient copy of the average at node 1
+  int im = canon_x(g, ci - 1), jm = canon(cj - 1, g.ny), km = canon(ck - 1, g.nz);
   R m1c = (R)0.5 * (AT(btc0, 0, ci, cj, ck) + AT(btc0, 0, ci + 1, cj, ck));
   R m1m = (R)0.5 * (AT(btc0, 0, im, cj, ck) + AT(btc0, 0, im + 1, cj, ck));
   R m2c = (R)0.5 * (AT(btc0, 1, ci, cj, ck) + AT(btc0, 1, ci, cj + 1, ck));
@@ -288,7 +290,9 @@ __global__ void k_update_src(Geo g, FC f, const R* up, const R* ajc, R* upc, R* 
 template <typename R>
 __global__ void k_finalize_np(Geo g, const typename AccOf<R>::T* a, R qn, R* np) {
   GRID_LOOP_HEAD
-  np[o] = (R)(acc_read(a, gidx(g, ci, cj, ck), fix_bits(0)) * (double)qn);
+  double v = acc_read(a, gidx(g, ci, cj, ck), fix_bits(0)) * (double)qn;
+  if (g.ox) v *= face_scale(g, ci);
+  np[o] = (R)v;
 }
 
 // update_up's division by np and node->edge average (gutsp.f:1704-1756,
@@ -297,14 +301,24 @@ template <typename R>
 __global__ void k_finalize_up(Geo g, const typename AccOf<R>::T* a, const R* np, R qn, R* up) {
   GRID_LOOP_HEAD
   int64_t q0 = gidx(g, ci, cj, ck);
-  int64_t qn1[3] = {gidx(g, canon(ci + 1, g.nx), cj, ck), gidx(g, ci, canon(cj + 1, g.ny), ck),
+  const int ix = canon_x(g, ci + 1);
+  int64_t qn1[3] = {gidx(g, ix, cj, ck), gidx(g, ci, canon(cj + 1, g.ny), ck),
                     gidx(g, ci, cj, canon(ck + 1, g.nz))};
   R n0 = np[q0];
+  // np of an open x face node carries its share factor; the velocity sums
+  // take the same factor so that up is their plain ratio
+  const R s0 = (R)face_scale(g, ci), s1x = (R)face_scale(g, ix);
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     R n1 = np[qn1[m]];
-    R c0 = n0 != (R)0 ? (R)acc_read(a, q0 + m * g.nn, fix_bits(1)) * qn / n0 : (R)0;
-    R c1 = n1 != (R)0 ? (R)acc_read(a, qn1[m] + m * g.nn, fix_bits(1)) * qn / n1 : (R)0;
+    R a0 = (R)acc_read(a, q0 + m * g.nn, fix_bits(1)) * qn;
+    R a1 = (R)acc_read(a, qn1[m] + m * g.nn, fix_bits(1)) * qn;
+    if (g.ox) {
+      a0 *= s0;
+      a1 *= m == 0 ? s1x : s0;
+    }
+    R c0 = n0 != (R)0 ? a0 / n0 : (R)0;
+    R c1 = n1 != (R)0 ? a1 / n1 : (R)0;
     up[o + m * g.nn] = (R)0.5 * (c0 + c1);
   }
 }

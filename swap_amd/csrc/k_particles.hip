@@ -62,7 +62,7 @@ __global__ void k_deposit_simple(Geo g, int64_t n, PDev<R> P, int mode,
 #pragma unroll
   for (int cc = 0; cc < 8; cc++) {
     int u = ci + (cc & 1), v = cj + ((cc >> 2) & 1), t = ck + ((cc >> 1) & 1);
-    int64_t node = gidx(g, node_of(u, g.ncx), node_of(v, g.ncy), node_of(t, g.ncz));
+    int64_t node = gidx(g, node_of_x(g, u), node_of(v, g.ncy), node_of(t, g.ncz));
     R wr = w[cc] * r;
     if (mode == 0) acc_add(acc, node, wr, fix_bits(0));
     else

@@ -148,6 +148,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       bh[a] = b[3 + a];
       int cen = (bl[a] + bh[a]) >> 1;  // keep the box centre in [0, nc)
       int sh = (cen >= 0 ? cen / nc[a] : -((-cen + nc[a] - 1) / nc[a])) * nc[a];
+      if (a == 0 && g.ox) sh = 0;  // an open x has no periodic images
       bl[a] -= sh;
       bh[a] -= sh;
     }
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     if (tiled)
       for (int q = tid; q < Tn; q += PASS_WG) {
         int a = q % T[0], b = (q / T[0]) % T[1], c = q / (T[0] * T[1]);
-        int64_t node = gidx(g, node_of(bl[0] + a, g.ncx), node_of(bl[1] + b, g.ncy),
+        int64_t node = gidx(g, node_of_x(g, bl[0] + a), node_of(bl[1] + b, g.ncy),
                             node_of(bl[2] + c, g.ncz));
         gtile[2 * q] = make_float4(S[node], S[node + g.nn], S[node + 2 * g.nn],
                                    S[node + 3 * g.nn]);
@@ -236,8 +237,9 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
       R w8[8], q[M], vn[3] = {0, 0, 0};
       weights8(f0, f1, f2, w8);
-      int ux = unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
+      int ux = g.ox ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
           uz = unwrap(ck, cen[2], g.ncz);
+      bool alive = true;  // false once the particle left through an open x face
       const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
       if constexpr (MODE == 1) {
         q[0] = r;
@@ -280,6 +282,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
           ux += half_move(f0, vn[0], K.s[0]);
           uy += half_move(f1, vn[1], K.s[1]);
           uz += half_move(f2, vn[2], K.s[2]);
+          if (g.ox) alive = ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}: no deposit
           weights8(f0, f1, f2, w8);
           q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
         }
@@ -287,7 +290,8 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
       const bool inw = windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
                        ly < W[1] - 1 && lz < W[2] - 1;
-      if (inw) {
+      if (!alive) {
+      } else if (inw) {
         const int key = lx + W[0] * (ly + W[1] * lz);
         if (KP > 1 && key != cur) {
           if (cur >= 0) {
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         misses++;
 #pragma unroll
         for (int cc = 0; cc < 8; cc++) {
-          int64_t node = gidx(g, node_of(ux + (cc & 1), g.ncx),
+          int64_t node = gidx(g, node_of_x(g, ux + (cc & 1)),
                               node_of(uy + ((cc >> 2) & 1), g.ncy),
                               node_of(uz + ((cc >> 1) & 1), g.ncz));
 #pragma unroll
@@ -333,10 +337,14 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         uy += half_move(f1, vn[1], K.s[1]);
         uz += half_move(f2, vn[2], K.s[2]);
         P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
-        P.cell[p] = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
-        bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
-        bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
-        bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+        if (g.ox && (!alive || ux < 0 || ux >= g.ncx)) {
+          P.cell[p] = DEAD_CELL;  // the sort at the end of the step drops it
+        } else {
+          P.cell[p] = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
+          bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
+          bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
+          bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+        }
       }
     }
     // the first cell of a lane that moved on is its own
@@ -395,7 +403,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   if (windowed) {
     for (int qn = tid; qn < Wn; qn += PASS_WG) {
       int lx = qn % W[0], ly = (qn / W[0]) % W[1], lz = qn / (W[0] * W[1]);
-      int64_t node = gidx(g, node_of(u0[0] + lx, g.ncx), node_of(u0[1] + ly, g.ncy),
+      int64_t node = gidx(g, node_of_x(g, u0[0] + lx), node_of(u0[1] + ly, g.ncy),
                           node_of(u0[2] + lz, g.ncz));
 #pragma unroll
       for (int m = 0; m < M; m++) {

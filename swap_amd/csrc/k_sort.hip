@@ -70,10 +70,17 @@ __device__ __forceinline__ uint32_t tile_key(uint32_t c, int ntx, int nty) {
          TILE_X * ((uint32_t)(j % TILE_Y) + TILE_Y * (uint32_t)(k % TILE_Z));
 }
 
+// the key of a particle that left through an open x face is one past the
+// last tile key: it sorts behind every live particle
+__device__ __forceinline__ uint32_t sort_key(uint32_t c, int ntx, int nty, uint32_t dead_key) {
+  return c == DEAD_CELL ? dead_key : tile_key(c, ntx, nty);
+}
+
 // A: one workgroup per chunk.  Keys below 2^nbits; padding positions get
 // the largest nbits-bit key and, the sort being stable, end up last.
 __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restrict__ cell,
-                                              int ntx, int nty, int nbits, uint32_t* cnt,
+                                              int ntx, int nty, uint32_t dead_key, int nbits,
+                                              uint32_t* cnt,
                                               uint32_t* nrun, uint32_t* segkey,
                                               uint32_t* seginfo, uint32_t* nsegc,
                                               uint32_t* perm) {
@@ -91,12 +98,12 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
     uint4 a = src[0], b = src[1];
     uint32_t cs[SE] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int e = 0; e < SE; e++) key[e] = tile_key(cs[e], ntx, nty);
+    for (int e = 0; e < SE; e++) key[e] = sort_key(cs[e], ntx, nty, dead_key);
   } else {
 #pragma unroll
     for (int e = 0; e < SE; e++) {
       int l = t * SE + e;
-      key[e] = l < m ? tile_key(cell[c0 + l], ntx, nty) : padkey;
+      key[e] = l < m ? sort_key(cell[c0 + l], ntx, nty, dead_key) : padkey;
     }
   }
 #pragma unroll
@@ -378,42 +385,48 @@ int post(swap_ctx* c) {
   return 0;
 }
 
-// exclusive scan of n counts into out (n+1 entries: out[n] = total)
-int scan(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out) {
+// exclusive scan of n counts into out (n+1 entries: out[n] = total); bsum
+// holds n / 4096 + 1 block sums (the context's sblk: up to nkeys + 1 counts)
+int scan(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out, uint32_t* bsum = nullptr) {
+  if (!bsum) bsum = c->sblk;
   int64_t nb = (n + SCAN_T * SCAN_E - 1) / (SCAN_T * SCAN_E);
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_T), 0, c->stream, n, in, out,
-                     c->sblk);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, c->stream, nb, c->sblk, out + n);
-  hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, out, c->sblk);
+                     bsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, c->stream, nb, bsum, out + n);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, out, bsum);
   return post(c);
 }
 
 }  // namespace
 
-int so_scan_u32(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out) {
-  return scan(c, n, in, out);
+int so_scan_u32(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out, uint32_t* bsum) {
+  return scan(c, n, in, out, bsum);
 }
 
 int so_sort(swap_ctx* c) {
-  const int64_t n = c->n, nk = c->nkeys;
+  int64_t n = c->n;
+  const int64_t nk = c->nkeys;
+  // an open x run has one more key, nk, for the particles that left; they
+  // end up behind the live ones, which are the first sstart[nk]
+  const int64_t nkd = nk + (c->g.ox ? 1 : 0);
   const int ntx = c->ntile[0], nty = c->ntile[1];
   if (n > 0) {
     const int64_t nch = (n + SCH - 1) / SCH;
     int nbits = 0;
-    while (nbits < 32 && (1ull << nbits) < (unsigned long long)nk) nbits++;
-    HIPCHK(c, hipMemsetAsync(c->scnt, 0, sizeof(uint32_t) * nk, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->snrun, 0, sizeof(uint32_t) * nk, c->stream));
+    while (nbits < 32 && (1ull << nbits) < (unsigned long long)nkd) nbits++;
+    HIPCHK(c, hipMemsetAsync(c->scnt, 0, sizeof(uint32_t) * nkd, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->snrun, 0, sizeof(uint32_t) * nkd, c->stream));
     hipLaunchKernelGGL(k_sortA, dim3((unsigned)nch), dim3(ST), 0, c->stream, n, c->cell, ntx,
-                       nty, nbits, c->scnt, c->snrun, c->segkey, c->seginfo, c->nsegc,
-                       c->srbase);
+                       nty, (uint32_t)nk, nbits, c->scnt, c->snrun, c->segkey, c->seginfo,
+                       c->nsegc, c->srbase);
     if (int e = post(c)) return e;
-    if (int e = scan(c, nk, c->scnt, c->sstart)) return e;
-    if (int e = scan(c, nk, c->snrun, c->srstart)) return e;
-    HIPCHK(c, hipMemsetAsync(c->scur, 0, sizeof(uint32_t) * nk, c->stream));
+    if (int e = scan(c, nkd, c->scnt, c->sstart)) return e;
+    if (int e = scan(c, nkd, c->snrun, c->srstart)) return e;
+    HIPCHK(c, hipMemsetAsync(c->scur, 0, sizeof(uint32_t) * nkd, c->stream));
     hipLaunchKernelGGL(k_sortB, dim3((unsigned)nch), dim3(256), 0, c->stream, c->nsegc,
                        c->segkey, c->seginfo, c->srstart, c->scur, c->sruns);
     hipLaunchKernelGGL(k_sortC, dim3(8192), dim3(256), 0, c->stream, c->segkey, c->sstart,
-                       c->srstart, nk, c->sruns, c->segbase);
+                       c->srstart, nkd, c->sruns, c->segbase);
     Move Mv;
     Mv.cell = c->cell; Mv.cell2 = c->cell2;
     for (int a = 0; a < 9; a++) { Mv.p[a] = c->p[a]; Mv.p2[a] = c->p2[a]; }
@@ -431,6 +444,13 @@ int so_sort(swap_ctx* c) {
     std::swap(c->bp, c->bp2);
     std::swap(c->mr, c->mr2);
     std::swap(c->id, c->id2);
+    if (c->g.ox) {
+      uint32_t live = 0;
+      HIPCHK(c, hipMemcpyAsync(&live, c->sstart + nk, 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      c->escaped += n - (int64_t)live;
+      c->n = n = live;
+    }
   } else {
     HIPCHK(c, hipMemsetAsync(c->sstart, 0, sizeof(uint32_t) * (nk + 1), c->stream));
   }

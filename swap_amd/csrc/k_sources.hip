@@ -1,5 +1,7 @@
 // k_sources.hip -- particle loading, pickup-ion sources, FP64 diagnostics.
-#include "swap_internal.h"
+#include <algorithm>
+#include <cmath>
+#include "particle_math.h"
 
 namespace {
 
@@ -110,7 +112,7 @@ __global__ void k_maxwellian(int64_t n, int64_t off, PDev<R> P, uint32_t k0, uin
 template <typename R>
 __global__ void k_ionize_sw(int64_t n, int64_t off, PDev<R> P, uint32_t k0, uint32_t k1,
                             uint32_t stream, double vsw, int ncx, int ncy, int ncz, float beta_pu,
-                            float* bp, float* mr, uint32_t* id) {
+                            float* bp, float* mr, uint32_t* id, uint32_t id0) {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
   uint32_t r[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 7u};
@@ -133,7 +135,122 @@ __global__ void k_ionize_sw(int64_t n, int64_t off, PDev<R> P, uint32_t k0, uint
   P.v1x[q] = (R)v[0]; P.v1y[q] = (R)v[1]; P.v1z[q] = (R)v[2];
   if (bp) bp[q] = beta_pu;
   if (mr) mr[q] = 1.0f;
-  if (id) id[q] = (uint32_t)q;
+  if (id) id[q] = id0 + (uint32_t)p;
+}
+
+// Inflow through the open x face at cell coordinate 0 (build-defined; the
+// reference instead refills its boundary layer to the upstream density,
+// check_min_den_boundary, gutsp.f:575-816).  Part of swap_step, right after
+// the particle pass: the ions of an upstream drifting Maxwellian that cross
+// the face during the step.  Slot s of the slab (-L, 0] x (0, ncy] x (0, ncz]
+// (cell units, positions at the start of the step) draws a position and a
+// velocity as the loader does and makes the pass's two half moves: an ion
+// inside after the first deposits there with the pass's moments, one inside
+// after the second is appended.  Philox counter (slot, step, stream, draw);
+// explicit fma so that the host restatement rounds identically.
+struct InflowDev {
+  double vsw, vth, sh[3];  // sh: dt / 2 / cell size per axis
+  int L, nc[3];
+};
+struct InflowIon {
+  double th[3], te[3], v[3];  // cell coordinates after each half move
+  bool dep, keep;
+};
+__device__ __forceinline__ double wrap_open(double t, double nc) {
+  if (t > nc || t <= 0.0) {  // back into (0, nc]
+    t -= (ceil(t / nc) - 1.0) * nc;
+    if (t <= 0.0) t += nc;
+    else if (t > nc) t -= nc;
+  }
+  return t;
+}
+__device__ __forceinline__ InflowIon inflow_slot(const InflowDev& d, uint32_t slot, uint32_t step,
+                                                 uint32_t k0, uint32_t k1, uint32_t stream) {
+#pragma clang fp contract(off)
+  uint32_t r[3][4];
+  for (int q = 0; q < 3; q++) {
+    r[q][0] = slot;
+    r[q][1] = step;
+    r[q][2] = stream;
+    r[q][3] = (uint32_t)q;
+    philox4x32_10(r[q], k0, k1);
+  }
+  InflowIon o;
+  const uint32_t* ur[6] = {&r[0][3], &r[1][0], &r[1][1], &r[1][2], &r[1][3], &r[2][0]};
+  for (int a = 0; a < 3; a++) {
+    double u1 = 1.0 - u24(*ur[2 * a]);  // (0,1]
+    double u2 = u24(*ur[2 * a + 1]);
+    o.v[a] = (a == 0 ? d.vsw : 0.0) + d.vth * sqrt(-log(u1)) * cos(PI_REF * u2);
+  }
+  double t0[3];
+  t0[0] = (1.0 - u24(r[0][0])) * (double)d.L - (double)d.L;  // (-L, 0], exact
+  t0[1] = (1.0 - u24(r[0][1])) * (double)d.nc[1];
+  t0[2] = (1.0 - u24(r[0][2])) * (double)d.nc[2];
+  for (int a = 0; a < 3; a++) {
+    o.th[a] = fma(o.v[a], d.sh[a], t0[a]);
+    o.te[a] = fma(o.v[a], d.sh[a], o.th[a]);
+  }
+  for (int a = 1; a < 3; a++) {  // periodic y and z
+    o.th[a] = wrap_open(o.th[a], (double)d.nc[a]);
+    o.te[a] = wrap_open(o.te[a], (double)d.nc[a]);
+  }
+  const double ncx = (double)d.nc[0];
+  o.dep = o.th[0] > 0.0 && o.th[0] <= ncx;
+  o.keep = o.te[0] > 0.0 && o.te[0] <= ncx && o.th[0] <= ncx;
+  return o;
+}
+// right-closed split of a cell coordinate t in (0, nc]
+__device__ __forceinline__ int split_t(double t, double& f) {
+  double i = ceil(t) - 1.0;
+  f = t - i;
+  return (int)i;
+}
+
+__global__ void k_inflow_count(uint32_t nslot, InflowDev d, uint32_t step, uint32_t k0,
+                               uint32_t k1, uint32_t stream, uint32_t* cnt) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nslot) return;
+  cnt[q] = inflow_slot(d, q, step, k0, k1, stream).keep ? 1u : 0u;
+}
+
+template <typename R>
+__global__ void k_inflow_fill(Geo g, uint32_t nslot, int64_t n0, PDev<R> P, InflowDev d,
+                              uint32_t step, uint32_t k0, uint32_t k1, uint32_t stream,
+                              const uint32_t* off, float bpv, float* bp, float* mr, uint32_t* id,
+                              uint32_t id0, typename AccOf<R>::T* acc) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nslot) return;
+  const InflowIon o = inflow_slot(d, q, step, k0, k1, stream);
+  if (o.dep) {  // the pass's moments at x^{n+1/2}: 1/beta_p and v/beta_p
+    double f[3];
+    int ijk[3];
+    for (int a = 0; a < 3; a++) ijk[a] = split_t(o.th[a], f[a]);
+    R w[8];
+    pm::weights8((R)f[0], (R)f[1], (R)f[2], w);
+    const R rr = (R)1 / (R)bpv;
+    const R qm[4] = {rr, rr * (R)o.v[0], rr * (R)o.v[1], rr * (R)o.v[2]};
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      int64_t node = gidx(g, node_of_x(g, ijk[0] + (cc & 1)), node_of(ijk[1] + ((cc >> 2) & 1), g.ncy),
+                          node_of(ijk[2] + ((cc >> 1) & 1), g.ncz));
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+        acc_add(acc, node + (int64_t)m * g.nn, w[cc] * qm[m], fix_bits(m));
+    }
+  }
+  if (o.keep) {
+    double f[3];
+    int ijk[3];
+    for (int a = 0; a < 3; a++) ijk[a] = split_t(o.te[a], f[a]);
+    int64_t p = n0 + off[q];
+    P.cell[p] = pack_cell(ijk[0], ijk[1], ijk[2]);
+    P.fx[p] = (R)f[0]; P.fy[p] = (R)f[1]; P.fz[p] = (R)f[2];
+    P.vx[p] = (R)o.v[0]; P.vy[p] = (R)o.v[1]; P.vz[p] = (R)o.v[2];
+    P.v1x[p] = (R)o.v[0]; P.v1y[p] = (R)o.v[1]; P.v1z[p] = (R)o.v[2];
+    if (bp) bp[p] = bpv;
+    if (mr) mr[p] = 1.0f;
+    if (id) id[p] = id0 + off[q];
+  }
 }
 
 // ---- FP64 reductions ----
@@ -265,10 +382,11 @@ int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
     SDISPATCH(c, hipLaunchKernelGGL((k_ionize_sw<R>), dim3(nblk(m, 256)), dim3(256), 0, c->stream,
                                     m, c->n, pdev<R>(c), k0, k1, 0x20000u + (uint32_t)step, vsw,
                                     c->cfg.ncx, c->cfg.ncy, c->cfg.ncz, (float)beta_pu, c->bp,
-                                    c->mr, c->id));
+                                    c->mr, c->id, (uint32_t)c->next_id));
     if (int e = post(c)) return e;
   }
   c->n += m;
+  c->next_id += m;
   c->injected += m;
   *injected = m;
   return 0;
@@ -359,7 +477,8 @@ __global__ void k_cloud_count(Geo g, double dx, double dy, double dz, CloudDev c
 template <typename R>
 __global__ void k_cloud_fill(Geo g, int64_t n0, PDev<R> P, CloudDev cl, uint32_t k0, uint32_t k1,
                              uint32_t stream, const uint32_t* cnt, const uint32_t* off,
-                             const float* bpn, float* bp, float* mr, uint32_t* id) {
+                             const float* bpn, float* bp, float* mr, uint32_t* id,
+                             uint32_t id0) {
   int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t nint_nodes = (int64_t)g.ncx * g.ncy * g.ncz;
   if (q >= nint_nodes) return;
@@ -373,7 +492,9 @@ __global__ void k_cloud_fill(Geo g, int64_t n0, PDev<R> P, CloudDev cl, uint32_t
     double f[3];
     for (int a = 0; a < 3; a++) {
       double t = (double)s[a] + u24(r[a]) - 0.5;  // cell coordinate, in (0.5, nc + 0.5)
-      if (t > (double)nc[a]) t -= (double)nc[a];
+      // past the last face: the periodic image, or on an open x the mirror
+      // image inside the domain
+      if (t > (double)nc[a]) t = a == 0 && g.ox ? 2.0 * nc[a] - t : t - (double)nc[a];
       double ci = ceil(t) - 1.0;
       ijk[a] = (int)ci;
       f[a] = t - ci;
@@ -385,7 +506,7 @@ __global__ void k_cloud_fill(Geo g, int64_t n0, PDev<R> P, CloudDev cl, uint32_t
     P.v1x[p] = (R)0; P.v1y[p] = (R)0; P.v1z[p] = (R)0;
     bp[p] = bpn[q];
     mr[p] = cl.mrat;
-    if (id) id[p] = (uint32_t)p;
+    if (id) id[p] = id0 + off[q] + l;
   }
 }
 
@@ -474,10 +595,11 @@ int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* inject
     SDISPATCH(c, hipLaunchKernelGGL((k_cloud_fill<R>), dim3(nblk(nin, 256)), dim3(256), 0,
                                     c->stream, g, c->n, pdev<R>(c), d, k0, k1,
                                     0x30000u + (uint32_t)step, c->ccnt, c->coff, c->cbp, c->bp,
-                                    c->mr, c->id));
+                                    c->mr, c->id, (uint32_t)c->next_id));
     if (int e = post(c)) return e;
   }
   c->n += tot;
+  c->next_id += tot;
   c->injected += tot;
   *injected = tot;
   return 0;
@@ -531,7 +653,71 @@ int sr_vbounds(swap_ctx* c) {
   return post(c);
 }
 
-int sr_outflow_compact(swap_ctx* c) {
-  (void)c;
+int sr_set_inflow(swap_ctx* c, double ppc, double vsw, double vth, double beta_p) {
+  if (!c->g.ox) return set_err(c, SWAP_EINVAL, "set_inflow needs SWAP_OUTFLOW_X");
+  if (!(ppc >= 0) || !(vth >= 0) || !std::isfinite(vsw) || !(beta_p > 0))
+    return set_err(c, SWAP_EINVAL, "set_inflow: bad parameters");
+  if (beta_p != 1.0 && !(c->cfg.flags & SWAP_SPECIES))
+    return set_err(c, SWAP_EINVAL, "set_inflow: beta_p != 1 needs SWAP_SPECIES");
+  // the sampler's speeds along x lie within vsw -+ vth sqrt(ln 2^24) = 4.08 vth
+  double reach = (fabs(vsw) + 4.08 * vth) * c->cfg.dt / c->cfg.dx;
+  if (!(reach < 1048576.0)) return set_err(c, SWAP_EINVAL, "set_inflow: slab too deep");
+  int L = std::max(1, (int)ceil(reach));
+  double ns = std::round(ppc * (double)L * (double)c->cfg.ncy * (double)c->cfg.ncz);
+  if (!(ns < 4.0e9)) return set_err(c, SWAP_EINVAL, "set_inflow: too many slots");
+  c->in_ppc = ppc;
+  c->in_vsw = vsw;
+  c->in_vth = vth;
+  c->in_bp = beta_p;
+  c->in_L = L;
+  c->in_slots = (int64_t)ns;
+  return 0;
+}
+
+int sr_inflow_step(swap_ctx* c) {
+  c->in_last = 0;
+  const uint32_t nslot = (uint32_t)c->in_slots;
+  if (!c->g.ox || nslot == 0) return 0;
+  InflowDev d;
+  d.vsw = c->in_vsw;
+  d.vth = c->in_vth;
+  d.sh[0] = 0.5 * c->cfg.dt / c->cfg.dx;
+  d.sh[1] = 0.5 * c->cfg.dt / c->cfg.dy;
+  d.sh[2] = 0.5 * c->cfg.dt / c->cfg.dz;
+  d.L = c->in_L;
+  d.nc[0] = c->cfg.ncx; d.nc[1] = c->cfg.ncy; d.nc[2] = c->cfg.ncz;
+  if ((int64_t)nslot > c->islots) {
+    for (uint32_t** b : {&c->icnt, &c->ioff, &c->iblk}) {
+      if (*b) HIPCHK(c, hipFree(*b));
+      *b = nullptr;
+    }
+    c->islots = 0;
+    HIPCHK(c, hipMalloc(&c->icnt, 4 * ((size_t)nslot + 1)));
+    HIPCHK(c, hipMalloc(&c->ioff, 4 * ((size_t)nslot + 1)));
+    HIPCHK(c, hipMalloc(&c->iblk, 4 * ((size_t)nslot / 4096 + 4)));
+    c->islots = nslot;
+  }
+  const uint32_t k0 = (uint32_t)c->cfg.seed, k1 = (uint32_t)(c->cfg.seed >> 32);
+  const uint32_t stream = 0x50000u, step = (uint32_t)c->step_count;
+  hipLaunchKernelGGL(k_inflow_count, dim3(nblk(nslot, 256)), dim3(256), 0, c->stream, nslot, d,
+                     step, k0, k1, stream, c->icnt);
+  if (int e = post(c)) return e;
+  if (int e = so_scan_u32(c, nslot, c->icnt, c->ioff, c->iblk)) return e;
+  uint32_t tot = 0;
+  HIPCHK(c, hipMemcpyAsync(&tot, c->ioff + nslot, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->n + (int64_t)tot > c->cap)
+    return set_err(c, SWAP_ECAP, "inflow: %lld + %u ions exceed capacity %lld",
+                   (long long)c->n, tot, (long long)c->cap);
+  SDISPATCH(c, hipLaunchKernelGGL((k_inflow_fill<R>), dim3(nblk(nslot, 256)), dim3(256), 0,
+                                  c->stream, c->g, nslot, c->n, pdev<R>(c), d, step, k0, k1,
+                                  stream, c->ioff, (float)c->in_bp, c->bp, c->mr, c->id,
+                                  (uint32_t)c->next_id, (typename AccOf<R>::T*)c->macc));
+  if (int e = post(c)) return e;
+  c->n += tot;
+  c->next_id += tot;
+  c->injected += tot;
+  c->in_last = tot;
+  c->vb_valid = false;  // the new velocities may lie outside the pass windows' bounds
   return 0;
 }

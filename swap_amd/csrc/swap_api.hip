@@ -202,8 +202,9 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
     delete c;
     return SWAP_EINVAL;
   }
-  if (cfg->flags & SWAP_OUTFLOW_X) {
-    fprintf(stderr, "swap_create: inflow/outflow x is not available in this build\n");
+  if ((cfg->flags & SWAP_OUTFLOW_X) && (cfg->ncx < 4 || cfg->sort_every < 1)) {
+    fprintf(stderr, "swap_create: an open x needs ncx >= 4 and sort_every >= 1 "
+                    "(the sort removes escaped ions)\n");
     delete c;
     return SWAP_EINVAL;
   }
@@ -218,6 +219,7 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   g.ncx = cfg->ncx; g.ncy = cfg->ncy; g.ncz = cfg->ncz;
   g.nx = g.ncx + 2; g.ny = g.ncy + 2; g.nz = g.ncz + 2;
   g.nn = (int64_t)g.nx * g.ny * g.nz;
+  g.ox = (cfg->flags & SWAP_OUTFLOW_X) ? 1 : 0;
   c->qn = 1.0 / (cfg->dx * cfg->dy * cfg->dz * cfg->beta);
   c->dtsub = cfg->dt / cfg->ntsub;
   c->ntf = cfg->ntsub;
@@ -279,11 +281,12 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   DMF(c->seginfo, 4 * cp);
   DMF(c->segbase, 4 * cp);
   DMF(c->nsegc, 4 * (cp / 4096 + 2));
-  DMF(c->scnt, 4 * (size_t)(c->nkeys + 1));
-  DMF(c->sstart, 4 * (size_t)(c->nkeys + 1));
-  DMF(c->scur, 4 * (size_t)(c->nkeys + 1));
-  DMF(c->snrun, 4 * (size_t)(c->nkeys + 1));
-  DMF(c->srstart, 4 * (size_t)(c->nkeys + 1));
+  // counts and starts per key, plus the key of escaped particles and the total
+  DMF(c->scnt, 4 * (size_t)(c->nkeys + 2));
+  DMF(c->sstart, 4 * (size_t)(c->nkeys + 2));
+  DMF(c->scur, 4 * (size_t)(c->nkeys + 2));
+  DMF(c->snrun, 4 * (size_t)(c->nkeys + 2));
+  DMF(c->srstart, 4 * (size_t)(c->nkeys + 2));
   DMF(c->vb, 16 * sizeof(int));
   c->vb_next = c->vb + 8;
   DMF(c->d_zero, 4 * sizeof(int));
@@ -310,7 +313,7 @@ int swap_destroy(swap_ctx* c) {
                 c->bp2, c->mr, c->mr2, c->id, c->id2, c->Ep, c->vpl, c->vmi, c->wght, c->ijkp,
                 c->sruns, c->srbase, c->segkey, c->seginfo, c->segbase, c->nsegc, c->scnt,
                 c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items, c->ccnt,
-                c->coff, c->cbp,
+                c->coff, c->cbp, c->icnt, c->ioff, c->iblk,
                 c->boxes, c->boxes_next, c->vb, c->d_zero};
   for (void* p : ps) dfree(p);
   for (int f = 0; f < SWAP_F_COUNT; f++) dfree(c->fld[f]);
@@ -365,6 +368,7 @@ int swap_load_particles(swap_ctx* c, int64_t n, const double* xp, const double* 
     HIPCHK(c, hipMemcpyAsync((char*)dtmp + 2 * b, vp1, b, hipMemcpyHostToDevice, c->stream));
   }
   c->n = n;
+  c->next_id = n;
   int e = pl_load_abs(c, n, dtmp, (double*)((char*)dtmp + b), (double*)((char*)dtmp + 2 * b));
   if (!e && (c->cfg.flags & SWAP_SPECIES)) {
     std::vector<float> t(std::max<int64_t>(n, 1));
@@ -402,6 +406,7 @@ int swap_load_particles_split(swap_ctx* c, int64_t n, const uint32_t* cell, cons
                      (long long)q);
   }
   c->n = n;
+  c->next_id = n;
   if (n > 0) {
     HIPCHK(c, hipMemcpyAsync(c->cell, cell, 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
     for (int a = 0; a < 3; a++) {
@@ -441,6 +446,7 @@ int swap_load_maxwellian(swap_ctx* c, int64_t n, double vsw, double vth, double 
   if ((beta_p != 1.0 || mrat != 1.0) && !(c->cfg.flags & SWAP_SPECIES))
     return set_err(c, SWAP_EINVAL, "load: beta_p/mrat != 1 need SWAP_SPECIES");
   c->n = n;
+  c->next_id = n;
   TRY(pl_maxwellian_full(c, 0, n, vsw, vth, (float)beta_p, (float)mrat, 1u));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_sorted = -1;
@@ -569,6 +575,12 @@ int swap_set_particle_array(swap_ctx* c, int which, const void* host) {
     case SWAP_P_ID:
       if (!c->id) return set_err(c, SWAP_ESTATE, "ids are kept only with SWAP_TRACK_IDS");
       HIPCHK(c, hipMemcpyAsync(c->id, host, 4 * n, hipMemcpyHostToDevice, c->stream));
+      {  // appended ions continue after the largest id given
+        const uint32_t* h = (const uint32_t*)host;
+        int64_t next = 0;
+        for (size_t q = 0; q < n; q++) next = std::max<int64_t>(next, (int64_t)h[q] + 1);
+        c->next_id = next;
+      }
       break;
     default:
       return set_err(c, SWAP_EINVAL, "unknown particle array %d", which);
@@ -686,6 +698,8 @@ int swap_get_vp_final(swap_ctx* c) {
 
 int swap_move_ion_half(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
+  if (c->g.ox)
+    return set_err(c, SWAP_EINVAL, "move_ion_half wraps x; an open x moves only in swap_step");
   c->n_sorted = -1;  // boxes of the work items no longer describe the particles
   c->mgen++;
   return pl_move_half(c);
@@ -783,7 +797,8 @@ int swap_face_to_center(swap_ctx* c, int in, int out) {
 
 int swap_field_subcycles(swap_ctx* c, int ntf, double dtsub) {
   if (!c || ntf < 0 || !(dtsub > 0)) return SWAP_EINVAL;
-  const bool fused = !(c->cfg.flags & SWAP_UNFUSED_FIELDS);
+  // the fused kernels wrap every axis; an open x runs the per-routine kernels
+  const bool fused = !(c->cfg.flags & SWAP_UNFUSED_FIELDS) && !c->g.ox;
   for (int q = 0; q < ntf; q++)
     TRY(fused ? subcycle_fused(c, dtsub, q == ntf - 1) : subcycle_impl(c, dtsub));
   return 0;
@@ -856,6 +871,7 @@ int swap_step(swap_ctx* c, int nsteps) {
     {
       Stage t(c, SWAP_K_P3);
       TRY(pl_pass(c, 3));
+      TRY(sr_inflow_step(c));  // an open x: the ions entering during this step
     }
     {
       Stage t(c, SWAP_K_GLUE);
@@ -865,7 +881,7 @@ int swap_step(swap_ctx* c, int nsteps) {
     }
     // P3 reduced the cell boxes of x^{n+1} and the bounds of v^{n+1/2}
     std::swap(c->vb, c->vb_next);
-    c->vb_valid = c->n > 0;
+    c->vb_valid = c->n > 0 && c->in_last == 0;  // inflow ions were not in P3's bounds
     if (c->n_sorted >= 0) {
       std::swap(c->boxes, c->boxes_next);
       c->steps_since_sort++;
@@ -878,6 +894,11 @@ int swap_step(swap_ctx* c, int nsteps) {
       Stage t(c, SWAP_K_FIELD);
       TRY(swap_field_subcycles(c, ntf, dtsub));
     }
+    // an open x: the sort drops the ions that P3 marked as gone
+    if (c->g.ox) {
+      Stage t(c, SWAP_K_SORT);
+      TRY(swap_sort(c));
+    }
     c->step_count++;
     c->step_gen = c->mgen;  // macc: the final deposit of this step
   }
@@ -889,6 +910,11 @@ int swap_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_p
   if (!c || !injected) return SWAP_EINVAL;
   c->mgen++;
   return sr_ionize_sw(c, step, ndot, vsw, beta_pu, injected);
+}
+
+int swap_set_inflow(swap_ctx* c, double ppc, double vsw, double vth, double beta_p) {
+  if (!c) return SWAP_EINVAL;
+  return sr_set_inflow(c, ppc, vsw, vth, beta_p);
 }
 
 int swap_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cloud, int64_t* injected,

@@ -17,6 +17,7 @@ struct Geo {
   int nx, ny, nz;     // storage extents including ghosts
   int ncx, ncy, ncz;  // physical cells = nx-2 ...
   int64_t nn;         // nx*ny*nz
+  int ox;             // x inflow/outflow instead of periodic
 };
 
 SWAP_HD int64_t gidx(const Geo& g, int i, int j, int k) {
@@ -32,6 +33,27 @@ SWAP_HD int node_of(int u, int nc) {
   if (r < 0) r += nc;
   return r + 1;
 }
+
+// The x axis of an inflow/outflow run (Geo::ox) has no periodic image: its
+// ghost layers are zero-gradient copies of the nearest interior node, and
+// deposits on the inflow face node (storage 0) fold onto the first interior
+// node.  canon_x/node_of_x are canon/node_of for x under that rule.
+SWAP_HD int canon_x(const Geo& g, int s) {
+  return g.ox ? (s < 1 ? 1 : (s > g.nx - 2 ? g.nx - 2 : s)) : canon(s, g.nx);
+}
+SWAP_HD int node_of_x(const Geo& g, int u) {
+  return g.ox ? (u < 1 ? 1 : (u > g.ncx ? g.ncx : u)) : node_of(u, g.ncx);
+}
+// Density of an open x node: the particle domain is cells 0..ncx-1; node 1
+// collects the whole of cell 0 (through the fold above) and half of cell 1,
+// node ncx half of cell ncx-1.  Dividing their deposits by those shares (1.5
+// and 0.5 cells) keeps a uniform plasma uniform up to both faces.
+SWAP_HD double face_scale(const Geo& g, int s) {
+  return !g.ox ? 1.0 : (s == 1 ? 2.0 / 3.0 : (s == g.ncx ? 2.0 : 1.0));
+}
+// cell of a particle that left through an open x face; the sort that
+// follows removes it
+constexpr uint32_t DEAD_CELL = 0xFFFFFFFFu;
 
 // packed cell: i | j<<11 | k<<21 (0-based cell indices)
 SWAP_HD uint32_t pack_cell(int i, int j, int k) {

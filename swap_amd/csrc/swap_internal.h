@@ -91,6 +91,15 @@ struct swap_ctx {
   // counters
   unsigned long long* dcnt = nullptr;  // [0] window misses [1] escapes [2] scratch [3] chex
   int64_t injected = 0, escaped = 0;
+  int64_t next_id = 0;  // id of the next appended ion (ids stay unique when ions leave)
+  // inflow through an open x face (swap_set_inflow): upstream ions per cell,
+  // drift, thermal speed, beta_p, slab depth (cells), slots per step, the
+  // last step's count and the slot buffers
+  double in_ppc = 0, in_vsw = 0, in_vth = 0, in_bp = 1;
+  int in_L = 0;
+  int64_t in_slots = 0, in_last = 0;
+  uint32_t *icnt = nullptr, *ioff = nullptr, *iblk = nullptr;
+  int64_t islots = 0;
   double* dred = nullptr;     // FP64 reduction scratch
   float* dmin = nullptr;      // check_time_step minimum
   void* scratch = nullptr;    // general device scratch
@@ -157,8 +166,10 @@ int pl_box_reset(swap_ctx* c);
 
 // ---- sort (k_sort.hip) ----
 int so_sort(swap_ctx* c);
-// exclusive scan of n uint32 counts (n <= nkeys) into out[0..n], out[n] = total
-int so_scan_u32(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out);
+// exclusive scan of n uint32 counts into out[0..n], out[n] = total; bsum: n / 4096 + 1
+// block sums, the context's own (n <= nkeys + 1) when null
+int so_scan_u32(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out,
+                uint32_t* bsum = nullptr);
 
 // ---- sources / diagnostics (k_sources.hip) ----
 int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
@@ -166,7 +177,8 @@ int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
 int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* injected,
                     int64_t* chex);
 int sr_diagnostics(swap_ctx* c, swap_diag* d);
-int sr_outflow_compact(swap_ctx* c);
+int sr_set_inflow(swap_ctx* c, double ppc, double vsw, double vth, double beta_p);
+int sr_inflow_step(swap_ctx* c);  // after the final particle pass of swap_step
 int sr_vbounds(swap_ctx* c);  // signed per-axis velocity bounds into c->vb
 
 // ---- denoiser (k_denoise.hip) ----

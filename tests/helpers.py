@@ -36,11 +36,11 @@ class Pair:
     """A GPU context and an FP64 oracle on the same grid and the same load."""
 
     def __init__(self, ncx=16, ncy=8, ncz=8, ppc=8, precision=32, sort_every=2, flags=4,
-                 seed=20261203, width=64, load=True):
+                 seed=20261203, width=64, load=True, capacity=None, vsw=400.0):
         from swap_amd import Swap  # noqa: F401
-        self.p = p = physics.workload(ncx, ncy, ncz, ppc)
-        self.s = physics.make_swap(p, precision=precision, sort_every=sort_every, flags=flags,
-                                   seed=seed)
+        self.p = p = physics.workload(ncx, ncy, ncz, ppc, vsw=vsw)
+        self.s = physics.make_swap(p, capacity=capacity, precision=precision,
+                                   sort_every=sort_every, flags=flags, seed=seed)
         self.o = O.Oracle(ncx + 2, ncy + 2, ncz + 2, p["dx"], p["dx"], p["dx"], p["dt"],
                           p["ntsub"], p["alpha"], p["beta"], width=width)
         self.n = p["npart"]
