@@ -31,6 +31,7 @@
 // record per node), filled with coalesced loads when the item starts; on
 // later steps the boxes have drifted apart and the gather reads S through
 // the caches.
+#include <type_traits>
 #include "particle_math.h"
 
 using namespace pm;
@@ -107,10 +108,18 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// node of unwrapped x node coordinate u: periodic, or on an open x (OX) the
+// clamp of node_of_x.  A template flag, so that the periodic kernels carry
+// none of the open-x code (their registers set their occupancy).
+template <bool OX>
+__device__ __forceinline__ int xnode(const Geo& g, int u) {
+  return OX ? (u < 1 ? 1 : (u > g.ncx ? g.ncx : u)) : node_of(u, g.ncx);
+}
+
 // KP particles per lane: PASS_KP right after a sort (cells in runs), 1
 // otherwise (the cells of neighbouring particles differ; the scan across the
-// wavefront then merges neighbours that still share a cell)
-template <typename R, int MODE, int KP>
+// wavefront then merges neighbours that still share a cell).  OX: open x.
+template <typename R, int MODE, int KP, bool OX>
 __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
                                                   const R* __restrict__ S,
                                                   typename AccOf<R>::T* __restrict__ gacc,
@@ -148,7 +157,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       bh[a] = b[3 + a];
       int cen = (bl[a] + bh[a]) >> 1;  // keep the box centre in [0, nc)
       int sh = (cen >= 0 ? cen / nc[a] : -((-cen + nc[a] - 1) / nc[a])) * nc[a];
-      if (a == 0 && g.ox) sh = 0;  // an open x has no periodic images
+      if (a == 0 && OX) sh = 0;  // an open x has no periodic images
       bl[a] -= sh;
       bh[a] -= sh;
     }
@@ -196,7 +205,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     if (tiled)
       for (int q = tid; q < Tn; q += PASS_WG) {
         int a = q % T[0], b = (q / T[0]) % T[1], c = q / (T[0] * T[1]);
-        int64_t node = gidx(g, node_of_x(g, bl[0] + a), node_of(bl[1] + b, g.ncy),
+        int64_t node = gidx(g, xnode<OX>(g, bl[0] + a), node_of(bl[1] + b, g.ncy),
                             node_of(bl[2] + c, g.ncz));
         gtile[2 * q] = make_float4(S[node], S[node + g.nn], S[node + 2 * g.nn],
                                    S[node + 3 * g.nn]);
@@ -237,7 +246,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
       R w8[8], q[M], vn[3] = {0, 0, 0};
       weights8(f0, f1, f2, w8);
-      int ux = g.ox ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
+      int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
           uz = unwrap(ck, cen[2], g.ncz);
       bool alive = true;  // false once the particle left through an open x face
       const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
@@ -282,7 +291,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
           ux += half_move(f0, vn[0], K.s[0]);
           uy += half_move(f1, vn[1], K.s[1]);
           uz += half_move(f2, vn[2], K.s[2]);
-          if (g.ox) alive = ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}: no deposit
+          if (OX) alive = ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}: no deposit
           weights8(f0, f1, f2, w8);
           q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
         }
@@ -324,7 +333,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         misses++;
 #pragma unroll
         for (int cc = 0; cc < 8; cc++) {
-          int64_t node = gidx(g, node_of_x(g, ux + (cc & 1)),
+          int64_t node = gidx(g, xnode<OX>(g, ux + (cc & 1)),
                               node_of(uy + ((cc >> 2) & 1), g.ncy),
                               node_of(uz + ((cc >> 1) & 1), g.ncz));
 #pragma unroll
@@ -337,7 +346,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         uy += half_move(f1, vn[1], K.s[1]);
         uz += half_move(f2, vn[2], K.s[2]);
         P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
-        if (g.ox && (!alive || ux < 0 || ux >= g.ncx)) {
+        if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
           P.cell[p] = DEAD_CELL;  // the sort at the end of the step drops it
         } else {
           P.cell[p] = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
@@ -403,7 +412,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   if (windowed) {
     for (int qn = tid; qn < Wn; qn += PASS_WG) {
       int lx = qn % W[0], ly = (qn / W[0]) % W[1], lz = qn / (W[0] * W[1]);
-      int64_t node = gidx(g, node_of_x(g, u0[0] + lx), node_of(u0[1] + ly, g.ncy),
+      int64_t node = gidx(g, xnode<OX>(g, u0[0] + lx), node_of(u0[1] + ly, g.ncy),
                           node_of(u0[2] + lz, g.ncz));
 #pragma unroll
       for (int m = 0; m < M; m++) {
@@ -444,8 +453,9 @@ int pl_pass(swap_ctx* c, int mode) {
   if (nb == 0) return 0;
   dim3 grid((unsigned)nb), blk(PASS_WG);
   const int* dn = c->n_sorted < 0 ? c->d_zero : c->d_nitems;
-  auto launch = [&](auto R0) {
+  auto launch = [&](auto R0, auto OX0) {
     using R = decltype(R0);
+    constexpr bool OX = decltype(OX0)::value;
     PDev<R> P = pdev<R>(c);
     PCoef<R> K = pcoef<R>(c);
     const R* S = (const R*)c->S;
@@ -453,31 +463,33 @@ int pl_pass(swap_ctx* c, int mode) {
     A* acc = (A*)c->macc;
     const bool runs = c->n_sorted >= 0 && c->steps_since_sort == 0;
     if (mode == 1 && runs)
-      hipLaunchKernelGGL((k_pass<R, 1, PASS_KP>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
+      hipLaunchKernelGGL((k_pass<R, 1, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
                          c->dcnt);
     else if (mode == 1)
-      hipLaunchKernelGGL((k_pass<R, 1, 1>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
+      hipLaunchKernelGGL((k_pass<R, 1, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
                          c->dcnt);
     else if (mode == 2 && runs)
-      hipLaunchKernelGGL((k_pass<R, 2, PASS_KP>), grid, blk, 0, c->stream, c->g, P, K, S,
+      hipLaunchKernelGGL((k_pass<R, 2, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S,
                          acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
                          c->vb_next, c->dcnt);
     else if (mode == 2)
-      hipLaunchKernelGGL((k_pass<R, 2, 1>), grid, blk, 0, c->stream, c->g, P, K, S,
+      hipLaunchKernelGGL((k_pass<R, 2, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S,
                          acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
                          c->vb_next, c->dcnt);
     else if (runs)
-      hipLaunchKernelGGL((k_pass<R, 3, PASS_KP>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
+      hipLaunchKernelGGL((k_pass<R, 3, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
                          c->dcnt);
     else
-      hipLaunchKernelGGL((k_pass<R, 3, 1>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
+      hipLaunchKernelGGL((k_pass<R, 3, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
                          c->dcnt);
   };
-  if (c->prec == 64) launch(0.0);
-  else launch(0.0f);
+  using Open = std::true_type;
+  using Periodic = std::false_type;
+  if (c->prec == 64) c->g.ox ? launch(0.0, Open()) : launch(0.0, Periodic());
+  else c->g.ox ? launch(0.0f, Open()) : launch(0.0f, Periodic());
   return post(c);
 }
