@@ -251,6 +251,27 @@ int swap_set_inflow(swap_ctx *ctx, double ppc, double vsw, double vth, double be
 /* ---- diagnostics (FP64 reductions; Energy_diag part_init.f:14-139) ---- */
 int swap_diagnostics(swap_ctx *ctx, swap_diag *out);
 
+/* Energy_diag (part_init.f:14-139), the record maind.f:503-504 writes to
+   c.energy.dat each step: energies in J with the reference's constants
+   (q = 1.6e-19, mu0 = 3.14159 * 4e-7, epsilon = 8.85e-12, inputs.f:19,50-57),
+   m_q = mion/q, cell volume dx dy dz.  Sums run over the unique nodes (the
+   reference also sums its i = 1 ghost layer and a single y plane of its
+   quasi-1-D grid) and weight every ion by 1/(beta beta_p) (the reference
+   multiplies by beta_p there, part_init.f:69-70).  Evp: ions' kinetic
+   energy (mass mion/mrat); Euf: 1/2 mion np |up|^2; EB1x/y/z: 1/(2 mu0)
+   (m_q b1_c)^2; EB1: the same of b1 - b0 over all components; EE:
+   epsilon/2 (m_q E km_to_m)^2; input_E: the kinetic energy of every ion
+   loaded or added (sources, inflow) less that of ions that left through an
+   open x (the reference's input_E, global.f:47); total_E = Evp + EE + EB1
+   and norm_E = total_E / input_E (maind.f's "Normalized energy"). */
+typedef struct {
+  double input_E, Evp, Euf, EB1, EB1x, EB1y, EB1z, EE, total_E, norm_E;
+} swap_energy;
+int swap_energy_diag(swap_ctx *ctx, double mion, const double b0[3], swap_energy *out);
+/* restore the input-energy account (J, for ions of mass mion) after a
+   resume; loads reset it to the loaded ions' kinetic energy */
+int swap_set_input_energy(swap_ctx *ctx, double mion, double input_E);
+
 /* ---- learned moment denoiser (build-only; no reference counterpart) ----
    A 3-D convolutional net, 4 -> 32 -> 32 -> 32 -> 4 channels, 3x3x3 kernels
    with periodic padding, leaky ReLU between layers and a residual output, that maps

@@ -65,6 +65,8 @@ def _load(width):
         "or_move_ion_half": ([vp, i32, i32, vp, vp], None),
         "or_step": ([vp, vp], None),
         "or_set_open_x": ([vp, i32], None),
+        "or_energy_diag": ([vp, vp, dbl, vp, vp], None),
+        "or_ke": ([vp, vp, i32, i32], dbl),
         "or_inflow": ([C.c_uint64, C.c_uint32, vp, dbl, vp, dbl, dbl, dbl, vp, vp, vp, vp,
                        i64], i64),
         "or_set_inflow": ([vp, C.c_uint64, dbl, dbl, dbl, dbl, C.c_uint32], None),
@@ -89,12 +91,16 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
+ENERGY_KEYS = ("input_E", "Evp", "Euf", "EB1", "EB1x", "EB1y", "EB1z", "EE", "total_E",
+               "norm_E")
+
+
 class OrState(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "b1", "b12", "b1p2", "bt", "btc", "np", "up", "aj", "E", "nu",
         "xp", "vp", "vp1", "vplus", "vminus", "Ep", "wght", "beta_p", "mrat",
         "ijkp", "id")] + [("Ni_tot", C.c_int), ("Ni_max", C.c_int), ("escaped", C.c_int),
-                          ("injected", C.c_int)]
+                          ("injected", C.c_int), ("input_k", C.c_double)]
 
 
 class Oracle:
@@ -165,16 +171,35 @@ class Oracle:
 
     def step(self, st):
         """st: dict of numpy arrays (see OrState); advances one step in place."""
-        s = OrState()
-        for n, _ in OrState._fields_[:-4]:
-            setattr(s, n, _p(st.get(n)))
-        s.Ni_tot = st["Ni_tot"]
-        s.Ni_max = st["xp"].shape[1]
+        s = self._state(st)
         self.lib.or_step(self.h, C.byref(s))
         # an open x removes escaped ions: the live ones are the first Ni_tot
         st["Ni_tot"] = s.Ni_tot
         st["escaped"] = s.escaped
         st["injected"] = s.injected
+        st["input_k"] = s.input_k
+
+    def _state(self, st):
+        s = OrState()
+        for n, _ in OrState._fields_[:-5]:
+            setattr(s, n, _p(st.get(n)))
+        s.Ni_tot = st["Ni_tot"]
+        s.Ni_max = st["xp"].shape[1]
+        s.input_k = st.get("input_k", 0.0)
+        return s
+
+    def ke(self, st, l0=1, l1=None):
+        """kinetic energy / ion mass of ions l0 <= l < l1 (1-based)"""
+        s = self._state(st)
+        return self.lib.or_ke(self.h, C.byref(s), l0, st["Ni_tot"] + 1 if l1 is None else l1)
+
+    def energy_diag(self, st, mion, b0):
+        """Energy_diag's record as a dict (J)"""
+        s = self._state(st)
+        out = np.zeros(10)
+        b0 = np.asarray(b0, np.float64)
+        self.lib.or_energy_diag(self.h, C.byref(s), mion, b0.ctypes.data, out.ctypes.data)
+        return dict(zip(ENERGY_KEYS, out.tolist()))
 
     def set_open_x(self, on=True):
         """build-defined inflow/outflow x instead of periodic"""

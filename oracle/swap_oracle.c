@@ -725,7 +725,10 @@ int or_remove_escaped(const orc *h, or_state *s) {
   int w = 0;
   for (int l = 0; l < N; l++) {
     real x = s->xp[l];
-    if (x <= q1 || x > qn) continue;
+    if (x <= q1 || x > qn) {
+      s->input_k -= or_ke(h, s, l + 1, l + 2);
+      continue;
+    }
     for (int m = 0; m < 3; m++) {
       s->xp[(size_t)m * M + w] = s->xp[(size_t)m * M + l];
       s->vp[(size_t)m * M + w] = s->vp[(size_t)m * M + l];
@@ -1093,4 +1096,55 @@ static void inflow_append(const orc *h, or_state *s, const double *t, const doub
   if (s->mrat) s->mrat[l] = R(1.0);
   if (s->id) s->id[l] = id;
   s->Ni_tot = l + 1;
+  s->input_k += or_ke(h, s, l + 1, l + 2);
+}
+
+/* kinetic energy without the ion mass: sum 1/2 |v km_to_m|^2 / (mrat beta
+   beta_p) over ions l0 <= l < l1 (part_init.f:66-71 with the weight
+   1/(beta beta_p) throughout) */
+double or_ke(const orc *h, const or_state *s, int l0, int l1) {
+  double e = 0;
+  const int M = s->Ni_max;
+  for (int l = l0; l < l1; l++) {
+    double bp = s->beta_p ? (double)s->beta_p[l - 1] : 1.0;
+    double mr = s->mrat ? (double)s->mrat[l - 1] : 1.0;
+    double v2 = 0;
+    for (int m = 1; m <= 3; m++) {
+      double v = (double)s->vp[(size_t)(m - 1) * M + (l - 1)] * 1e3;
+      v2 += v * v;
+    }
+    e += 0.5 * v2 / (mr * (double)h->beta * bp);
+  }
+  return e;
+}
+
+/* Energy_diag, part_init.f:14-139: the grid loop of :55-72 over the unique
+   nodes, vol = dx_cell dy_cell dz_cell km_to_m^3, m_q = mion/q; constants of
+   inputs.f:19,50-57 */
+void or_energy_diag(const orc *h, const or_state *s, double mion, const double b0[3],
+                    double out[10]) {
+  const double q = 1.6e-19, pi = 3.14159, mu0 = pi * 4.0e-7, eps = 8.85e-12, km = 1e3;
+  const double m_q = mion / q;
+  double Euf = 0, EB1 = 0, EBc[3] = {0, 0, 0}, EE = 0;
+  for (int k = 2; k <= h->nz - 1; k++)
+    for (int j = 2; j <= h->ny - 1; j++)
+      for (int i = 2; i <= h->nx - 1; i++) {
+        double vol = (double)h->dx_cell[i - 1] * (double)h->dy_cell[j - 1] *
+                     (double)h->dz_cell[k - 1] * km * km * km;
+        double denf = (double)s->np[F3(i, j, k)] / (km * km * km);
+        for (int m = 1; m <= 3; m++) {
+          double b = (double)s->b1[F4(i, j, k, m)];
+          double e = (double)s->E[F4(i, j, k, m)];
+          double u = (double)s->up[F4(i, j, k, m)];
+          EBc[m - 1] += vol / (2.0 * mu0) * (m_q * b) * (m_q * b);
+          EB1 += vol / (2.0 * mu0) * (m_q * (b - b0[m - 1])) * (m_q * (b - b0[m - 1]));
+          EE += eps * vol / 2.0 * (m_q * e * km) * (m_q * e * km);
+          Euf += 0.5 * mion * denf * vol * (u * km) * (u * km);
+        }
+      }
+  double Evp = mion * or_ke(h, s, 1, s->Ni_tot + 1);
+  double input = mion * s->input_k, total = Evp + EE + EB1;
+  double o[10] = {input, Evp, Euf, EB1, EBc[0], EBc[1], EBc[2], EE, total,
+                  input != 0 ? total / input : 0.0};
+  memcpy(out, o, sizeof(o));
 }

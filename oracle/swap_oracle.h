@@ -108,6 +108,8 @@ typedef struct {
   int Ni_tot, Ni_max;
   int escaped;     /* ions or_step removed through an open x */
   int injected;    /* ions its inflow appended */
+  double input_k;  /* input-energy account without the ion mass: + for ions
+                      appended, - for ions removed (or_energy_diag) */
 } or_state;
 void or_step(orc *h, or_state *s);
 /* --- open x (build-defined inflow/outflow; the reference is periodic):
@@ -115,6 +117,14 @@ void or_step(orc *h, or_state *s);
        layers divided by their share of the particle domain, no x wrap in
        move_ion_half, ions outside (q(1), q(nx-1)] removed after each half
        move --- */
+/* Energy_diag, part_init.f:14-139, over the unique nodes 2..n-1 and with
+   the weight 1/(beta beta_p): out = {input_E, Evp, Euf, EB1, EB1x, EB1y,
+   EB1z, EE, total_E, norm_E} in J for ions of mass mion (mion/mrat), with
+   input_E = mion * s->input_k; b0: the uniform background field */
+void or_energy_diag(const orc *h, const or_state *s, double mion, const double b0[3],
+                    double out[10]);
+/* kinetic energy / mion of ions [l0, l1) (1-based l0, exclusive l1) */
+double or_ke(const orc *h, const or_state *s, int l0, int l1);
 void or_set_open_x(orc *h, int on);
 int or_remove_escaped(const orc *h, or_state *s);
 /* inflow through the open x face in every later or_step (ppc = 0: none);

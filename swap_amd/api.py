@@ -44,6 +44,14 @@ class Config(C.Structure):
                 ("seed", C.c_uint64)]
 
 
+ENERGY_KEYS = ("input_E", "Evp", "Euf", "EB1", "EB1x", "EB1y", "EB1z", "EE", "total_E",
+               "norm_E")
+
+
+class Energy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ENERGY_KEYS]
+
+
 class Diag(C.Structure):
     _fields_ = [("charge", C.c_double), ("momentum", C.c_double * 3), ("kinetic", C.c_double),
                 ("divb_l2", C.c_double), ("b_energy", C.c_double), ("nparticles", C.c_int64),
@@ -123,6 +131,8 @@ def load_library():
         "swap_set_inflow": ([vp, dbl, dbl, dbl, dbl], i32),
         "swap_ionize_cloud": ([vp, i32, C.POINTER(Cloud), C.POINTER(i64), C.POINTER(i64)], i32),
         "swap_diagnostics": ([vp, C.POINTER(Diag)], i32),
+        "swap_energy_diag": ([vp, dbl, vp, C.POINTER(Energy)], i32),
+        "swap_set_input_energy": ([vp, dbl, dbl], i32),
         "swap_profile": ([vp, i32], i32),
         "swap_profile_read": ([vp, vp, vp, i32], i32),
         "swap_set_seed": ([vp, C.c_uint64], i32),
@@ -467,6 +477,18 @@ class Swap:
         self._ck(self.lib.swap_profile_read(self.h, ms.ctypes.data, cnt.ctypes.data, len(ms)),
                  "profile_read")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.STAGES)}
+
+    def energy_diag(self, mion, b0):
+        """Energy_diag's record (part_init.f:14-139) as a dict, J; b0: the
+        uniform background field (b1 units)"""
+        e = Energy()
+        b = np.ascontiguousarray(b0, np.float64)
+        self._ck(self.lib.swap_energy_diag(self.h, mion, b.ctypes.data, C.byref(e)),
+                 "energy_diag")
+        return {k: getattr(e, k) for k in ENERGY_KEYS}
+
+    def set_input_energy(self, mion, input_E):
+        self._ck(self.lib.swap_set_input_energy(self.h, mion, input_E), "set_input_energy")
 
     def diagnostics(self):
         d = Diag()

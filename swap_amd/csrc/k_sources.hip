@@ -282,6 +282,52 @@ __global__ void k_diag_particles(int64_t n, PDev<R> P, double beta, double mion,
   }
 }
 
+// mass-free kinetic energy scale * sum 1/2 |v|^2 / (mrat beta_p) (km/s)^2 ->
+// (m/s)^2 of particles [0, n) of P into *out; Energy_diag's Evp
+// (part_init.f:66-71) without the ion mass, with the weight 1/(beta beta_p)
+// (scale = 1/beta, or -1/beta for ions that leave)
+template <typename R>
+__global__ void k_ke_sum(int64_t n, PDev<R> P, double scale, double* out) {
+  double s = 0;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    double bp = P.bp ? (double)P.bp[p] : 1.0;
+    double mr = P.mr ? (double)P.mr[p] : 1.0;
+    double vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
+    s += 0.5e6 * (vx * vx + vy * vy + vz * vz) / (mr * bp);
+  }
+  s = wave_sum(s);
+  if (__lane_id() == 0) atomicAdd(out, s * scale);
+}
+
+// Energy_diag's grid sums (part_init.f:55-72) over the unique nodes:
+// [0..2] sum b1_c^2, [3] sum |b1 - b0|^2, [4] sum |E|^2, [5] sum np |up|^2
+template <typename R>
+__global__ void k_energy_grid(Geo g, const double* b, const R* E, const R* up, const R* np,
+                              double b0x, double b0y, double b0z, double* out) {
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  const double b0[3] = {b0x, b0y, b0z};
+  int64_t ncell = (int64_t)g.ncx * g.ncy * g.ncz;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ncell;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int i = 1 + (int)(q % g.ncx), j = 1 + (int)((q / g.ncx) % g.ncy),
+        k = 1 + (int)(q / ((int64_t)g.ncx * g.ncy));
+    int64_t o = gidx(g, i, j, k);
+    double n = (double)np[o];
+    for (int m = 0; m < 3; m++) {
+      double bc = b[o + m * g.nn], e = (double)E[o + m * g.nn], u = (double)up[o + m * g.nn];
+      s[m] += bc * bc;
+      s[3] += (bc - b0[m]) * (bc - b0[m]);
+      s[4] += e * e;
+      s[5] += n * u * u;
+    }
+  }
+  for (int m = 0; m < 6; m++) {
+    double t = wave_sum(s[m]);
+    if (__lane_id() == 0) atomicAdd(out + m, t);
+  }
+}
+
 // grid sums over the unique dual cells: [5] sum (div b1)^2, [6] sum |b1|^2
 __global__ void k_diag_grid(Geo g, const double* b, double dx, double dy, double dz,
                             double* out) {
@@ -384,6 +430,7 @@ int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
                                     c->cfg.ncx, c->cfg.ncy, c->cfg.ncz, (float)beta_pu, c->bp,
                                     c->mr, c->id, (uint32_t)c->next_id));
     if (int e = post(c)) return e;
+    if (int e = sr_input_add(c, c->n, m, 1.0)) return e;
   }
   c->n += m;
   c->next_id += m;
@@ -714,10 +761,69 @@ int sr_inflow_step(swap_ctx* c) {
                                   stream, c->ioff, (float)c->in_bp, c->bp, c->mr, c->id,
                                   (uint32_t)c->next_id, (typename AccOf<R>::T*)c->macc));
   if (int e = post(c)) return e;
+  if (int e = sr_input_add(c, c->n, tot, 1.0)) return e;
   c->n += tot;
   c->next_id += tot;
   c->injected += tot;
   c->in_last = tot;
   c->vb_valid = false;  // the new velocities may lie outside the pass windows' bounds
+  return 0;
+}
+
+// the input energy account (Energy_diag's input_E, global.f:47): ions that
+// enter (loads, sources, inflow) add their kinetic energy, ions that leave
+// through an open x subtract theirs; mass-free, on the device
+int sr_input_add(swap_ctx* c, int64_t off, int64_t n, double sign) {
+  if (n <= 0) return 0;
+  double* acc = c->dred + DRED_INPUT;
+  SDISPATCH(c, {
+    PDev<R> P = pdev<R>(c);
+    P.cell += off;
+    for (R** a : {&P.fx, &P.fy, &P.fz, &P.vx, &P.vy, &P.vz, &P.v1x, &P.v1y, &P.v1z}) *a += off;
+    if (P.bp) P.bp += off;
+    if (P.mr) P.mr += off;
+    hipLaunchKernelGGL((k_ke_sum<R>), dim3((unsigned)std::min<int64_t>(1024, nblk(n, 256))),
+                       dim3(256), 0, c->stream, n, P, sign / c->cfg.beta, acc);
+  });
+  return post(c);
+}
+int sr_input_reset(swap_ctx* c) {
+  HIPCHK(c, hipMemsetAsync(c->dred + DRED_INPUT, 0, sizeof(double), c->stream));
+  return 0;
+}
+
+// Energy_diag (part_init.f:14-139) in SI units with the reference's constants
+// (inputs.f:19,50-57): over the unique nodes (the reference sums the i = 1
+// ghost layer too and one y plane, j = 2, of its quasi-1-D grid) and with
+// the weight 1/(beta beta_p) for every ion (the reference divides by beta
+// and multiplies by beta_p, part_init.f:69-70)
+int sr_energy_diag(swap_ctx* c, double mion, const double b0[3], swap_energy* d) {
+  double* r = c->dred + DRED_ENERGY;  // [0] sum KE/m, [1..6] grid sums
+  HIPCHK(c, hipMemsetAsync(r, 0, 8 * sizeof(double), c->stream));
+  if (c->n > 0)
+    SDISPATCH(c, hipLaunchKernelGGL((k_ke_sum<R>), dim3(1024), dim3(256), 0, c->stream, c->n,
+                                    pdev<R>(c), 1.0 / c->cfg.beta, r));
+  SDISPATCH(c, hipLaunchKernelGGL((k_energy_grid<R>), dim3(1024), dim3(256), 0, c->stream, c->g,
+                                  c->b1, (const R*)c->fld[SWAP_F_E], (const R*)c->fld[SWAP_F_UP],
+                                  (const R*)c->fld[SWAP_F_NP], b0[0], b0[1], b0[2], r + 1));
+  if (int e = post(c)) return e;
+  double h[8], in = 0;
+  HIPCHK(c, hipMemcpyAsync(h, r, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&in, c->dred + DRED_INPUT, sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double q = 1.6e-19, pi = 3.14159, mu0 = pi * 4.0e-7, eps0 = 8.85e-12, km = 1e3;
+  const double m_q = mion / q, vol = c->cfg.dx * c->cfg.dy * c->cfg.dz * km * km * km;
+  const double bfac = vol / (2.0 * mu0) * m_q * m_q;
+  d->Evp = mion * h[0];
+  d->EB1x = bfac * h[1];
+  d->EB1y = bfac * h[2];
+  d->EB1z = bfac * h[3];
+  d->EB1 = bfac * h[4];
+  d->EE = eps0 * vol / 2.0 * (m_q * km) * (m_q * km) * h[5];
+  d->Euf = 0.5 * mion * vol / (km * km * km) * km * km * h[6];
+  d->input_E = mion * in;
+  d->total_E = d->Evp + d->EE + d->EB1;
+  d->norm_E = d->input_E != 0 ? d->total_E / d->input_E : 0.0;
   return 0;
 }

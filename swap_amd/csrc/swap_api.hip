@@ -171,6 +171,12 @@ int subcycle_impl(swap_ctx* c, double dtsub) {
 }
 // the same subcycle as two LDS-tiled launches; the corrector writes into
 // b12's buffer, so f_update_tlev is the rotation b12 <- b1 <- corrected
+// a load starts the input-energy account from the loaded ions
+int load_input_energy(swap_ctx* c) {
+  TRY(sr_input_reset(c));
+  return sr_input_add(c, 0, c->n, 1.0);
+}
+
 int subcycle_fused(swap_ctx* c, double dtsub, bool last) {
   TRY(fl_subcycle_fused(c, dtsub, last));
   double* corrected = c->b12;
@@ -387,6 +393,7 @@ int swap_load_particles(swap_ctx* c, int64_t n, const double* xp, const double* 
   if (dtmp) HIPCHK(c, hipFree(dtmp));
   c->n_sorted = -1;
   c->vb_valid = false;
+  if (!e) TRY(load_input_energy(c));
   return e;
 }
 
@@ -435,7 +442,7 @@ int swap_load_particles_split(swap_ctx* c, int64_t n, const uint32_t* cell, cons
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_sorted = -1;
   c->vb_valid = false;
-  return 0;
+  return load_input_energy(c);
 }
 
 int swap_load_maxwellian(swap_ctx* c, int64_t n, double vsw, double vth, double beta_p,
@@ -451,7 +458,7 @@ int swap_load_maxwellian(swap_ctx* c, int64_t n, double vsw, double vth, double 
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_sorted = -1;
   c->vb_valid = false;
-  return 0;
+  return load_input_energy(c);
 }
 
 int swap_get_particle_array(swap_ctx* c, int which, void* host) {
@@ -927,6 +934,20 @@ int swap_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cloud, int64_t* i
 int swap_diagnostics(swap_ctx* c, swap_diag* d) {
   if (!c || !d) return SWAP_EINVAL;
   return sr_diagnostics(c, d);
+}
+
+int swap_energy_diag(swap_ctx* c, double mion, const double b0[3], swap_energy* d) {
+  if (!c || !b0 || !d || !(mion > 0)) return SWAP_EINVAL;
+  return sr_energy_diag(c, mion, b0, d);
+}
+
+int swap_set_input_energy(swap_ctx* c, double mion, double input_E) {
+  if (!c || !(mion > 0) || !std::isfinite(input_E)) return SWAP_EINVAL;
+  double k = input_E / mion;
+  HIPCHK(c, hipMemcpyAsync(c->dred + DRED_INPUT, &k, sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int swap_profile(swap_ctx* c, int enable) {

@@ -177,6 +177,12 @@ int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
 int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* injected,
                     int64_t* chex);
 int sr_diagnostics(swap_ctx* c, swap_diag* d);
+// slots of the FP64 reduction scratch dred: the input-energy account and the
+// energy diagnostic's sums
+constexpr int DRED_INPUT = 32, DRED_ENERGY = 40;
+int sr_input_add(swap_ctx* c, int64_t off, int64_t n, double sign);
+int sr_input_reset(swap_ctx* c);
+int sr_energy_diag(swap_ctx* c, double mion, const double b0[3], swap_energy* d);
 int sr_set_inflow(swap_ctx* c, double ppc, double vsw, double vth, double beta_p);
 int sr_inflow_step(swap_ctx* c);  // after the final particle pass of swap_step
 int sr_vbounds(swap_ctx* c);  // signed per-axis velocity bounds into c->vb

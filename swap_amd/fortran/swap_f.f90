@@ -59,6 +59,11 @@ module swap_hip_c
      integer(c_int64_t) :: seed
   end type swap_config
 
+  ! Energy_diag's record (part_init.f:14-139), J
+  type, bind(C) :: swap_energy
+     real(c_double) :: input_E, Evp, Euf, EB1, EB1x, EB1y, EB1z, EE, total_E, norm_E
+  end type swap_energy
+
   interface
      integer(c_int) function swap_create(cfg, out) bind(C, name="swap_create")
        import :: c_int, c_ptr, swap_config
@@ -209,6 +214,20 @@ module swap_hip_c
        type(c_ptr), value :: ctx
        integer(c_int), value :: nsteps
      end function swap_step
+     integer(c_int) function swap_energy_diag(ctx, mion, b0, out) &
+          bind(C, name="swap_energy_diag")
+       import :: c_int, c_ptr, c_double, swap_energy
+       type(c_ptr), value :: ctx
+       real(c_double), value :: mion
+       real(c_double), intent(in) :: b0(3)
+       type(swap_energy), intent(out) :: out
+     end function swap_energy_diag
+     integer(c_int) function swap_set_input_energy(ctx, mion, input_E) &
+          bind(C, name="swap_set_input_energy")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: mion, input_E
+     end function swap_set_input_energy
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s

@@ -21,6 +21,12 @@ memory order of this package's numpy arrays, e.g. a grid (nx,ny,nz,3) is
   c.vp_0.dat      maind.f:551-552   m; vp(Ni_max,3)
   c.mrat_0.dat    maind.f:553-554   m; mrat(Ni_max)
   c.beta_p_0.dat  maind.f:555-556   m; beta_p(Ni_max)
+  c.energy.dat    maind.f:503-504   m; input_E, input_EeP, Evp, Euf, EB1, EB1x, EB1y, EB1z,
+                                    EE, EeP, input_chex, input_bill (12 REAL per step)
+
+The energy record's electron-pressure and chemistry terms (input_EeP, EeP,
+input_chex, input_bill) are not tracked by the time-step path and are
+written as 0.
 
 Particle records hold Ni_max (the context capacity) entries, the first n
 live, the rest zero, as the reference writes its whole arrays.  The other
@@ -28,10 +34,11 @@ files of maind.f:268-333 hold quantities outside the time-step path
 (temperatures, separated populations, energy budgets) and are not written.
 
 Checkpoint (replaces restart.vars, maind.f:588-607): an .npz of the
-configuration, the B levels b1 and b12, and the split particle arrays (cell,
-fraction, vp, vp1, beta_p, mrat, id).  Everything else a step reads is
-recomputed by the step, so a run resumed from a checkpoint continues bit for
-bit (the moment sums are exact and independent of particle order).
+configuration, the B levels b1 and b12, the split particle arrays (cell,
+fraction, vp, vp1, beta_p, mrat, id) and the input-energy account.
+Everything else a step reads is recomputed by the step, so a run resumed
+from a checkpoint continues bit for bit (the moment sums are exact and
+independent of particle order).
 """
 import os
 import struct
@@ -157,6 +164,42 @@ class OutputWriter:
             write_record(self.f[k], a)
 
 
+ENERGY_RECORD = ("input_E", "input_EeP", "Evp", "Euf", "EB1", "EB1x", "EB1y", "EB1z", "EE",
+                 "EeP", "input_chex", "input_bill")
+
+
+class EnergyWriter:
+    """c.energy.dat (maind.f:287, 503-504): per step, m then 12 REAL"""
+
+    def __init__(self, out_dir):
+        os.makedirs(out_dir, exist_ok=True)
+        self.f = open(os.path.join(out_dir, "c.energy.dat"), "wb")
+
+    def write(self, m, e):
+        """e: a swap_energy_diag dict"""
+        write_record(self.f, m)
+        write_record(self.f, np.array([e.get(k, 0.0) for k in ENERGY_RECORD], np.float32))
+
+    def close(self):
+        self.f.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def read_energy(out_dir):
+    """(m, dict) per record of c.energy.dat"""
+    recs = read_records(os.path.join(out_dir, "c.energy.dat"))
+    out = []
+    for a, b in zip(recs[::2], recs[1::2]):
+        (m,) = struct.unpack("<i", a)
+        out.append((m, dict(zip(ENERGY_RECORD, np.frombuffer(b, "<f4").tolist()))))
+    return out
+
+
 def read_output(out_dir, name, shape):
     """(m, array) pairs of one c.*.dat file; shape of one record's array"""
     recs = read_records(os.path.join(out_dir, _NAMES.get(name, name)))
@@ -183,6 +226,8 @@ def save_checkpoint(path, s):
         arrays["mrat"] = s.particle_array(api.P_MRAT)
     if s.cfg.flags & api.TRACK_IDS:
         arrays["id"] = s.particle_array(api.P_ID)
+    # the input-energy account without the ion mass (mion = 1)
+    arrays["input_k"] = np.array([s.energy_diag(1.0, (0.0, 0.0, 0.0))["input_E"]])
     np.savez(path, cfg=np.array([cfg[k] for k in _CFG], np.float64),
              cfg_int=np.array([int(cfg[k]) for k in ("ncx", "ncy", "ncz", "ntsub", "capacity",
                                                     "precision", "flags", "sort_every")],
@@ -205,4 +250,6 @@ def load_checkpoint(path):
                  z["mrat"] if "mrat" in z.files else None)
     if "id" in z.files:
         s.set_particle_array(api.P_ID, z["id"])
+    if "input_k" in z.files:
+        s.set_input_energy(1.0, float(z["input_k"][0]))
     return s
