@@ -65,6 +65,7 @@ def _load(width):
         "or_move_ion_half": ([vp, i32, i32, vp, vp], None),
         "or_step": ([vp, vp], None),
         "or_set_open_x": ([vp, i32], None),
+        "or_read_load": ([C.c_char_p, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
         "or_energy_diag": ([vp, vp, dbl, vp, vp], None),
         "or_ke": ([vp, vp, i32, i32], dbl),
         "or_inflow": ([C.c_uint64, C.c_uint32, vp, dbl, vp, dbl, dbl, dbl, vp, vp, vp, vp,
@@ -320,3 +321,24 @@ def load_maxwellian(n, seed, nc, vsw, vth, stream=1):
     lib.or_load_maxwellian(n, seed, stream, nc.ctypes.data, vsw, vth, cell.ctypes.data,
                            frac.ctypes.data, v.ctypes.data)
     return cell, frac, v
+
+
+def read_load(path):
+    """the load file read by the oracle's C reader: (cell, frac[3,n], vp[3,n],
+    vp1[3,n], beta_p, mrat, id); beta_p, mrat and id are zeros where the
+    file holds none"""
+    lib, _ = _load(64)
+    n = C.c_int64()
+    nc = np.zeros(3, np.int32)
+    p = path.encode()
+    if lib.or_read_load(p, 0, C.byref(n), nc.ctypes.data, *([None] * 7)) != 0:
+        raise ValueError("%s: not a load file" % path)
+    m = n.value
+    cell = np.zeros(m, np.uint32)
+    frac, vp, vp1 = np.zeros((3, m)), np.zeros((3, m)), np.zeros((3, m))
+    bp, mr, ids = np.zeros(m, np.float32), np.zeros(m, np.float32), np.zeros(m, np.uint32)
+    if lib.or_read_load(p, m, C.byref(n), nc.ctypes.data, cell.ctypes.data, frac.ctypes.data,
+                        vp.ctypes.data, vp1.ctypes.data, bp.ctypes.data, mr.ctypes.data,
+                        ids.ctypes.data) != 0:
+        raise ValueError("%s: short load file" % path)
+    return cell, frac, vp, vp1, bp, mr, ids

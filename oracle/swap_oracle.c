@@ -13,6 +13,7 @@
  */
 #include "swap_oracle.h"
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1147,4 +1148,47 @@ void or_energy_diag(const orc *h, const or_state *s, double mion, const double b
   double o[10] = {input, Evp, Euf, EB1, EBc[0], EBc[1], EBc[2], EE, total,
                   input != 0 ? total / input : 0.0};
   memcpy(out, o, sizeof(o));
+}
+
+/* the load file of swap_amd/io.py (header "SWAPLD01", n, nc, flags, seed,
+   d; cell, frac, vp, vp1 [, beta_p, mrat] [, id]), read with plain fread:
+   with cap = 0 only *n and nc are filled; else up to cap ions, arrays as
+   [3][cap] (beta_p, mrat, id may be NULL).  Returns 0, or -1 if the file
+   is not a load file or is short. */
+int or_read_load(const char *path, int64_t cap, int64_t *n, int nc[3], uint32_t *cell,
+                 double *frac, double *vp, double *vp1, float *bp, float *mr, uint32_t *id) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  char magic[8];
+  int32_t hdr[4];
+  int64_t nn;
+  uint64_t seed;
+  double d[3];
+  int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "SWAPLD01", 8) == 0 &&
+           fread(&nn, 8, 1, f) == 1 && fread(hdr, 4, 4, f) == 4 && fread(&seed, 8, 1, f) == 1 &&
+           fread(d, 8, 3, f) == 3;
+  if (!ok) { fclose(f); return -1; }
+  *n = nn;
+  nc[0] = hdr[0]; nc[1] = hdr[1]; nc[2] = hdr[2];
+  if (cap == 0) { fclose(f); return 0; }
+  int64_t m = nn < cap ? nn : cap;
+  ok = fread(cell, 4, (size_t)m, f) == (size_t)m;
+  if (ok && m < nn) ok = fseek(f, 4 * (nn - m), SEEK_CUR) == 0;
+  double *dst[3] = {frac, vp, vp1};
+  for (int q = 0; q < 3 && ok; q++)
+    for (int a = 0; a < 3 && ok; a++) {
+      ok = fread(dst[q] + (size_t)a * cap, 8, (size_t)m, f) == (size_t)m;
+      if (ok && m < nn) ok = fseek(f, 8 * (nn - m), SEEK_CUR) == 0;
+    }
+  if (ok && (hdr[3] & 1)) {
+    float *fd[2] = {bp, mr};
+    for (int q = 0; q < 2 && ok; q++) {
+      if (fd[q]) ok = fread(fd[q], 4, (size_t)m, f) == (size_t)m;
+      else ok = fseek(f, 4 * m, SEEK_CUR) == 0;
+      if (ok && m < nn) ok = fseek(f, 4 * (nn - m), SEEK_CUR) == 0;
+    }
+  }
+  if (ok && (hdr[3] & 2) && id) ok = fread(id, 4, (size_t)m, f) == (size_t)m;
+  fclose(f);
+  return ok ? 0 : -1;
 }

@@ -160,6 +160,9 @@ void or_cloud_counts(const int nc[3], double dx, double dy, double dz, double be
                      uint32_t *cnt, float *bp);
 void or_cloud_ions(const int nc[3], const uint32_t *cnt, uint64_t seed, uint32_t step,
                    int open_x, uint32_t *cell, double *frac, int64_t m);
+/* the load file written by swap_amd/io.py (see swap_oracle.c) */
+int or_read_load(const char *path, int64_t cap, int64_t *n, int nc[3], uint32_t *cell,
+                 double *frac, double *vp, double *vp1, float *bp, float *mr, uint32_t *id);
 /* the inflow candidates of one step (build-defined): see swap_oracle.c */
 int64_t or_inflow(uint64_t seed, uint32_t step, const int nc[3], double dt, const double d[3],
                   double ppc, double vsw, double vth, double *th, double *te, double *v,

@@ -210,6 +210,83 @@ def read_output(out_dir, name, shape):
     return out
 
 
+# ---------------- load file ----------------
+#
+# The seeded initial load in a form any code can read with plain binary I/O
+# (the oracle reads it in C, oracle/swap_oracle.c or_read_load), so that
+# both codes start from one file.  Little-endian:
+#   header (64 bytes): char magic[8] = "SWAPLD01"; int64 n; int32 ncx, ncy,
+#     ncz; int32 flags (1: beta_p and mrat follow, 2: ids follow); uint64
+#     seed; float64 dx, dy, dz
+#   uint32 cell[n] (i | j<<11 | k<<21, 0-based cells)
+#   float64 frac[3][n] (in-cell offsets in (0,1], right-closed cells)
+#   float64 vp[3][n], vp1[3][n] (km/s)
+#   [float32 beta_p[n], mrat[n]] [uint32 id[n]]
+# The absolute position is x = (i + 1 + frac) dx (q(i) = i dx, grd7
+# initial.f:193-196 on a uniform grid).
+
+LOAD_MAGIC = b"SWAPLD01"
+
+
+def write_load(path, cell, frac, vp, vp1, nc, d, seed=0, beta_p=None, mrat=None, ids=None):
+    cell = np.ascontiguousarray(cell, "<u4")
+    n = cell.size
+    flags = (1 if beta_p is not None else 0) | (2 if ids is not None else 0)
+    with open(path, "wb") as f:
+        f.write(LOAD_MAGIC)
+        f.write(struct.pack("<q3ii", n, nc[0], nc[1], nc[2], flags))
+        f.write(struct.pack("<Q3d", seed, d[0], d[1], d[2]))
+        f.write(cell.tobytes())
+        for a in (frac, vp, vp1):
+            f.write(np.ascontiguousarray(np.asarray(a, np.float64).reshape(3, n), "<f8").tobytes())
+        if beta_p is not None:
+            f.write(np.ascontiguousarray(beta_p, "<f4").tobytes())
+            f.write(np.ascontiguousarray(np.ones(n) if mrat is None else mrat, "<f4").tobytes())
+        if ids is not None:
+            f.write(np.ascontiguousarray(ids, "<u4").tobytes())
+
+
+def read_load(path):
+    """dict of the load file's header fields and arrays"""
+    with open(path, "rb") as f:
+        if f.read(8) != LOAD_MAGIC:
+            raise ValueError("%s: not a SWAPLD01 load file" % path)
+        n, ncx, ncy, ncz, flags = struct.unpack("<q3ii", f.read(24))
+        seed, dx, dy, dz = struct.unpack("<Q3d", f.read(32))
+        out = {"n": n, "nc": (ncx, ncy, ncz), "seed": seed, "d": (dx, dy, dz)}
+        out["cell"] = np.frombuffer(f.read(4 * n), "<u4").copy()
+        for k in ("frac", "vp", "vp1"):
+            out[k] = np.frombuffer(f.read(24 * n), "<f8").reshape(3, n).copy()
+        if flags & 1:
+            out["beta_p"] = np.frombuffer(f.read(4 * n), "<f4").copy()
+            out["mrat"] = np.frombuffer(f.read(4 * n), "<f4").copy()
+        if flags & 2:
+            out["id"] = np.frombuffer(f.read(4 * n), "<u4").copy()
+    return out
+
+
+def save_load(path, s):
+    """the context's particles as a load file"""
+    sp = s.cfg.flags & api.SPECIES
+    write_load(path, s.particle_array(api.P_CELL), s.particle_array(api.P_FRAC),
+               s.particle_array(api.P_VP), s.particle_array(api.P_VP1),
+               (s.cfg.ncx, s.cfg.ncy, s.cfg.ncz), (s.cfg.dx, s.cfg.dy, s.cfg.dz), s.cfg.seed,
+               s.particle_array(api.P_BETA) if sp else None,
+               s.particle_array(api.P_MRAT) if sp else None,
+               s.particle_array(api.P_ID) if s.cfg.flags & api.TRACK_IDS else None)
+
+
+def load_file(s, path):
+    """load a load file into the context (same grid)"""
+    z = read_load(path)
+    if tuple(z["nc"]) != (s.cfg.ncx, s.cfg.ncy, s.cfg.ncz):
+        raise ValueError("%s: grid %s, context %s" % (path, z["nc"],
+                                                     (s.cfg.ncx, s.cfg.ncy, s.cfg.ncz)))
+    s.load_split(z["cell"], z["frac"], z["vp"], z["vp1"], z.get("beta_p"), z.get("mrat"))
+    if "id" in z and s.cfg.flags & api.TRACK_IDS:
+        s.set_particle_array(api.P_ID, z["id"])
+
+
 # ---------------- checkpoint ----------------
 
 _CFG = ("ncx", "ncy", "ncz", "dx", "dy", "dz", "dt", "ntsub", "alpha", "beta", "nu",

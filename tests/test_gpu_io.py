@@ -1,6 +1,7 @@
 """Output files and checkpoints of a device run: the per-output records hold
 the device fields and particles (FP32, reference layouts), and a run resumed
-from a checkpoint continues bit for bit like the uninterrupted run."""
+from a checkpoint continues bit for bit like the uninterrupted run; the load
+file is the identical seeded load for both codes."""
 import numpy as np
 import pytest
 
@@ -63,3 +64,38 @@ def test_checkpoint_resume_is_bit_exact(tmp_path):
     for which in (api.P_CELL, api.P_FRAC, api.P_VP, api.P_VP1, api.P_BETA):
         xa, xb = a.particle_array(which), b.particle_array(which)
         assert np.array_equal(xa[..., oa], xb[..., ob]), which
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_load_file_is_the_identical_seeded_load(tmp_path, precision):
+    """the device's seeded load, saved as a load file, is the oracle's seeded
+    load when the oracle's C reader reads it, and a second context loaded
+    from the file steps bit for bit like the first"""
+    import oracle as O
+    from helpers import Pair
+    a = Pair(precision=precision, flags=api.TRACK_IDS)
+    p = a.p
+    path = str(tmp_path / "load.bin")
+    io.save_load(path, a.s)
+    cell, frac, v, v1, _, _, ids = O.read_load(path)
+    oc, of, ov = O.load_maxwellian(a.n, a.s.cfg.seed, (p["ncx"], p["ncy"], p["ncz"]),
+                                   p["vsw"], p["vth"])
+    rd = np.float32 if precision == 32 else np.float64
+    assert np.array_equal(cell, oc)
+    assert np.array_equal(frac, of.astype(rd).astype(np.float64))
+    assert np.max(np.abs(v - ov.astype(rd))) <= 1e-9 * p["vsw"]
+    assert np.array_equal(ids, np.arange(a.n, dtype=np.uint32))
+    b = Pair(precision=precision, flags=api.TRACK_IDS, load=False)
+    io.load_file(b.s, path)
+    for s in (a.s, b.s):
+        s.step(2)
+    # FP32 moments are exact fixed-point sums (bit for bit); the FP64 build
+    # adds doubles with atomics, whose order may differ between runs
+    for f in ("np", "up", "E", "b1"):
+        x, y = a.s.get_field(f), b.s.get_field(f)
+        if precision == 32:
+            assert np.array_equal(x, y), f
+        else:
+            assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(y)), f
+    if precision == 32:
+        assert np.array_equal(a.s.particle_array(api.P_CELL), b.s.particle_array(api.P_CELL))

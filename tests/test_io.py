@@ -1,11 +1,12 @@
 """The reference's on-disk formats without a GPU: Fortran unformatted
 records (4-byte length markers around each WRITE, -i4 -real-size 32) and the
 record layout of para.dat (maind.f:237-255) and c.coord.dat
-(initial.f:285-297)."""
+(initial.f:285-297); the load file read by both codes."""
 import struct
 import types
 
 import numpy as np
+import pytest
 
 from swap_amd import io, physics
 
@@ -53,3 +54,34 @@ def test_para_and_coord_layout(tmp_path):
     qx = np.frombuffer(c[3], "<f4")
     assert qx.size == 10 and np.allclose(np.diff(qx), p["dx"], rtol=1e-6)
     assert [len(x) // 4 for x in c[3:]] == [10, 8, 6, 6, 6]
+
+
+def test_load_file_round_trip_through_the_oracle_reader(tmp_path):
+    """the seeded load written by io.write_load reads back bit for bit in the
+    oracle's C reader and in io.read_load"""
+    import oracle as O
+    nc = (12, 6, 4)
+    n = 3000
+    cell, frac, v = O.load_maxwellian(n, 77, nc, 400.0, 30.0)
+    rng = np.random.default_rng(2)
+    bp = rng.random(n).astype(np.float32)
+    ids = np.arange(n, dtype=np.uint32)[::-1].copy()
+    path = str(tmp_path / "load.bin")
+    io.write_load(path, cell, frac, v, v * 0.5, nc, (360.0, 360.0, 360.0), seed=77,
+                  beta_p=bp, mrat=np.full(n, 0.25), ids=ids)
+    c2, f2, v2, v12, bp2, mr2, id2 = O.read_load(path)
+    assert np.array_equal(c2, cell) and np.array_equal(f2, frac)
+    assert np.array_equal(v2, v) and np.array_equal(v12, v * 0.5)
+    assert np.array_equal(bp2, bp) and np.all(mr2 == np.float32(0.25))
+    assert np.array_equal(id2, ids)
+    z = io.read_load(path)
+    assert z["n"] == n and tuple(z["nc"]) == nc and z["seed"] == 77
+    assert np.array_equal(z["cell"], cell) and np.array_equal(z["frac"], frac)
+    # a short file is refused by both readers
+    with open(path, "rb") as f:
+        head = f.read(64 + 4 * n + 100)
+    short = str(tmp_path / "short.bin")
+    with open(short, "wb") as f:
+        f.write(head)
+    with pytest.raises(ValueError):
+        O.read_load(short)
