@@ -108,6 +108,18 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// a[k] = v for a loop index k that is the same in every lane (the selects
+// keep a[] in registers; an indexed store would put it in scratch memory)
+template <typename T>
+__device__ __forceinline__ void lane_put(T (&a)[4], int k, T v) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) a[j] = k == j ? v : a[j];
+}
+template <typename T>
+__device__ __forceinline__ void lane_put(T (&a)[1], int, T v) {
+  a[0] = v;
+}
+
 // node of unwrapped x node coordinate u: periodic, or on an open x (OX) the
 // clamp of node_of_x.  A template flag, so that the periodic kernels carry
 // none of the open-x code (their registers set their occupancy).
@@ -224,8 +236,18 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   float vlo[3] = {3e38f, 3e38f, 3e38f}, vhi[3] = {-3e38f, -3e38f, -3e38f};
   unsigned long long misses = 0;
 
-  for (int64_t chunk = i0; chunk < i1; chunk += (int64_t)PASS_WG * KP) {
+  // P3 with 4 consecutive particles per lane (FP32) keeps the lane's new
+  // state in registers and writes each array with one 16-byte store: a store
+  // instruction then covers whole cache lines (per-particle stores left each
+  // line a quarter written per instruction, and lines reached HBM in pieces).
+  // Lane groups start at multiples of 4 particles, so those stores are
+  // aligned; a group that straddles the item's ends is written per particle.
+  constexpr bool BUFW = MODE == 3 && KP == 4 && sizeof(R) == 4;
+  const int64_t ib = BUFW ? (i0 & ~(int64_t)3) : i0;
+  for (int64_t chunk = ib; chunk < i1; chunk += (int64_t)PASS_WG * KP) {
     const int64_t pb = chunk + (int64_t)tid * KP;
+    float wv[BUFW ? 9 : 1][BUFW ? 4 : 1];  // v1 xyz, v xyz, fraction xyz per particle
+    uint32_t wc[BUFW ? 4 : 1];              // packed cell per particle
     int cur = -1;   // window cell of the running sums acc
     int hkey = -1;  // window cell of hd: the lane's first cell, once the lane left it
     R acc[8][M], hd[8][M];
@@ -240,6 +262,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     for (int k = 0; k < KP; k++) {
       const int64_t p = pb + k;
       if (p >= i1) break;
+      if (BUFW && p < i0) continue;  // before the item: another item's particle
       const uint32_t c = P.cell[p];
       const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
       R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
@@ -286,8 +309,14 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
             vlo[m] = fminf(vlo[m], (float)vn[m]);
             vhi[m] = fmaxf(vhi[m], (float)vn[m]);
           }
-          P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
-          P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
+          if constexpr (BUFW) {
+            lane_put(wv[0], k, (float)v[0]); lane_put(wv[1], k, (float)v[1]);
+            lane_put(wv[2], k, (float)v[2]); lane_put(wv[3], k, (float)vn[0]);
+            lane_put(wv[4], k, (float)vn[1]); lane_put(wv[5], k, (float)vn[2]);
+          } else {
+            P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
+            P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
+          }
           ux += half_move(f0, vn[0], K.s[0]);
           uy += half_move(f1, vn[1], K.s[1]);
           uz += half_move(f2, vn[2], K.s[2]);
@@ -345,14 +374,41 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         ux += half_move(f0, vn[0], K.s[0]);
         uy += half_move(f1, vn[1], K.s[1]);
         uz += half_move(f2, vn[2], K.s[2]);
-        P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
+        uint32_t cnew;
         if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
-          P.cell[p] = DEAD_CELL;  // the sort at the end of the step drops it
+          cnew = DEAD_CELL;  // the sort at the end of the step drops it
         } else {
-          P.cell[p] = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
+          cnew = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
           bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
           bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
           bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+        }
+        if constexpr (BUFW) {
+          lane_put(wv[6], k, (float)f0); lane_put(wv[7], k, (float)f1);
+          lane_put(wv[8], k, (float)f2); lane_put(wc, k, cnew);
+        } else {
+          P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
+          P.cell[p] = cnew;
+        }
+      }
+    }
+    if constexpr (BUFW) {
+      float* dst[9] = {(float*)P.v1x, (float*)P.v1y, (float*)P.v1z, (float*)P.vx,
+                       (float*)P.vy,  (float*)P.vz,  (float*)P.fx,  (float*)P.fy,
+                       (float*)P.fz};
+      if (pb >= i0 && pb + 4 <= i1) {  // pb is a multiple of 4: 16-byte aligned
+#pragma unroll
+        for (int a = 0; a < 9; a++)
+          *(float4*)(dst[a] + pb) = make_float4(wv[a][0], wv[a][1], wv[a][2], wv[a][3]);
+        *(uint4*)(P.cell + pb) = make_uint4(wc[0], wc[1], wc[2], wc[3]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int64_t p = pb + k;
+          if (p < i0 || p >= i1) continue;
+#pragma unroll
+          for (int a = 0; a < 9; a++) dst[a][p] = wv[a][k];
+          P.cell[p] = wc[k];
         }
       }
     }
