@@ -76,8 +76,13 @@ __device__ __forceinline__ uint32_t sort_key(uint32_t c, int ntx, int nty, uint3
   return c == DEAD_CELL ? dead_key : tile_key(c, ntx, nty);
 }
 
-// A: one workgroup per chunk.  Keys below 2^nbits; padding positions get
-// the largest nbits-bit key and, the sort being stable, end up last.
+// A: one workgroup per chunk.  The radix sort runs on key - (the chunk's
+// smallest key) over only as many bits as the chunk's key range needs: a
+// chunk is one z layer of a tile, whose particles stay within a tile or two
+// between sorts, so about 9 of the up to 32 key bits (3 passes instead of 6;
+// a chunk at the periodic x seam spans the grid and takes every pass).
+// Padding positions get the largest key of that width and, the sort being
+// stable, end up last.
 __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restrict__ cell,
                                               int ntx, int nty, uint32_t dead_key, int nbits,
                                               uint32_t* cnt,
@@ -92,6 +97,7 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
   const int64_t c0 = (int64_t)blockIdx.x * SCH;
   const int m = (int)min((int64_t)SCH, n - c0);
   const uint32_t padkey = nbits >= 32 ? 0xFFFFFFFFu : ((1u << nbits) - 1u);
+  __shared__ uint32_t red[2][ST / 64];
   uint32_t key[SE], idx[SE];
   if (t * SE + SE <= m) {
     const uint4* src = (const uint4*)(cell + c0) + 2 * t;  // c0 is a multiple of 4096
@@ -109,9 +115,38 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
 #pragma unroll
   for (int e = 0; e < SE; e++) idx[e] = (uint32_t)(t * SE + e);
 
+  // the chunk's key range (valid positions only) and the bits it needs
+  uint32_t lo = 0xFFFFFFFFu, hi = 0;
+#pragma unroll
+  for (int e = 0; e < SE; e++)
+    if (t * SE + e < m) {
+      lo = min(lo, key[e]);
+      hi = max(hi, key[e]);
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+  }
+  if ((t & 63) == 0) {
+    red[0][t >> 6] = lo;
+    red[1][t >> 6] = hi;
+  }
+  __syncthreads();
+  lo = red[0][0];
+  hi = red[1][0];
+  for (int q = 1; q < ST / 64; q++) {
+    lo = min(lo, red[0][q]);
+    hi = max(hi, red[1][q]);
+  }
+  int nbl = 0;
+  while (nbl < nbits && ((hi - lo) >> nbl) != 0u) nbl++;
+  const uint32_t padrel = nbl >= 32 ? 0xFFFFFFFFu : ((1u << nbl) - 1u);
+#pragma unroll
+  for (int e = 0; e < SE; e++) key[e] = t * SE + e < m ? key[e] - lo : padrel;
+
   // stable LSD radix sort, 4 bits per pass; thread t holds positions
   // t*SE .. t*SE+SE-1 before and after every pass
-  for (int sh = 0; sh < nbits; sh += 4) {
+  for (int sh = 0; sh < nbl; sh += 4) {
     int d[SE], r[SE];
 #pragma unroll
     for (int e = 0; e < SE; e++) d[e] = (int)((key[e] >> sh) & 15u);
@@ -160,7 +195,7 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
       idx[e] = sidx[t * SE + e];
     }
   }
-  if (nbits == 0) {
+  if (nbl == 0) {
 #pragma unroll
     for (int e = 0; e < SE; e++) skey[t * SE + e] = key[e];
   }
@@ -187,7 +222,7 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
   __syncthreads();
   for (uint32_t q = t; q < nseg; q += ST) {
     uint32_t st = scan[q], len = scan[q + 1] - st;
-    uint32_t k = skey[st];
+    uint32_t k = skey[st] + lo;  // back to the absolute key
     segkey[c0 + q] = k;
     seginfo[c0 + q] = st | (len << 12);
     atomicAdd(cnt + k, len);
