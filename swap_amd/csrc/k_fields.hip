@@ -990,36 +990,62 @@ int fl_copy_b(swap_ctx* c, const double* src, double* dst) {
 }
 // one fused subcycle: b1p2 = predictor(b1, b12); b12's buffer = corrector
 // (b1p2, b1); E and aj of the corrector are stored when last is set
-// tile of canonical columns per workgroup, z planes per chunk, threads of
-// the predictor (NP) and the corrector (NC).  FP32: 32x8 columns, 32 planes
-// and 512 threads measured best among 16..64 x 4..12 columns, 16..64 planes
-// and 256..1024 threads at 256x128x128 cells (more threads than columns hide
-// the latency of each z step's phases).  The FP64 build's LDS rings take
-// twice the bytes per position.
+// tile of canonical columns per workgroup and threads of the predictor (NP)
+// and the corrector (NC).  FP32: 32x8 columns and 512 threads measured best
+// among 16..64 x 4..12 columns and 256..1024 threads at 256x128x128 cells
+// (more threads than columns hide the latency of each z step's phases).  The
+// z chunks follow from occupancy (z_chunks): there the predictor holds two
+// workgroups per CU and marches 32 planes, the corrector (more LDS) one and
+// 64 planes, 0.11 ms/step less than 32 planes for both.  The FP64 build's
+// LDS rings take twice the bytes per position.
 template <typename R>
 struct FTile {
-  static constexpr int TX = 32, TY = 8, TZ = 32, NP = 512, NC = 512;
+  static constexpr int TX = 32, TY = 8, NP = 512, NC = 512;
 };
 template <>
 struct FTile<double> {
-  static constexpr int TX = 32, TY = 4, TZ = 32, NP = 128, NC = 128;
+  static constexpr int TX = 32, TY = 4, NP = 128, NC = 128;
 };
+
+// z chunks of a fused launch: as many workgroups as the chip holds at once
+// (CUs x the kernel's resident workgroups per CU, which its LDS sets), so a
+// launch runs in one round: with each workgroup marching its z chunk, a
+// second round of workgroups would double the time of a launch whose chip
+// was full in the first.  Computed once per context and kernel.
+template <typename Kern>
+int z_chunks(swap_ctx* c, Kern kern, int nthreads, int nxy, int* cache) {
+  if (*cache > 0) return *cache;
+  int per = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, nthreads, 0) != hipSuccess ||
+      per < 1)
+    per = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) !=
+          hipSuccess ||
+      cus < 1)
+    cus = 256;
+  *cache = std::max(1, std::min((cus * per) / std::max(nxy, 1), c->g.ncz));
+  return *cache;
+}
 
 int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last) {
   FC f = fc_of(c);
   const Geo& g = c->g;
-  DISPATCH(c, constexpr int TX = FTile<R>::TX, TY = FTile<R>::TY, TZ = FTile<R>::TZ;
+  DISPATCH(c, constexpr int TX = FTile<R>::TX, TY = FTile<R>::TY;
            constexpr int NP = FTile<R>::NP, NC = FTile<R>::NC;
-           const int nzc = (g.ncz + TZ - 1) / TZ, tz = (g.ncz + nzc - 1) / nzc;
-           const dim3 grid((g.ncx + TX - 1) / TX, (g.ncy + TY - 1) / TY, nzc);
-           hipLaunchKernelGGL((k_sub_predict<R, TX, TY, NP>), grid, dim3(NP), 0, c->stream, g, f,
+           const int gx = (g.ncx + TX - 1) / TX, gy = (g.ncy + TY - 1) / TY;
+           const int zp = z_chunks(c, k_sub_predict<R, TX, TY, NP>, NP, gx * gy, &c->fz_pred);
+           const int zc = z_chunks(c, k_sub_correct<R, TX, TY, NC>, NC, gx * gy, &c->fz_corr);
+           const int tzp = (g.ncz + zp - 1) / zp, tzc = (g.ncz + zc - 1) / zc;
+           hipLaunchKernelGGL((k_sub_predict<R, TX, TY, NP>), dim3(gx, gy, (g.ncz + tzp - 1) / tzp),
+                              dim3(NP), 0, c->stream, g, f,
                                  (const double*)c->b1, (const double*)c->b12,
                                  (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
-                                 (const R*)F<R>(c, SWAP_F_NU), 2.0 * dtsub, tz, c->b1p2);
-           hipLaunchKernelGGL((k_sub_correct<R, TX, TY, NC>), grid, dim3(NC), 0, c->stream, g, f,
+                                 (const R*)F<R>(c, SWAP_F_NU), 2.0 * dtsub, tzp, c->b1p2);
+           hipLaunchKernelGGL((k_sub_correct<R, TX, TY, NC>), dim3(gx, gy, (g.ncz + tzc - 1) / tzc),
+                              dim3(NC), 0, c->stream, g, f,
                               (const double*)c->b1p2, (const double*)c->b1,
                               (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
-                              (const R*)F<R>(c, SWAP_F_NU), dtsub, tz, c->b12,
+                              (const R*)F<R>(c, SWAP_F_NU), dtsub, tzc, c->b12,
                               last ? F<R>(c, SWAP_F_E) : (R*)nullptr,
                               last ? F<R>(c, SWAP_F_AJ) : (R*)nullptr));
   return post(c);

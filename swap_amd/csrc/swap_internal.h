@@ -100,6 +100,7 @@ struct swap_ctx {
   int64_t in_slots = 0, in_last = 0;
   uint32_t *icnt = nullptr, *ioff = nullptr, *iblk = nullptr;
   int64_t islots = 0;
+  int fz_pred = 0, fz_corr = 0;  // z chunks of the fused field launches (0: not yet chosen)
   double* dred = nullptr;     // FP64 reduction scratch
   float* dmin = nullptr;      // check_time_step minimum
   void* scratch = nullptr;    // general device scratch
