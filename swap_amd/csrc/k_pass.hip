@@ -61,9 +61,13 @@ __device__ __forceinline__ void win_add(double* a, int i, double v, int) { atomi
 // Lane L belongs to the run starting at lane s(L).  Hillis-Steele inside each
 // row of 16 lanes (row_shr 1,2,4,8), then the row totals cross rows with
 // row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3); a contribution is
-// taken only from a lane of the same run.
+// taken only from a lane of the same run.  A step whose condition holds in
+// no lane adds nothing and is skipped (lv, the same in every lane): after a
+// drift the runs are a few lanes long and most waves need only the first
+// shifts.
 struct SegCond {
   bool c1, c2, c4, c8, c15, c31;
+  unsigned lv;  // bit b: step b is needed by some lane
 };
 __device__ __forceinline__ SegCond seg_cond(int L, int s) {
   SegCond c;
@@ -73,6 +77,9 @@ __device__ __forceinline__ SegCond seg_cond(int L, int s) {
   c.c8 = L - 8 >= s;
   c.c15 = ((L >> 4) & 1) && s <= (L & ~15) - 1;
   c.c31 = (L >> 5) && s <= 31;
+  c.lv = __builtin_amdgcn_readfirstlane(
+      (__ballot(c.c1) ? 1u : 0u) | (__ballot(c.c2) ? 2u : 0u) | (__ballot(c.c4) ? 4u : 0u) |
+      (__ballot(c.c8) ? 8u : 0u) | (__ballot(c.c15) ? 16u : 0u) | (__ballot(c.c31) ? 32u : 0u));
   return c;
 }
 template <int CTRL, int ROWS>
@@ -87,16 +94,25 @@ __device__ __forceinline__ double dppf(double v) {
   int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, true);
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
-template <typename T>
-__device__ __forceinline__ T segscan(T v, const SegCond& c) {
-  T t;
-  t = dppf<0x111, 0xf>(v); v = c.c1 ? v + t : v;   // row_shr:1
-  t = dppf<0x112, 0xf>(v); v = c.c2 ? v + t : v;   // row_shr:2
-  t = dppf<0x114, 0xf>(v); v = c.c4 ? v + t : v;   // row_shr:4
-  t = dppf<0x118, 0xf>(v); v = c.c8 ? v + t : v;   // row_shr:8
-  t = dppf<0x142, 0xa>(v); v = c.c15 ? v + t : v;  // row_bcast:15
-  t = dppf<0x143, 0xc>(v); v = c.c31 ? v + t : v;  // row_bcast:31
-  return v;
+// one step of the scan on every value of a lane's sums
+template <int CTRL, int ROWS, typename T, int M>
+__device__ __forceinline__ void seg_step(T (&a)[8][M], bool cond) {
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      T t = dppf<CTRL, ROWS>(a[cc][m]);
+      a[cc][m] = cond ? a[cc][m] + t : a[cc][m];
+    }
+}
+template <typename T, int M>
+__device__ __forceinline__ void segscan(T (&a)[8][M], const SegCond& c) {
+  if (c.lv & 1u) seg_step<0x111, 0xf>(a, c.c1);    // row_shr:1
+  if (c.lv & 2u) seg_step<0x112, 0xf>(a, c.c2);    // row_shr:2
+  if (c.lv & 4u) seg_step<0x114, 0xf>(a, c.c4);    // row_shr:4
+  if (c.lv & 8u) seg_step<0x118, 0xf>(a, c.c8);    // row_shr:8
+  if (c.lv & 16u) seg_step<0x142, 0xa>(a, c.c15);  // row_bcast:15
+  if (c.lv & 32u) seg_step<0x143, 0xc>(a, c.c31);  // row_bcast:31
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -428,10 +444,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     if (hm != ~0ull) {
       const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
       const SegCond sc = seg_cond(lane, 63 - __clzll(hm & le));
-#pragma unroll
-      for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-        for (int m = 0; m < M; m++) acc[cc][m] = segscan(acc[cc][m], sc);
+      segscan(acc, sc);
     }
     const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
     if (tail && cur >= 0) {
