@@ -268,8 +268,8 @@ __global__ void k_build_src(Geo g, FC f, const R* aj, const R* up, const R* btc,
   for (int m = 0; m < 3; m++) {
     ajc[o + m * g.nn] = jc[m];
     if (upc) upc[o + m * g.nn] = uc[m];
-    S[o + m * g.nn] = jc[m] - uc[m];
-    S[o + (3 + m) * g.nn] = btc[oc + m * g.nn];
+    S[o * SREC + m] = jc[m] - uc[m];
+    S[o * SREC + 3 + m] = btc[oc + m * g.nn];
   }
 }
 
@@ -281,7 +281,7 @@ __global__ void k_update_src(Geo g, FC f, const R* up, const R* ajc, R* upc, R* 
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     if (upc) upc[o + m * g.nn] = uc[m];
-    S[o + m * g.nn] = ajc[o + m * g.nn] - uc[m];
+    S[o * SREC + m] = ajc[o + m * g.nn] - uc[m];
   }
 }
 

@@ -235,9 +235,8 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
         int a = q % T[0], b = (q / T[0]) % T[1], c = q / (T[0] * T[1]);
         int64_t node = gidx(g, xnode<OX>(g, bl[0] + a), node_of(bl[1] + b, g.ncy),
                             node_of(bl[2] + c, g.ncz));
-        gtile[2 * q] = make_float4(S[node], S[node + g.nn], S[node + 2 * g.nn],
-                                   S[node + 3 * g.nn]);
-        gtile[2 * q + 1] = make_float4(S[node + 4 * g.nn], S[node + 5 * g.nn], 0.f, 0.f);
+        gtile[2 * q] = *(const float4*)(S + node * SREC);
+        gtile[2 * q + 1] = *(const float4*)(S + node * SREC + 4);
       }
   }
   if (MODE == 3 && tid < 12) sred[tid] = (tid % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;

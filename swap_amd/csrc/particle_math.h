@@ -27,19 +27,43 @@ __device__ __forceinline__ void corner_off(const Geo& g, int64_t o[8]) {
   o[4] = sy; o[5] = sx + sy; o[6] = sy + sz; o[7] = sx + sy + sz;
 }
 
-// trilinear gather of the 6-component source S = (ajc-upc, btc)
-// (gutsp.f:217-251); A = gathered (aj - up), B = gathered btc
+// The gather source S = (ajc-upc, btc) is node-major: SREC values per node
+// (6 used, 2 zero, swap_internal.h), so a corner is two 16-byte loads in FP32.
+
+// the 6 source values of node q (node index, not scaled by SREC)
+template <typename R>
+__device__ __forceinline__ void src_node(const R* __restrict__ S, int64_t q, R v[6]) {
+  const R* p = S + q * SREC;
+#pragma unroll
+  for (int m = 0; m < 6; m++) v[m] = p[m];
+}
+template <>
+__device__ __forceinline__ void src_node<float>(const float* __restrict__ S, int64_t q,
+                                                float v[6]) {
+  const float4 a = *(const float4*)(S + q * SREC);
+  const float2 b = *(const float2*)(S + q * SREC + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y;
+}
+
+// trilinear gather of S (gutsp.f:217-251); A = gathered (aj - up), B =
+// gathered btc; the corners are summed in the reference's corner order
 template <typename R>
 __device__ __forceinline__ void gather6(const Geo& g, const R* __restrict__ S, int64_t base,
                                         const R w[8], R A[3], R B[3]) {
   int64_t off[8];
   corner_off(g, off);
+  R v[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int m = 0; m < 6; m++) {
-    const R* s = S + (int64_t)m * g.nn + base;
-    R v = s[off[0]] * w[0] + s[off[1]] * w[1] + s[off[2]] * w[2] + s[off[3]] * w[3] +
-          s[off[4]] * w[4] + s[off[5]] * w[5] + s[off[6]] * w[6] + s[off[7]] * w[7];
-    if (m < 3) A[m] = v; else B[m - 3] = v;
+  for (int k = 0; k < 8; k++) {
+    R c[6];
+    src_node(S, base + off[k], c);
+#pragma unroll
+    for (int m = 0; m < 6; m++) v[m] += c[m] * w[k];
+  }
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    A[m] = v[m];
+    B[m] = v[3 + m];
   }
 }
 

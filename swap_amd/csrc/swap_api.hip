@@ -248,7 +248,7 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   DMF(c->a, rs * 3 * nn);
   DMF(c->ct, rs * 3 * nn);
   DMF(c->btc0, rs * 3 * nn);
-  DMF(c->S, rs * 6 * nn);
+  DMF(c->S, rs * SREC * nn);  // node-major gather source
   DMF(c->macc, 8 * 4 * nn);
   c->scratch_bytes = std::max<size_t>(8 * nn, 1 << 16);
   DMF(c->scratch, c->scratch_bytes);

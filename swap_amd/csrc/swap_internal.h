@@ -14,6 +14,7 @@ constexpr int PASS_WG = 512;   // threads per particle-pass workgroup
 constexpr int PASS_KP = 4;     // consecutive particles per lane
 constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values (64 KB)
 constexpr int GTILE_NODES = 512;  // LDS gather tile, nodes of 32 B (16 KB)
+constexpr int SREC = 8;  // values per node of the gather source S (6 used, 2 zero)
 
 struct Item {
   int32_t start, end, tile, pad;
