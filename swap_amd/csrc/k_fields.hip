@@ -254,6 +254,18 @@ __global__ void k_a_from(Geo g, const R* aj, const R* up, R* a) {
   for (int m = 0; m < 3; m++) a[o + m * g.nn] = aj[oc + m * g.nn] - up[oc + m * g.nn];
 }
 
+// one node record of the gather source (6 values; the 2 pad values stay 0)
+template <typename R>
+__device__ __forceinline__ void put_src(R* p, const R v[6]) {
+#pragma unroll
+  for (int m = 0; m < 6; m++) p[m] = v[m];
+}
+template <>
+__device__ __forceinline__ void put_src<float>(float* p, const float v[6]) {
+  *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+  *(float2*)(p + 4) = make_float2(v[4], v[5]);
+}
+
 // get_Ep's node fields (gutsp.f:183-184): ajc = face_to_center(aj),
 // upc = face_to_center(up); the particle gather source is (ajc-upc, btc)
 template <typename R>
@@ -264,13 +276,15 @@ __global__ void k_build_src(Geo g, FC f, const R* aj, const R* up, const R* btc,
   f2c3(g, f, aj, ci, cj, ck, jc);
   f2c3(g, f, up, ci, cj, ck, uc);
   int64_t oc = gidx(g, ci, cj, ck);
+  R rec[6];
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     ajc[o + m * g.nn] = jc[m];
     if (upc) upc[o + m * g.nn] = uc[m];
-    S[o * SREC + m] = jc[m] - uc[m];
-    S[o * SREC + 3 + m] = btc[oc + m * g.nn];
+    rec[m] = jc[m] - uc[m];
+    rec[3 + m] = btc[oc + m * g.nn];
   }
+  put_src(S + o * SREC, rec);
 }
 
 template <typename R>
