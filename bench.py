@@ -7,8 +7,8 @@ particle state (split-precision positions), FP64 B advance.  A "step" is one
 full swap_step: the three particle passes, the counting sort when due, and
 all field subcycles.  Particles are generated on the device before timing.
 
-Prints one JSON line (the driver's contract) with the roofline of the
-dominant kernel and a CPU baseline (the FP64 oracle on a bounded sample).
+Prints one JSON line with the roofline of the dominant kernel and a CPU
+baseline (the FP64 oracle on a bounded sample).
 
 Other workloads: --workload pickup (a neutral cloud injecting 28-amu ions
 before every step), --precision 64 (the same kernels in FP64), --cells/--ppc

@@ -29,8 +29,8 @@ def initparameters(b0_init=0.2e-9, ion_amu=1, nf_init=0.1e15, dt_frac=0.05, dx_f
 
 
 def workload(ncx, ncy, ncz, ppc, vsw=400.0, vth=30.0, **kw):
-    """Uniform drifting-Maxwellian proton run: BASELINE's first two configs
-    (64x32x32 x 8 ppc, and 256x128x128 x 64 ppc).  beta = N_p / (V n0) makes
+    """Uniform drifting-Maxwellian proton run, e.g. 64x32x32 cells x 8 ppc
+    or 256x128x128 cells x 64 ppc.  beta = N_p / (V n0) makes
     the deposited density n0 (the explicit form of get_beta, misc.f:462-475)."""
     p = initparameters(**kw)
     npart = ncx * ncy * ncz * ppc
