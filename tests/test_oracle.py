@@ -681,3 +681,54 @@ def test_chex_draw_follows_the_ion_id():
     assert np.array_equal(hp, h[perm]) and 0 < h.sum() < n
     hi = O.chex(cell[perm], f[:, perm], v[:, perm], d, p["dt"], cl, 9, 2)
     assert not np.array_equal(hi, h[perm])
+
+
+@pytest.mark.parametrize("pol", [1, -1])
+def test_field_solve_whistler_dispersion(pol):
+    """ions at rest (up = 0), uniform n, nu = 0: the subcycle (curlB,
+    predict_B, correct_B, f_update_tlev; maind.f:430-476) advances electron
+    MHD, where a circularly polarised wave along B0 turns at
+    omega = k^2 B0 / (alpha n) and keeps its amplitude.  At k dz = 2 pi / 32
+    the discrete curls and averages move omega by well under 1 %; a dropped
+    Hall term, a wrong sign or a transposed cross product fails."""
+    nc = (4, 4, 32)
+    o = make(nc=nc)
+    n0, b0, alpha, dz = 1e14, 0.019161675, PAR["alpha"], PAR["dx"]
+    k = 2 * math.pi / (nc[2] * dz)
+    omega = k * k * b0 / (alpha * n0)
+    w_grid = (math.pi / dz) ** 2 * b0 / (alpha * n0)  # the grid-scale whistler
+    dts = 0.5 / w_grid
+    nsub = int(round(1.0 / (omega * dts)))             # about one radian of phase
+    o.set_dtsub(dts, nsub)
+    zc = o.coord("qz") + 0.5 * dz                       # Bx, By sit at k + 1/2 in z
+    amp = 1e-4 * b0
+    b1 = o.grid()
+    b1[2] = b0
+    b1[0] += amp * np.cos(k * zc)[:, None, None]
+    b1[1] += pol * amp * np.sin(k * zc)[:, None, None]
+    b12, b1p2, bt = b1.copy(), b1.copy(), b1.copy()
+    E, aj, up = o.grid(), o.grid(), o.grid()
+    npg = np.full(o.grid(1).shape, n0)
+    nu = np.zeros(o.grid(1).shape)
+    inner = slice(1, nc[2] + 1)                         # the unique z layers
+
+    def mode():
+        psi = (b1[0] + 1j * pol * b1[1])[inner, 2, 2]
+        return np.mean(psi * np.exp(-1j * k * zc[inner]))
+
+    p0 = mode()
+    for _ in range(nsub):
+        o.curlB(bt, npg, aj)
+        o.predict_B(b1, b12, b1p2, bt, E, aj, up, npg, nu)
+        o.correct_B(b1, b1p2, E, aj, up, npg, nu)
+        o.f_update_tlev(b1, b12, b1p2, bt)
+    p1 = mode()
+    # linearised: d(bx, by)/dt = omega (-by, bx) for E = aj x B, aj = curl B /
+    # (alpha n), so bx + i pol by turns as exp(i pol omega t)
+    w_meas = np.angle(p1 / p0) / (nsub * dts)
+    assert w_meas == pytest.approx(pol * omega, rel=1e-2)
+    assert abs(p1) / abs(p0) == pytest.approx(1.0, abs=1e-2)
+    # the wave stays a single mode along z: no other harmonic grows
+    psi = (b1[0] + 1j * pol * b1[1])[inner, 2, 2]
+    rest = psi - p1 * np.exp(1j * k * zc[inner]) - np.mean(psi - p1 * np.exp(1j * k * zc[inner]))
+    assert np.max(np.abs(rest)) < 1e-2 * amp
