@@ -22,9 +22,9 @@
 // and the additions degrade to one per particle.  The window covers the
 // item's deposit footprint and is added to HBM once per item.  The window of P1/P2
 // is the item's cell box at x^n; P3's is that box widened by the half-step
-// displacement range from signed per-axis velocity bounds (reduced by the
-// previous P3).  Particles outside the window deposit straight to HBM
-// (counted as misses).
+// displacement range from signed per-axis velocity bounds of the item's tile
+// (reduced by the previous P3).  Particles outside the window deposit
+// straight to HBM (counted as misses).
 //
 // Right after a sort, P2 and P3 gather the 6-component node source
 // S = (ajc - upc, btc) from an LDS copy of the item's node box (one 32-byte
@@ -174,11 +174,13 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   int64_t i0, i1;
   int bl[3] = {0, 0, 0}, bh[3] = {-1, -1, -1};
   bool windowed = false;
+  int tile = -1;
   const int w = blockIdx.x;
   if (w < nitems) {
     Item it = items[w];
     i0 = it.start;
     i1 = it.end;
+    tile = it.tile;
     const int* b = boxes + 6 * w;
     for (int a = 0; a < 3; a++) {
       bl[a] = b[a];
@@ -201,17 +203,24 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   if (MODE == 3 && windowed) {
     // the cell shift of a half move by d in [dlo, dhi] cells lies in
     // [floor(dlo), ceil(dhi)]; 10 % of the largest speed covers the change of
-    // v within one step (rotation by Omega dt and the E x B acceleration)
+    // v within one step (rotation by Omega dt and the E x B acceleration).
+    // The bounds are those of the item's tile at the previous P3 (the grid's
+    // when the tile had no particles then), so that fast ions (pickup ions
+    // near a source, say) widen only the windows of their own region; the
+    // range is clamped to WIN_SHIFT cells.  Particles outside the window
+    // deposit straight to HBM.
+    const int* tb = vb + 6 * (tile + 1);
+    if (!(tb[0] <= tb[3] && tb[1] <= tb[4] && tb[2] <= tb[5])) tb = vb;
     float vmax_abs = 0.f;
     for (int a = 0; a < 3; a++)
-      vmax_abs = fmaxf(vmax_abs, fmaxf(-fval(vb[a]), fval(vb[3 + a])));
+      vmax_abs = fmaxf(vmax_abs, fmaxf(-fval(tb[a]), fval(tb[3 + a])));
     float sl = 0.1f * vmax_abs;
     for (int a = 0; a < 3; a++) {
-      float dlo = (float)K.s[a] * (fval(vb[a]) - sl), dhi = (float)K.s[a] * (fval(vb[3 + a]) + sl);
-      if (!(dlo >= -64.f && dhi <= 64.f)) windowed = false;  // unset or absurd bounds
+      float dlo = (float)K.s[a] * (fval(tb[a]) - sl), dhi = (float)K.s[a] * (fval(tb[3 + a]) + sl);
+      if (!(dlo <= dhi)) windowed = false;  // unset bounds (NaN)
       else {
-        lo_m[a] = (int)floorf(dlo);
-        hi_m[a] = (int)ceilf(dhi);
+        lo_m[a] = (int)floorf(fminf(fmaxf(dlo, -WIN_SHIFT), WIN_SHIFT));
+        hi_m[a] = (int)ceilf(fminf(fmaxf(dhi, -WIN_SHIFT), WIN_SHIFT));
       }
     }
   }
@@ -457,6 +466,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   if (misses) atomicAdd(dcnt, misses);
   if (MODE == 3) {
     // the next step's windows: cell box of x^{n+1} and signed velocity bounds
+    // (the grid's and the item's tile's)
     for (int a = 0; a < 3; a++) {
       int m0 = wave_min(bmin[a]), m1 = wave_max(bmax[a]);
       int k0 = wave_min(fkey(vlo[a])), k1 = wave_max(fkey(vhi[a]));
@@ -471,8 +481,14 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   __syncthreads();
   if (MODE == 3 && tid < 6) {
     if (w < nitems) {
-      if (tid < 3) atomicMin(boxes_next + 6 * w + tid, sred[tid]);
-      else atomicMax(boxes_next + 6 * w + tid, sred[tid]);
+      int* tn = vb_next + 6 * (tile + 1);
+      if (tid < 3) {
+        atomicMin(boxes_next + 6 * w + tid, sred[tid]);
+        atomicMin(tn + tid, sred[6 + tid]);
+      } else {
+        atomicMax(boxes_next + 6 * w + tid, sred[tid]);
+        atomicMax(tn + tid, sred[6 + tid]);
+      }
     }
     if (tid < 3) atomicMin(vb_next + tid, sred[6 + tid]);
     else atomicMax(vb_next + tid, sred[6 + tid]);
@@ -491,9 +507,9 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   }
 }
 
-__global__ void k_pass_reset(int64_t nitems, int* boxes_next, int* vb_next) {
+__global__ void k_pass_reset(int64_t nitems, int64_t nvb, int* boxes_next, int* vb_next) {
   int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (q < 6) vb_next[q] = q < 3 ? 0x7fffffff : -0x7fffffff - 1;
+  if (q < nvb) vb_next[q] = (q % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;
   if (q >= nitems * 6) return;
   boxes_next[q] = (q % 6) < 3 ? 0x7fffffff : -0x7fffffff;
 }
@@ -507,9 +523,9 @@ int post(swap_ctx* c) {
 
 // reset the box/velocity-bound outputs of P3
 int pl_box_reset(swap_ctx* c) {
-  int64_t n6 = std::max<int64_t>(c->max_items * 6, 6);
+  int64_t n6 = std::max<int64_t>(c->max_items * 6, vb_ints(c));
   hipLaunchKernelGGL(k_pass_reset, dim3(nblk(n6, 256)), dim3(256), 0, c->stream, c->max_items,
-                     c->boxes_next, c->vb_next);
+                     vb_ints(c), c->boxes_next, c->vb_next);
   return post(c);
 }
 

@@ -687,13 +687,17 @@ __global__ void k_vbounds(int64_t n, PDev<R> P, int* out) {
     }
   }
 }
-__global__ void k_vbounds_init(int* out) {
-  if (threadIdx.x < 6) out[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : -0x7fffffff - 1;
+// empties the grid's bounds and every tile's (a pass then sizes its windows
+// from the grid's bounds until a P3 has reduced the tiles')
+__global__ void k_vbounds_init(int64_t nvb, int* out) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < nvb) out[q] = (q % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;
 }
 }  // namespace
 
 int sr_vbounds(swap_ctx* c) {
-  hipLaunchKernelGGL(k_vbounds_init, dim3(1), dim3(64), 0, c->stream, c->vb);
+  hipLaunchKernelGGL(k_vbounds_init, dim3(nblk(vb_ints(c), 256)), dim3(256), 0, c->stream,
+                     vb_ints(c), c->vb);
   if (c->n > 0)
     SDISPATCH(c, hipLaunchKernelGGL((k_vbounds<R>), dim3(1024), dim3(256), 0, c->stream, c->n,
                                     pdev<R>(c), c->vb));

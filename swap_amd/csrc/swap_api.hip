@@ -293,8 +293,9 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   DMF(c->scur, 4 * (size_t)(c->nkeys + 2));
   DMF(c->snrun, 4 * (size_t)(c->nkeys + 2));
   DMF(c->srstart, 4 * (size_t)(c->nkeys + 2));
-  DMF(c->vb, 16 * sizeof(int));
-  c->vb_next = c->vb + 8;
+  DMF(c->vb_mem, 2 * vb_ints(c) * sizeof(int));
+  c->vb = c->vb_mem;
+  c->vb_next = c->vb_mem + vb_ints(c);
   DMF(c->d_zero, 4 * sizeof(int));
   DMF(c->sblk, 4 * (size_t)(c->nkeys / 4096 + 4));
   c->max_items = c->ntiles + c->cap / ITEM_MAX + 1;
@@ -320,7 +321,7 @@ int swap_destroy(swap_ctx* c) {
                 c->sruns, c->srbase, c->segkey, c->seginfo, c->segbase, c->nsegc, c->scnt,
                 c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items, c->ccnt,
                 c->coff, c->cbp, c->icnt, c->ioff, c->iblk,
-                c->boxes, c->boxes_next, c->vb, c->d_zero};
+                c->boxes, c->boxes_next, c->vb_mem, c->d_zero};
   for (void* p : ps) dfree(p);
   for (int f = 0; f < SWAP_F_COUNT; f++) dfree(c->fld[f]);
   for (int a = 0; a < 9; a++) {

@@ -13,6 +13,7 @@ constexpr int ITEM_MAX = 32768;
 constexpr int PASS_WG = 512;   // threads per particle-pass workgroup
 constexpr int PASS_KP = 4;     // consecutive particles per lane
 constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values (64 KB)
+constexpr float WIN_SHIFT = 3.f;  // largest half-move cell shift a P3 window covers
 constexpr int GTILE_NODES = 512;  // LDS gather tile, nodes of 32 B (16 KB)
 constexpr int SREC = 8;  // values per node of the gather source S (6 used, 2 zero)
 
@@ -72,7 +73,9 @@ struct swap_ctx {
   uint32_t* srbase = nullptr; // per sorted chunk position: source | segment << 12
   uint32_t *segkey = nullptr, *seginfo = nullptr, *segbase = nullptr;  // per chunk segment
   uint32_t* nsegc = nullptr;  // segments per chunk
-  int *vb = nullptr, *vb_next = nullptr;  // signed velocity bounds, float keys
+  // signed velocity bounds (float keys): 6 for the whole grid, then 6 per
+  // tile; vb and vb_next are the two halves of vb_mem and swap every step
+  int *vb = nullptr, *vb_next = nullptr, *vb_mem = nullptr;
   bool vb_valid = false;
   int* d_zero = nullptr;
   Item* items = nullptr;
@@ -113,6 +116,9 @@ struct swap_ctx {
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev[SWAP_K_COUNT];  // start, stop, start, stop ...
 };
+
+// ints in each half of vb_mem: the grid's 6 bounds, then 6 per tile
+inline int64_t vb_ints(const swap_ctx* c) { return 6 * (c->ntiles + 1); }
 
 // error helpers
 int set_err(swap_ctx* c, int code, const char* fmt, ...);
