@@ -86,9 +86,10 @@ def main():
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--sort-every", type=int, default=2)
     ap.add_argument("--precision", type=int, default=32)
-    ap.add_argument("--workload", choices=("uniform", "pickup"), default="uniform",
+    ap.add_argument("--workload", choices=("uniform", "pickup", "outflow"), default="uniform",
                     help="pickup: a neutral cloud at the domain centre injects 28-amu ions "
-                         "(photoionization, charge exchange) before every step")
+                         "(photoionization, charge exchange) before every step; outflow: "
+                         "inflow/outflow x with the upstream plasma flowing in at x = 0")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates HBM byte counters)")
@@ -111,23 +112,29 @@ def main():
     npart = p["npart"]
     nn = (ncx + 2) * (ncy + 2) * (ncz + 2)
     pickup = args.workload == "pickup"
+    outflow = args.workload == "outflow"
     cloud = physics.cloud(p) if pickup else None
-    # the cloud adds a few tenths of a percent of ions per step
-    cap = npart + (npart // 50 + 2000000) * pickup
+    # the cloud adds a few tenths of a percent of ions per step; an open x
+    # holds this step's entrants before the sort drops the ions that left
+    cap = npart + (npart // 50 + 2000000) * pickup + (npart // 10) * outflow
+    flags = (api.SPECIES if pickup else 0) | (api.OUTFLOW_X if outflow else 0)
     s = physics.make_swap(p, capacity=cap, precision=args.precision,
-                          sort_every=args.sort_every, flags=api.SPECIES if pickup else 0)
+                          sort_every=args.sort_every, flags=flags)
     s.load_maxwellian(npart, p["vsw"], p["vth"])
+    if outflow:
+        s.set_inflow(args.ppc, p["vsw"], p["vth"])
     print("bench: loaded %d particles on %dx%dx%d cells" % (npart, ncx, ncy, ncz),
           file=sys.stderr, flush=True)
 
     def run(k, step0):
         """k steps; returns the particle-steps done"""
-        if not pickup:
+        if not pickup and not outflow:
             s.step(k)
             return s.n * k
         done = 0
         for i in range(k):
-            s.ionize_cloud(step0 + i, cloud)  # Ionize_pluto_mp before the step
+            if pickup:
+                s.ionize_cloud(step0 + i, cloud)  # Ionize_pluto_mp before the step
             done += s.n
             s.step(1)
         return done
@@ -162,18 +169,23 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32" if args.precision == 32 else "fp64",
         "data": "synthetic: seeded drifting Maxwellian (Philox), uniform B0" +
-                (", neutral-cloud pickup ions" if pickup else ""),
+                (", neutral-cloud pickup ions" if pickup else "") +
+                (", inflow at x = 0" if outflow else ""),
         "config": {"workload": "%dx%dx%d cells x %d ppc (%d %s), 10 B subcycles/step, "
-                               "periodic, %s particles, FP64 B advance"
+                               "%s, %s particles, FP64 B advance"
                                % (ncx, ncy, ncz, args.ppc, npart,
-                                  "ions incl. cloud pickup ions (mean)" if pickup else "protons",
+                                  "ions incl. cloud pickup ions (mean)" if pickup else
+                                  "protons (mean)" if outflow else "protons",
+                                  "inflow/outflow x, periodic y z" if outflow else "periodic",
                                   "split-precision FP32" if args.precision == 32
                                   else "FP64"),
                    "cells": [ncx, ncy, ncz], "ppc": args.ppc, "n_particles": npart,
                    "sort_every": args.sort_every, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, tcfg),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kname, tcfg) if args.workload == "uniform"
+                     else None,
                      "alg_bytes_per_launch": alg,
                      "avg_launch_ms": ms_tot / cnt},
         "step_roofline": {"alg_bytes": step_alg, "achieved_GBs": step_alg / (ms_step * 1e-3) /
