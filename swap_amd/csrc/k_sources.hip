@@ -253,6 +253,25 @@ __global__ void k_inflow_fill(Geo g, uint32_t nslot, int64_t n0, PDev<R> P, Infl
   }
 }
 
+// The inflow ions' velocities lie in [lo, hi] per axis (the Box-Muller draw
+// has u1 >= 2^-24, so |v - vsw xhat| <= 4.0794 vth).  Folded into the bounds
+// P3 reduced (the grid's, and those of the tiles in the first ntxi tile
+// columns along x, which hold every cell an entering ion reaches), they keep
+// the next P3's windows valid.
+struct VBox {
+  float lo[3], hi[3];
+};
+__global__ void k_inflow_bounds(int* vbn, int ntx, int ntxi, int ntyz, VBox b) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > ntxi * ntyz) return;
+  int* t = q == ntxi * ntyz ? vbn  // the grid, or tile (q % ntxi, q / ntxi)
+                            : vbn + 6 * ((int64_t)ntx * (q / ntxi) + q % ntxi + 1);
+  for (int a = 0; a < 3; a++) {
+    atomicMin(t + a, pm::fkey(b.lo[a]));
+    atomicMax(t + 3 + a, pm::fkey(b.hi[a]));
+  }
+}
+
 // ---- FP64 reductions ----
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -770,8 +789,19 @@ int sr_inflow_step(swap_ctx* c) {
   c->next_id += tot;
   c->injected += tot;
   c->in_last = tot;
-  c->vb_valid = false;  // the new velocities may lie outside the pass windows' bounds
-  return 0;
+  const double a = 4.08 * c->in_vth;
+  const double lo[3] = {c->in_vsw - a, -a, -a}, hi[3] = {c->in_vsw + a, a, a};
+  VBox b;
+  for (int q = 0; q < 3; q++) {  // one float step outward covers the rounding to R
+    b.lo[q] = std::nextafter((float)lo[q], -INFINITY);
+    b.hi[q] = std::nextafter((float)hi[q], INFINITY);
+  }
+  // entering ions end the step in cells [0, L) of x
+  const int ntyz = c->ntile[1] * c->ntile[2];
+  const int ntxi = std::min(c->ntile[0], (c->in_L + TILE_X - 1) / TILE_X);
+  hipLaunchKernelGGL(k_inflow_bounds, dim3(nblk(ntxi * ntyz + 1, 256)), dim3(256), 0, c->stream,
+                     c->vb_next, c->ntile[0], ntxi, ntyz, b);
+  return post(c);
 }
 
 // the input energy account (Energy_diag's input_E, global.f:47): ions that

@@ -889,7 +889,7 @@ int swap_step(swap_ctx* c, int nsteps) {
     }
     // P3 reduced the cell boxes of x^{n+1} and the bounds of v^{n+1/2}
     std::swap(c->vb, c->vb_next);
-    c->vb_valid = c->n > 0 && c->in_last == 0;  // inflow ions were not in P3's bounds
+    c->vb_valid = c->n > 0;  // the inflow ions' bounds were folded in
     if (c->n_sorted >= 0) {
       std::swap(c->boxes, c->boxes_next);
       c->steps_since_sort++;
