@@ -241,7 +241,8 @@ int swap_ionize_cloud(swap_ctx *ctx, int step, const swap_cloud *cloud, int64_t 
    cell, bulk vsw along x, thermal vth as swap_load_maxwellian draws it)
    that cross the face during the step (the exact kinetic flux): candidates
    fill a slab of L = ceil((|vsw| + 4.08 vth) dt/dx) cells in front of the
-   face at the start of the step and make the step's two half moves; one
+   face at the start of the step, round(ppc L) in each y-z column of cells,
+   and make the step's two half moves; one
    inside after the first deposits into the step's moments at x^{n+1/2},
    one inside after the second is appended with vp1 = vp, mrat = 1 and
    beta_p as given, its id continuing the id counter.  The step count keys

@@ -1000,9 +1000,11 @@ void or_chex(int64_t n, const uint32_t *cell, const float *fx, const float *fy, 
 /* the inflow candidates of one step (build-defined inflow through the open
    x face; cf. check_min_den_boundary gutsp.f:575-816, which refills a
    boundary layer instead): slab depth L = max(1, ceil((|vsw| + 4.08 vth)
-   dt/dx)) cells, nslot = round(ppc L ncy ncz) candidates; slot s draws
-   Philox counter (s, step, 0x50000, d) d = 0..2 keyed by the seed, start
-   position x = (1-u) L - L, y = (1-u) ncy, z = (1-u) ncz (cell units),
+   dt/dx)) cells, spc = round(ppc L) candidates per y-z column of the face
+   and nslot = spc ncy ncz; slot s belongs to column c = s / spc (y index
+   c % ncy, z index c / ncy), draws Philox counter (s, step, 0x50000, d)
+   d = 0..2 keyed by the seed, start position x = (1-u) L - L,
+   y = jy + (1-u), z = jz + (1-u) (cell units, inside its column),
    velocity as the loader, then two half moves t += v dt/(2 d) (explicit fma;
    y and z wrapped into (0, nc]).  flag bit 0: inside after the first half
    move (deposits at x^{n+1/2}); bit 1: inside after the second and not past
@@ -1026,7 +1028,8 @@ int64_t or_inflow(uint64_t seed, uint32_t step, const int nc[3], double dt, cons
   if (!(reach < 1048576.0)) return -1;
   int L = (int)ceil(reach);
   if (L < 1) L = 1;
-  double ns = round(ppc * (double)L * (double)nc[1] * (double)nc[2]);
+  double spc = round(ppc * (double)L);
+  double ns = spc * (double)nc[1] * (double)nc[2];
   if (!(ns < 4.0e9)) return -1;
   uint32_t nslot = (uint32_t)ns;
   int64_t m = 0;
@@ -1044,8 +1047,9 @@ int64_t or_inflow(uint64_t seed, uint32_t step, const int nc[3], double dt, cons
       vv[a] = (a == 0 ? vsw : 0.0) + vth * sqrt(-log(u1)) * cos(3.14159 * u2);
     }
     t0[0] = (1.0 - or_u24(r[0][0])) * (double)L - (double)L;
-    t0[1] = (1.0 - or_u24(r[0][1])) * (double)nc[1];
-    t0[2] = (1.0 - or_u24(r[0][2])) * (double)nc[2];
+    uint32_t col = q / (uint32_t)spc;
+    t0[1] = (double)(col % (uint32_t)nc[1]) + (1.0 - or_u24(r[0][1]));
+    t0[2] = (double)(col / (uint32_t)nc[1]) + (1.0 - or_u24(r[0][2]));
     for (int a = 0; a < 3; a++) {
       a1[a] = fma(vv[a], sh[a], t0[a]);
       a2[a] = fma(vv[a], sh[a], a1[a]);

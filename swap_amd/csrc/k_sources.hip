@@ -151,6 +151,7 @@ __global__ void k_ionize_sw(int64_t n, int64_t off, PDev<R> P, uint32_t k0, uint
 struct InflowDev {
   double vsw, vth, sh[3];  // sh: dt / 2 / cell size per axis
   int L, nc[3];
+  uint32_t spc;  // slots per y-z column of the face
 };
 struct InflowIon {
   double th[3], te[3], v[3];  // cell coordinates after each half move
@@ -184,8 +185,9 @@ __device__ __forceinline__ InflowIon inflow_slot(const InflowDev& d, uint32_t sl
   }
   double t0[3];
   t0[0] = (1.0 - u24(r[0][0])) * (double)d.L - (double)d.L;  // (-L, 0], exact
-  t0[1] = (1.0 - u24(r[0][1])) * (double)d.nc[1];
-  t0[2] = (1.0 - u24(r[0][2])) * (double)d.nc[2];
+  const uint32_t col = slot / d.spc;  // slots of one y-z column are consecutive
+  t0[1] = (double)(col % (uint32_t)d.nc[1]) + (1.0 - u24(r[0][1]));
+  t0[2] = (double)(col / (uint32_t)d.nc[1]) + (1.0 - u24(r[0][2]));
   for (int a = 0; a < 3; a++) {
     o.th[a] = fma(o.v[a], d.sh[a], t0[a]);
     o.te[a] = fma(o.v[a], d.sh[a], o.th[a]);
@@ -213,43 +215,93 @@ __global__ void k_inflow_count(uint32_t nslot, InflowDev d, uint32_t step, uint3
   cnt[q] = inflow_slot(d, q, step, k0, k1, stream).keep ? 1u : 0u;
 }
 
+// One workgroup takes the slots of a patch of IN_PY x IN_PZ face columns.
+// The half-step deposits of its entrants land within a few cells of the
+// face and of the patch, so they are summed in an LDS window (x nodes
+// [0, XW), y and z nodes from the patch start minus the thermal reach) and
+// the window is added to the moments once; a corner outside the window
+// (or every corner, when the window exceeds IN_WIN values) goes straight to
+// HBM.  Fixed-point sums make the result independent of that split.
+constexpr int IN_PY = 4, IN_PZ = 4, IN_WIN = 4096;
+struct InflowWin {
+  int XW, WY, WZ, ey, ez;
+  bool on;
+};
+__device__ __forceinline__ void in_win_add(unsigned long long* a, int i, float v, int bits) {
+  atomicAdd(a + i, to_fix(v, bits));
+}
+__device__ __forceinline__ void in_win_add(double* a, int i, double v, int) { atomicAdd(a + i, v); }
+
 template <typename R>
-__global__ void k_inflow_fill(Geo g, uint32_t nslot, int64_t n0, PDev<R> P, InflowDev d,
-                              uint32_t step, uint32_t k0, uint32_t k1, uint32_t stream,
-                              const uint32_t* off, float bpv, float* bp, float* mr, uint32_t* id,
-                              uint32_t id0, typename AccOf<R>::T* acc) {
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nslot) return;
-  const InflowIon o = inflow_slot(d, q, step, k0, k1, stream);
-  if (o.dep) {  // the pass's moments at x^{n+1/2}: 1/beta_p and v/beta_p
-    double f[3];
-    int ijk[3];
-    for (int a = 0; a < 3; a++) ijk[a] = split_t(o.th[a], f[a]);
-    R w[8];
-    pm::weights8((R)f[0], (R)f[1], (R)f[2], w);
-    const R rr = (R)1 / (R)bpv;
-    const R qm[4] = {rr, rr * (R)o.v[0], rr * (R)o.v[1], rr * (R)o.v[2]};
+__global__ __launch_bounds__(256) void k_inflow_fill(
+    Geo g, InflowWin win, int64_t n0, PDev<R> P, InflowDev d, uint32_t step, uint32_t k0,
+    uint32_t k1, uint32_t stream, const uint32_t* off, float bpv, float* bp, float* mr,
+    uint32_t* id, uint32_t id0, typename AccOf<R>::T* acc) {
+  using A = typename AccOf<R>::T;
+  __shared__ A lacc[IN_WIN];
+  const int npy = (d.nc[1] + IN_PY - 1) / IN_PY;
+  const int py0 = (int)(blockIdx.x % npy) * IN_PY, pz0 = (int)(blockIdx.x / npy) * IN_PZ;
+  const int by = py0 - win.ey, bz = pz0 - win.ez;  // window origin in y and z
+  const int nwin = win.on ? win.XW * win.WY * win.WZ * 4 : 0;
+  const int nwv = nwin / 4;
+  for (int i = threadIdx.x; i < nwin; i += blockDim.x) lacc[i] = (A)0;
+  __syncthreads();
+  const uint32_t nps = (uint32_t)(IN_PY * IN_PZ) * d.spc;
+  for (uint32_t sidx = threadIdx.x; sidx < nps; sidx += blockDim.x) {
+    const uint32_t cp = sidx / d.spc;
+    const int jy = py0 + (int)(cp % IN_PY), jz = pz0 + (int)(cp / IN_PY);
+    if (jy >= d.nc[1] || jz >= d.nc[2]) continue;
+    const uint32_t q = ((uint32_t)jz * (uint32_t)d.nc[1] + (uint32_t)jy) * d.spc + (sidx - cp * d.spc);
+    const InflowIon o = inflow_slot(d, q, step, k0, k1, stream);
+    if (o.dep) {  // the pass's moments at x^{n+1/2}: 1/beta_p and v/beta_p
+      double f[3];
+      int ijk[3];
+      for (int a = 0; a < 3; a++) ijk[a] = split_t(o.th[a], f[a]);
+      R w[8];
+      pm::weights8((R)f[0], (R)f[1], (R)f[2], w);
+      const R rr = (R)1 / (R)bpv;
+      const R qm[4] = {rr, rr * (R)o.v[0], rr * (R)o.v[1], rr * (R)o.v[2]};
+      const int dy = pm::wrapc(ijk[1] - by, g.ncy), dz = pm::wrapc(ijk[2] - bz, g.ncz);
+      const bool inw = win.on && ijk[0] + 1 < win.XW && dy + 1 < win.WY && dz + 1 < win.WZ;
 #pragma unroll
-    for (int cc = 0; cc < 8; cc++) {
-      int64_t node = gidx(g, node_of_x(g, ijk[0] + (cc & 1)), node_of(ijk[1] + ((cc >> 2) & 1), g.ncy),
-                          node_of(ijk[2] + ((cc >> 1) & 1), g.ncz));
+      for (int cc = 0; cc < 8; cc++) {
+        const int cx = cc & 1, cy = (cc >> 2) & 1, cz = (cc >> 1) & 1;
+        if (inw) {
+          const int wi = ((dz + cz) * win.WY + dy + cy) * win.XW + ijk[0] + cx;
 #pragma unroll
-      for (int m = 0; m < 4; m++)
-        acc_add(acc, node + (int64_t)m * g.nn, w[cc] * qm[m], fix_bits(m));
+          for (int m = 0; m < 4; m++) in_win_add(lacc, wi + m * nwv, w[cc] * qm[m], fix_bits(m));
+        } else {
+          int64_t node = gidx(g, node_of_x(g, ijk[0] + cx), node_of(ijk[1] + cy, g.ncy),
+                              node_of(ijk[2] + cz, g.ncz));
+#pragma unroll
+          for (int m = 0; m < 4; m++)
+            acc_add(acc, node + (int64_t)m * g.nn, w[cc] * qm[m], fix_bits(m));
+        }
+      }
+    }
+    if (o.keep) {
+      double f[3];
+      int ijk[3];
+      for (int a = 0; a < 3; a++) ijk[a] = split_t(o.te[a], f[a]);
+      int64_t p = n0 + off[q];
+      P.cell[p] = pack_cell(ijk[0], ijk[1], ijk[2]);
+      P.fx[p] = (R)f[0]; P.fy[p] = (R)f[1]; P.fz[p] = (R)f[2];
+      P.vx[p] = (R)o.v[0]; P.vy[p] = (R)o.v[1]; P.vz[p] = (R)o.v[2];
+      P.v1x[p] = (R)o.v[0]; P.v1y[p] = (R)o.v[1]; P.v1z[p] = (R)o.v[2];
+      if (bp) bp[p] = bpv;
+      if (mr) mr[p] = 1.0f;
+      if (id) id[p] = id0 + off[q];
     }
   }
-  if (o.keep) {
-    double f[3];
-    int ijk[3];
-    for (int a = 0; a < 3; a++) ijk[a] = split_t(o.te[a], f[a]);
-    int64_t p = n0 + off[q];
-    P.cell[p] = pack_cell(ijk[0], ijk[1], ijk[2]);
-    P.fx[p] = (R)f[0]; P.fy[p] = (R)f[1]; P.fz[p] = (R)f[2];
-    P.vx[p] = (R)o.v[0]; P.vy[p] = (R)o.v[1]; P.vz[p] = (R)o.v[2];
-    P.v1x[p] = (R)o.v[0]; P.v1y[p] = (R)o.v[1]; P.v1z[p] = (R)o.v[2];
-    if (bp) bp[p] = bpv;
-    if (mr) mr[p] = 1.0f;
-    if (id) id[p] = id0 + off[q];
+  __syncthreads();
+  for (int i = threadIdx.x; i < nwv; i += blockDim.x) {
+    const int ux = i % win.XW, uy = (i / win.XW) % win.WY, uz = i / (win.XW * win.WY);
+    const int64_t node = gidx(g, node_of_x(g, ux), node_of(by + uy, g.ncy), node_of(bz + uz, g.ncz));
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const A v = lacc[i + m * nwv];
+      if (v != (A)0) atomicAdd(acc + node + (int64_t)m * g.nn, v);
+    }
   }
 }
 
@@ -733,7 +785,8 @@ int sr_set_inflow(swap_ctx* c, double ppc, double vsw, double vth, double beta_p
   double reach = (fabs(vsw) + 4.08 * vth) * c->cfg.dt / c->cfg.dx;
   if (!(reach < 1048576.0)) return set_err(c, SWAP_EINVAL, "set_inflow: slab too deep");
   int L = std::max(1, (int)ceil(reach));
-  double ns = std::round(ppc * (double)L * (double)c->cfg.ncy * (double)c->cfg.ncz);
+  // round(ppc L) slots per y-z column of the face, the columns' slots consecutive
+  double ns = std::round(ppc * (double)L) * (double)c->cfg.ncy * (double)c->cfg.ncz;
   if (!(ns < 4.0e9)) return set_err(c, SWAP_EINVAL, "set_inflow: too many slots");
   c->in_ppc = ppc;
   c->in_vsw = vsw;
@@ -756,6 +809,7 @@ int sr_inflow_step(swap_ctx* c) {
   d.sh[2] = 0.5 * c->cfg.dt / c->cfg.dz;
   d.L = c->in_L;
   d.nc[0] = c->cfg.ncx; d.nc[1] = c->cfg.ncy; d.nc[2] = c->cfg.ncz;
+  d.spc = (uint32_t)(nslot / ((uint64_t)c->cfg.ncy * (uint64_t)c->cfg.ncz));
   if ((int64_t)nslot > c->islots) {
     for (uint32_t** b : {&c->icnt, &c->ioff, &c->iblk}) {
       if (*b) HIPCHK(c, hipFree(*b));
@@ -779,10 +833,21 @@ int sr_inflow_step(swap_ctx* c) {
   if (c->n + (int64_t)tot > c->cap)
     return set_err(c, SWAP_ECAP, "inflow: %lld + %u ions exceed capacity %lld",
                    (long long)c->n, tot, (long long)c->cap);
-  SDISPATCH(c, hipLaunchKernelGGL((k_inflow_fill<R>), dim3(nblk(nslot, 256)), dim3(256), 0,
-                                  c->stream, c->g, nslot, c->n, pdev<R>(c), d, step, k0, k1,
-                                  stream, c->ioff, (float)c->in_bp, c->bp, c->mr, c->id,
-                                  (uint32_t)c->next_id, (typename AccOf<R>::T*)c->macc));
+  // deposit window of a patch: x nodes up to the half-step reach (th <= L/2
+  // cells), y and z nodes of the patch plus the thermal reach of a half move
+  InflowWin win;
+  win.XW = (c->in_L + 1) / 2 + 2;
+  win.ey = (int)std::ceil(4.08 * c->in_vth * d.sh[1]);
+  win.ez = (int)std::ceil(4.08 * c->in_vth * d.sh[2]);
+  win.WY = IN_PY + 2 * win.ey + 1;
+  win.WZ = IN_PZ + 2 * win.ez + 1;
+  win.on = (int64_t)win.XW * win.WY * win.WZ * 4 <= IN_WIN;
+  const unsigned npatch = (unsigned)(((c->cfg.ncy + IN_PY - 1) / IN_PY) *
+                                     ((c->cfg.ncz + IN_PZ - 1) / IN_PZ));
+  SDISPATCH(c, hipLaunchKernelGGL((k_inflow_fill<R>), dim3(npatch), dim3(256), 0, c->stream,
+                                  c->g, win, c->n, pdev<R>(c), d, step, k0, k1, stream, c->ioff,
+                                  (float)c->in_bp, c->bp, c->mr, c->id, (uint32_t)c->next_id,
+                                  (typename AccOf<R>::T*)c->macc));
   if (int e = post(c)) return e;
   if (int e = sr_input_add(c, c->n, tot, 1.0)) return e;
   c->n += tot;
