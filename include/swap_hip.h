@@ -56,8 +56,8 @@ enum {
                               swap_step are dropped by a sort at the end of
                               the step (swap_diag.escaped counts them), and
                               swap_set_inflow feeds the x = 0 face.
-                              Needs ncx >= 4 and sort_every >= 1; the field
-                              subcycles run unfused; no denoiser. */
+                              Needs ncx >= 4 and sort_every >= 1; no
+                              denoiser. */
   SWAP_TRACK_IDS = 4,      /* keep a particle id through sorts (tests) */
   SWAP_UNFUSED_FIELDS = 8  /* field subcycles of swap_step as one kernel per
                               reference routine (nine launches) instead of the

@@ -437,19 +437,42 @@ __device__ __forceinline__ int wrapc(int s, int n) {
   return r + 1;
 }
 
-// storage indices that hold canonical index c along an axis of n nodes
-__device__ __forceinline__ int images(int c, int n, int s[2]) {
+// storage indices that hold canonical index c along an axis of n nodes:
+// periodic, or (open, an inflow/outflow x) the zero-gradient ghost beside
+// each face node
+__device__ __forceinline__ int images(int c, int n, int s[2], bool open = false) {
   s[0] = c;
-  if (c == 1) { s[1] = n - 1; return 2; }
-  if (c == n - 2) { s[1] = 0; return 2; }
+  if (c == 1) { s[1] = open ? 0 : n - 1; return 2; }
+  if (c == n - 2) { s[1] = open ? n - 1 : 0; return 2; }
   return 1;
+}
+
+// LDS tile x position -> canonical x node: periodic, or on an open x the
+// clamp to [1, ncx] (the node whose copy a zero-gradient ghost holds)
+__device__ __forceinline__ int tile_cx(const Geo& g, int s) {
+  return g.ox ? (s < 1 ? 1 : (s > g.ncx ? g.ncx : s)) : wrapc(s, g.nx);
+}
+
+// An open x: after a phase has written an intermediate over the tile, the
+// positions of x ghosts (storage x < 1 or > ncx) take the value of the node
+// they clamp to -- what the per-routine kernels leave in a ghost layer, so
+// that the next phase's stencils read the same numbers.  a: component 0 of
+// ncomp planes PA apart; positions x0 - 2 + (t % PW) along x.
+template <int PW, int PA, int NT, typename T>
+__device__ __forceinline__ void ghost_x(T* a, int ncomp, int tid, int x0, int ncx) {
+  for (int t = tid; t < PA; t += NT) {
+    const int xp = t % PW, sx = x0 - 2 + xp;
+    if (sx >= 1 && sx <= ncx) continue;
+    const int src = t - xp + (sx < 1 ? 1 : ncx) - x0 + 2;
+    for (int m = 0; m < ncomp; m++) a[t + m * PA] = a[src + m * PA];
+  }
 }
 
 template <typename T>
 __device__ __forceinline__ void put3(const Geo& g, T* dst, int x, int y, int z, T v0, T v1,
                                      T v2) {
   int xs[2], ys[2], zs[2];
-  const int ax = images(x, g.nx, xs), ay = images(y, g.ny, ys), az = images(z, g.nz, zs);
+  const int ax = images(x, g.nx, xs, g.ox), ay = images(y, g.ny, ys), az = images(z, g.nz, zs);
   for (int c = 0; c < az; c++)
     for (int b = 0; b < ay; b++)
       for (int a = 0; a < ax; a++) {
@@ -502,11 +525,13 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
   const int X0 = 1 + blockIdx.x * TX, Y0 = 1 + blockIdx.y * TY, Z0 = 1 + blockIdx.z * tz;
   const int Z1 = min(Z0 + tz, g.ncz + 1);
   if (Z0 > g.ncz) return;  // empty z chunk (whole workgroup)
-  for (int t = tid; t < PW; t += NT) tcx[t] = wrapc(X0 - 2 + t, g.nx);
+  for (int t = tid; t < PW; t += NT) tcx[t] = tile_cx(g, X0 - 2 + t);
   for (int t = tid; t < PH; t += NT) tcy[t] = wrapc(Y0 - 2 + t, g.ny);
   __syncthreads();
   const int64_t nxy = (int64_t)g.nx * g.ny, nn = g.nn;
   const R al = (R)f.alpha;
+  // an open x and a tile whose halo reaches an x ghost (uniform per workgroup)
+  const bool xface = g.ox && (X0 - 2 < 1 || X0 + TX + 2 > g.ncx);
   // positions per phase: the whole plane (staging), curl [-1, T+1], E [0, T]
   constexpr int AW = TX + 3, AN = AW * (TY + 3), EW = TX + 1, EN = EW * (TY + 1);
   constexpr int KL = (PA + NT - 1) / NT, KA = (AN + NT - 1) / NT, KE = (EN + NT - 1) / NT;
@@ -597,6 +622,12 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
       }
     }
     __syncthreads();
+    if (xface && s >= Z0 - 3) {
+      const int r = (s + 2) & 1;
+      ghost_x<PW, PA, NT>(&sj[r][0][0], 3, tid, X0, g.ncx);
+      ghost_x<PW, PA, NT>(&sa[r][0][0], 3, tid, X0, g.ncx);
+      __syncthreads();
+    }
     if (s >= Z0 - 2) {  // ct at plane s+2 over x, y in [0, T+1]
       const int r = (s + 2) & 1, rm = r ^ 1;
       constexpr int W = TX + 2;
@@ -622,6 +653,10 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
       }
     }
     __syncthreads();
+    if (xface && s >= Z0 - 2) {
+      ghost_x<PW, PA, NT>(&sc[(s + 2) & 1][0][0], 3, tid, X0, g.ncx);
+      __syncthreads();
+    }
     if (s >= Z0 - 1) {  // E at plane s+1 over x, y in [0, T]
       const int r = (s + 1) & 1, rp = r ^ 1;
 #pragma unroll
@@ -635,6 +670,10 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
       }
     }
     __syncthreads();
+    if (xface && s >= Z0 - 1) {
+      ghost_x<PW, PA, NT>(&se[(s + 1) & 1][0][0], 3, tid, X0, g.ncx);
+      __syncthreads();
+    }
     if (s >= Z0 && oown) {  // b1p2 = b12 - coef curlE(E) at plane s
       const int r = s & 1, rp = r ^ 1;
       const int p = (olx + 2) + PW * (oly + 2);
@@ -673,11 +712,13 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
   const int X0 = 1 + blockIdx.x * TX, Y0 = 1 + blockIdx.y * TY, Z0 = 1 + blockIdx.z * tz;
   const int Z1 = min(Z0 + tz, g.ncz + 1);
   if (Z0 > g.ncz) return;  // empty z chunk (whole workgroup)
-  for (int t = tid; t < PW; t += NT) tcx[t] = wrapc(X0 - 2 + t, g.nx);
+  for (int t = tid; t < PW; t += NT) tcx[t] = tile_cx(g, X0 - 2 + t);
   for (int t = tid; t < PH; t += NT) tcy[t] = wrapc(Y0 - 2 + t, g.ny);
   __syncthreads();
   const int64_t nxy = (int64_t)g.nx * g.ny, nn = g.nn;
   const R al = (R)f.alpha;
+  // an open x and a tile whose halo reaches an x ghost (uniform per workgroup)
+  const bool xface = g.ox && (X0 - 2 < 1 || X0 + TX + 2 > g.ncx);
   constexpr int AW = TX + 3, AN = AW * (TY + 3), EW = TX + 1, EN = EW * (TY + 1);
   constexpr int KL = (PA + NT - 1) / NT, KA = (AN + NT - 1) / NT, KE = (EN + NT - 1) / NT;
   int lp[KL], lq[KL], ap[KA], aq[KA], ep[KE], eq[KE];
@@ -783,6 +824,13 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
       }
     }
     __syncthreads();
+    if (xface && s >= Z0 - 4) {
+      const int w = r3(s + 3);
+      ghost_x<PW, PA, NT>(&sj[w][0][0], 3, tid, X0, g.ncx);
+      ghost_x<PW, PA, NT>(&sa[w][0][0], 3, tid, X0, g.ncx);
+      ghost_x<PW, PA, NT>(&sw[w][0][0], 3, tid, X0, g.ncx);
+      __syncthreads();
+    }
     if (s >= Z0 - 2) {  // ct at plane s+2 over [0, T+1]: a x smoothed B
       const int r = (s + 2) & 1, wc = r3(s + 2), wm = r3(s + 1), wp = r3(s + 3);
       constexpr int W = TX + 2;
@@ -792,7 +840,9 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
         const R aa1 = (R)f.ry * (sa[wc][1][p] - sa[wc][1][p - PW]) + sa[wc][1][p - PW];
         const R aa2 = (R)f.rz * (sa[wc][2][p] - sa[wm][2][p]) + sa[wm][2][p];
         const R m1c = (R)0.5 * (sw[wc][0][p] + sw[wc][0][p + 1]);
-        const R m1m = (R)0.5 * (sw[wc][0][p - 1] + sw[wc][0][p]);
+        // on an open x the face average below node 1 is a copy of node 1's
+        const R m1m = g.ox && X0 - 2 + p % PW == 1 ? m1c
+                                                   : (R)0.5 * (sw[wc][0][p - 1] + sw[wc][0][p]);
         const R m2c = (R)0.5 * (sw[wc][1][p] + sw[wc][1][p + PW]);
         const R m2m = (R)0.5 * (sw[wc][1][p - PW] + sw[wc][1][p]);
         const R m3c = (R)0.5 * (sw[wc][2][p] + sw[wp][2][p]);
@@ -806,6 +856,10 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
       }
     }
     __syncthreads();
+    if (xface && s >= Z0 - 2) {
+      ghost_x<PW, PA, NT>(&sc[(s + 2) & 1][0][0], 3, tid, X0, g.ncx);
+      __syncthreads();
+    }
     if (s >= Z0 - 1) {  // E at plane s+1 over [0, T]
       const int r = (s + 1) & 1, rp = r ^ 1, w = r3(s + 1);
       const int ze = wrapc(s + 1, g.nz);
@@ -829,6 +883,10 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
       }
     }
     __syncthreads();
+    if (xface && s >= Z0 - 1) {
+      ghost_x<PW, PA, NT>(&se[(s + 1) & 1][0][0], 3, tid, X0, g.ncx);
+      __syncthreads();
+    }
     if (s >= Z0 && oown) {  // b = b1 - coef curlE(E) at plane s
       const int r = s & 1, rp = r ^ 1;
       const int p = (olx + 2) + PW * (oly + 2);

@@ -805,8 +805,7 @@ int swap_face_to_center(swap_ctx* c, int in, int out) {
 
 int swap_field_subcycles(swap_ctx* c, int ntf, double dtsub) {
   if (!c || ntf < 0 || !(dtsub > 0)) return SWAP_EINVAL;
-  // the fused kernels wrap every axis; an open x runs the per-routine kernels
-  const bool fused = !(c->cfg.flags & SWAP_UNFUSED_FIELDS) && !c->g.ox;
+  const bool fused = !(c->cfg.flags & SWAP_UNFUSED_FIELDS);
   for (int q = 0; q < ntf; q++)
     TRY(fused ? subcycle_fused(c, dtsub, q == ntf - 1) : subcycle_impl(c, dtsub));
   return 0;

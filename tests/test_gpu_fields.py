@@ -140,29 +140,36 @@ def test_check_time_step_halving():
     assert dtsub == pytest.approx(d_o, rel=1e-6)
 
 
-def _periodic_np(a):
-    """ghost layers of a storage array [..., nz, ny, nx] (boundary.f:9-52)"""
+def _periodic_np(a, open_x=False):
+    """ghost layers of a storage array [..., nz, ny, nx] (boundary.f:9-52);
+    open_x: zero-gradient copies along x instead"""
     for ax in (-1, -2, -3):
         n = a.shape[ax]
         d = [slice(None)] * a.ndim
         s = [slice(None)] * a.ndim
-        d[ax], s[ax] = 0, n - 2
+        d[ax], s[ax] = 0, (1 if open_x and ax == -1 else n - 2)
         a[tuple(d)] = a[tuple(s)]
-        d[ax], s[ax] = n - 1, 1
+        d[ax], s[ax] = n - 1, (n - 2 if open_x and ax == -1 else 1)
         a[tuple(d)] = a[tuple(s)]
     return a
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-@pytest.mark.parametrize("nc", [(16, 8, 8), (20, 6, 37), (2, 3, 70), (40, 9, 33)])
-def test_fused_subcycles_equal_per_routine_kernels(precision, nc):
+@pytest.mark.parametrize("nc,open_x", [((16, 8, 8), False), ((20, 6, 37), False),
+                                       ((2, 3, 70), False), ((40, 9, 33), False),
+                                       ((16, 8, 8), True), ((4, 6, 37), True),
+                                       ((40, 9, 33), True), ((33, 5, 9), True)])
+def test_fused_subcycles_equal_per_routine_kernels(precision, nc, open_x):
     """the LDS-tiled predictor/corrector pair (two launches per subcycle) gives
     bit for bit what the nine per-routine kernels give: B levels, and the E
     and aj the last corrector leaves.  Shapes off the 32x4 tile and the z
-    chunking, and a 2-cell axis (every halo wraps more than once)."""
+    chunking, and a 2-cell axis (every halo wraps more than once); with an
+    open x (zero-gradient x ghosts), tiles at either face, a grid narrower
+    than one tile and one a cell past it."""
     p = physics.workload(*nc, 2)
     out = []
-    for flags in (0, api.UNFUSED_FIELDS):
+    ox = api.OUTFLOW_X if open_x else 0
+    for flags in (ox, ox | api.UNFUSED_FIELDS):
         s = physics.make_swap(p, precision=precision, flags=flags, capacity=16)
         rng = np.random.default_rng(5)
         shp = (s.nz, s.ny, s.nx)
@@ -173,7 +180,7 @@ def test_fused_subcycles_equal_per_routine_kernels(precision, nc):
         up = rng.standard_normal((3,) + shp) * 30
         up[0] += 400
         for nm, a in (("b1", b), ("b12", b12), ("np", npf), ("up", up)):
-            s.set_field(nm, _periodic_np(a))
+            s.set_field(nm, _periodic_np(a, open_x))
         s.field_subcycles(3, p["dt"] / 10)
         out.append({f: s.get_field(f) for f in ("b1", "b12", "b1p2", "E", "aj")})
         s.close()
