@@ -48,13 +48,14 @@ def measured_traffic(kernel, cfg):
     return None if k is None else k["fetch_bytes"] + k["write_bytes"]
 
 
-def cpu_baseline(steps=15):
-    """FP64 oracle (1 thread) on the 64x32x32 x 8 ppc workload, `steps` steps."""
+def cpu_baseline(steps=5):
+    """FP64 oracle (1 thread) on a bounded sample of the workload: the same
+    64 ppc drifting Maxwellian on 64x32x32 cells, `steps` full steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
     from swap_amd import physics
-    p = physics.workload(64, 32, 32, 8)
+    p = physics.workload(64, 32, 32, 64)
     n = p["npart"]
     cell, frac, v = O.load_maxwellian(n, 20261203, (64, 32, 32), p["vsw"], p["vth"])
     i, j, k = cell & 2047, (cell >> 11) & 1023, cell >> 21
@@ -72,9 +73,9 @@ def cpu_baseline(steps=15):
         o.step(st)
     dt = time.perf_counter() - t0
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle (oracle/swap_oracle.c, 1 thread), 64x32x32 cells x 8 ppc "
-                      "(524,288 protons), %d full steps incl. 10 field subcycles, %.1f s"
-                      % (steps, dt)}
+            "sample": "FP64 oracle (oracle/swap_oracle.c, 1 thread), 64x32x32 cells x 64 ppc "
+                      "(%s protons), %d full steps incl. 10 field subcycles, %.1f s"
+                      % (format(n, ","), steps, dt)}
 
 
 def main():
