@@ -7,7 +7,7 @@
 
 // particle-pass tiling: particles are counting-sorted by (tile, cell in
 // tile); one work item is a run of at most ITEM_MAX particles of one tile
-constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 4;
+constexpr int TILE_X = 4, TILE_Y = 8, TILE_Z = 8;
 constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
 constexpr int ITEM_MAX = 32768;
 constexpr int PASS_WG = 512;   // threads per particle-pass workgroup

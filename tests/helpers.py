@@ -5,7 +5,8 @@ import oracle as O
 from swap_amd import physics
 
 # tile key of a packed cell, the sort order of swap_amd/csrc/k_sort.hip
-TILE = (8, 8, 4)
+# (TILE_X, TILE_Y, TILE_Z of swap_amd/csrc/swap_internal.h)
+TILE = (4, 8, 8)
 
 
 def unpack(cell):
