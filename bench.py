@@ -85,7 +85,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, nargs=3, default=[256, 128, 128])
     ap.add_argument("--ppc", type=int, default=64)
-    ap.add_argument("--sort-every", type=int, default=2)
+    ap.add_argument("--sort-every", type=int, default=8,
+                    help="longest interval between counting sorts; the step sorts "
+                         "earlier when a work item's box outgrows its deposit window")
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--workload", choices=("uniform", "pickup", "outflow"), default="uniform",
                     help="pickup: a neutral cloud at the domain centre injects 28-amu ions "

@@ -76,7 +76,9 @@ typedef struct {
   int64_t capacity;        /* particle slots (reference Ni_max) */
   int precision;           /* SWAP_FP32 or SWAP_FP64 */
   int flags;               /* SWAP_SPECIES | SWAP_OUTFLOW_X | ... */
-  int sort_every;          /* fused step: counting sort every k steps (0: never) */
+  int sort_every;          /* fused step: counting sort at least every k steps
+                              (0: never); earlier when the last step left a
+                              work item whose box outgrew its deposit window */
   uint64_t seed;           /* counter-based RNG key */
 } swap_config;
 

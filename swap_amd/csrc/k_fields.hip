@@ -1031,6 +1031,9 @@ int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub) {
   HIPCHK(c, hipGetLastError());
   float mn = 0;
   HIPCHK(c, hipMemcpyAsync(&mn, c->dmin, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  // the step's sort trigger rides on this synchronisation
+  HIPCHK(c, hipMemcpyAsync(&c->box_over, c->dcnt + 4, sizeof(c->box_over),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int h = 0;
   // every node passes unless some deltat <= 2 dtsub0 (compare in the

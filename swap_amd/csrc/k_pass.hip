@@ -200,6 +200,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   }
   const bool boxed = windowed;
   int lo_m[3] = {0, 0, 0}, hi_m[3] = {0, 0, 0};
+  float mid[3] = {0.f, 0.f, 0.f};  // centre of each axis's displacement range
   if (MODE == 3 && windowed) {
     // the cell shift of a half move by d in [dlo, dhi] cells lies in
     // [floor(dlo), ceil(dhi)]; 10 % of the largest speed covers the change of
@@ -219,8 +220,11 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       float dlo = (float)K.s[a] * (fval(tb[a]) - sl), dhi = (float)K.s[a] * (fval(tb[3 + a]) + sl);
       if (!(dlo <= dhi)) windowed = false;  // unset bounds (NaN)
       else {
-        lo_m[a] = (int)floorf(fminf(fmaxf(dlo, -WIN_SHIFT), WIN_SHIFT));
-        hi_m[a] = (int)ceilf(fminf(fmaxf(dhi, -WIN_SHIFT), WIN_SHIFT));
+        const float cl = fminf(fmaxf(dlo, -WIN_SHIFT), WIN_SHIFT);
+        const float ch = fminf(fmaxf(dhi, -WIN_SHIFT), WIN_SHIFT);
+        lo_m[a] = (int)floorf(cl);
+        hi_m[a] = (int)ceilf(ch);
+        mid[a] = 0.5f * (cl + ch);
       }
     }
   }
@@ -230,6 +234,25 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     W[a] = bh[a] + hi_m[a] - u0[a] + 2;
     cen[a] = (bl[a] + bh[a]) >> 1;
     T[a] = bh[a] - bl[a] + 2;  // gather tile: nodes bl .. bh+1
+  }
+  if (MODE == 3 && windowed) {
+    // a P3 window over the LDS budget (boxes grown by a few steps of drift)
+    // gives up cells of its shift range one at a time, the widest axis's
+    // side farther from the range's centre first: the bulk keeps the window
+    // and only the fastest particles deposit straight to HBM
+    for (int it = 0; it < 24 && (int64_t)W[0] * W[1] * W[2] * M > WIN_VALUES; it++) {
+      int a = 0;
+      for (int b = 1; b < 3; b++)
+        if (hi_m[b] - lo_m[b] > hi_m[a] - lo_m[a]) a = b;
+      if (hi_m[a] - lo_m[a] <= 0) break;
+      if ((float)hi_m[a] - mid[a] >= mid[a] - (float)lo_m[a]) {
+        hi_m[a]--;
+      } else {
+        lo_m[a]++;
+        u0[a]++;
+      }
+      W[a]--;
+    }
   }
   int64_t Wn64 = (int64_t)W[0] * W[1] * W[2];
   if (windowed && (W[0] < 2 || W[1] < 2 || W[2] < 2 || Wn64 * M > WIN_VALUES)) windowed = false;
@@ -481,6 +504,10 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   __syncthreads();
   if (MODE == 3 && tid < 6) {
     if (w < nitems) {
+      if (tid == 0 && sred[0] <= sred[3] && sred[1] <= sred[4] && sred[2] <= sred[5] &&
+          (int64_t)(sred[3] - sred[0] + 2) * (sred[4] - sred[1] + 2) * (sred[5] - sred[2] + 2) *
+                  4 > WIN_VALUES)
+        atomicAdd(dcnt + 4, 1ull);  // next step's P1 window would not fit
       int* tn = vb_next + 6 * (tile + 1);
       if (tid < 3) {
         atomicMin(boxes_next + 6 * w + tid, sred[tid]);
@@ -507,8 +534,10 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   }
 }
 
-__global__ void k_pass_reset(int64_t nitems, int64_t nvb, int* boxes_next, int* vb_next) {
+__global__ void k_pass_reset(int64_t nitems, int64_t nvb, int* boxes_next, int* vb_next,
+                             unsigned long long* box_over) {
   int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q == 0) *box_over = 0;
   if (q < nvb) vb_next[q] = (q % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;
   if (q >= nitems * 6) return;
   boxes_next[q] = (q % 6) < 3 ? 0x7fffffff : -0x7fffffff;
@@ -525,7 +554,7 @@ int post(swap_ctx* c) {
 int pl_box_reset(swap_ctx* c) {
   int64_t n6 = std::max<int64_t>(c->max_items * 6, vb_ints(c));
   hipLaunchKernelGGL(k_pass_reset, dim3(nblk(n6, 256)), dim3(256), 0, c->stream, c->max_items,
-                     vb_ints(c), c->boxes_next, c->vb_next);
+                     vb_ints(c), c->boxes_next, c->vb_next, c->dcnt + 4);
   return post(c);
 }
 

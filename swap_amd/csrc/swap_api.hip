@@ -254,7 +254,7 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   DMF(c->scratch, c->scratch_bytes);
   DMF(c->dred, 64 * sizeof(double));
   DMF(c->dmin, 16 * sizeof(float));
-  DMF(c->dcnt, 4 * sizeof(unsigned long long));
+  DMF(c->dcnt, 8 * sizeof(unsigned long long));
   DMF(c->d_nitems, 4 * sizeof(int));
   // particles (double-buffered for the sort)
   c->cap = cfg->capacity;
@@ -835,10 +835,12 @@ int swap_step(swap_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     // sort on schedule, or when particles were loaded (nothing sorted), or
     // when injected ions appended past the sorted range exceed 1/16 of all
-    // (the passes take such a tail without windows until the next sort)
+    // (the passes take such a tail without windows until the next sort), or
+    // when the last P3 found an item whose box outgrew its P1 window (its
+    // particles would all deposit straight to HBM)
     bool due = c->cfg.sort_every > 0 &&
                (c->n_sorted < 0 || c->steps_since_sort >= c->cfg.sort_every ||
-                (c->n - c->n_sorted) * 16 > c->n);
+                (c->n - c->n_sorted) * 16 > c->n || (c->steps_since_sort > 0 && c->box_over > 0));
     if (due) {
       Stage t(c, SWAP_K_SORT);
       TRY(swap_sort(c));

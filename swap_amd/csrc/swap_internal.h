@@ -93,7 +93,10 @@ struct swap_ctx {
   uint32_t *ccnt = nullptr, *coff = nullptr;
   float* cbp = nullptr;
   // counters
-  unsigned long long* dcnt = nullptr;  // [0] window misses [1] escapes [2] scratch [3] chex
+  // [0] window misses [1] escapes [2] scratch [3] chex [4] items whose next
+  // box (reduced by the last P3) outgrows a P1 window
+  unsigned long long* dcnt = nullptr;
+  unsigned long long box_over = 0;  // dcnt[4], read back with check_time_step
   int64_t injected = 0, escaped = 0;
   int64_t next_id = 0;  // id of the next appended ion (ids stay unique when ions leave)
   // inflow through an open x face (swap_set_inflow): upstream ions per cell,
