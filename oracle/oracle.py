@@ -75,6 +75,8 @@ def _load(width):
         "or_split_move": ([i64, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
         "or_stable_sort": ([i64, vp, C.c_uint32, vp], None),
         "or_load_maxwellian": ([i64, C.c_uint64, C.c_uint32, vp, dbl, dbl, vp, vp, vp], None),
+        "or_ionize_sw": ([vp, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl, C.c_uint64, C.c_uint32,
+                          vp, vp, vp, i64], i64),
         "or_cloud_counts": ([vp, dbl, dbl, dbl, dbl, dbl, vp, i32, C.c_uint64, C.c_uint32, vp,
                              vp], None),
         "or_cloud_ions": ([vp, vp, C.c_uint64, C.c_uint32, i32, vp, vp, i64], None),
@@ -183,7 +185,13 @@ class Oracle:
     def _state(self, st):
         s = OrState()
         for n, _ in OrState._fields_[:-5]:
-            setattr(s, n, _p(st.get(n)))
+            a = st.get(n)
+            if a is not None:
+                want = np.int32 if n == "ijkp" else (np.uint32 if n == "id" else self.dtype)
+                if a.dtype != want or not a.flags.c_contiguous:
+                    raise TypeError("oracle state %r: need a C-contiguous %s array, got %s"
+                                    % (n, np.dtype(want).name, a.dtype))
+            setattr(s, n, _p(a))
         s.Ni_tot = st["Ni_tot"]
         s.Ni_max = st["xp"].shape[1]
         s.input_k = st.get("input_k", 0.0)
@@ -262,6 +270,19 @@ def cloud_counts(nc, d, beta, dt, cloud, seed, step):
     lib.or_cloud_counts(nc.ctypes.data, d[0], d[1], d[2], beta, dt, cl.ctypes.data,
                         int(cloud["max_per_cell"]), seed, step, cnt.ctypes.data, bp.ctypes.data)
     return cnt, bp
+
+
+def ionize_sw(nc, d, dt, beta, ndot, vsw, beta_pu, seed, step):
+    """Ionize_sw_mp's new ions: (cell uint32[m], frac float64[3,m],
+    v float64[3,m])"""
+    lib, _ = _load(64)
+    nc = np.asarray(nc, np.int32)
+    args = (nc.ctypes.data, d[0], d[1], d[2], dt, beta, ndot, vsw, beta_pu, seed, step)
+    m = lib.or_ionize_sw(*args, None, None, None, 0)
+    cell = np.zeros(max(m, 1), np.uint32)
+    frac, v = np.zeros((3, max(m, 1))), np.zeros((3, max(m, 1)))
+    lib.or_ionize_sw(*args, cell.ctypes.data, frac.ctypes.data, v.ctypes.data, max(m, 1))
+    return cell[:m].copy(), frac[:, :m].copy(), v[:, :m].copy()
 
 
 def cloud_ions(nc, cnt, seed, step, open_x=False):

@@ -906,6 +906,52 @@ void or_load_maxwellian(int64_t n, uint64_t seed, uint32_t stream, const int nc[
   }
 }
 
+/* Ionize_sw_mp (chem_rates.f:508-646) with the build's RNG in place of
+   pad_ranf.  The count (:553-562): dNi = ndot dt beta beta_pu dx dy delz nz
+   with nz the reference's z extent including its two ghost layers (the
+   formula has no nx, ny: the reference's quasi-1-D volume); below one ion
+   dNi becomes 1 when dNi > u (Philox counter (step, 0, 0x10, 0), word 0),
+   then nint.  The new ions (:567-610): ion p = 0..dNi-1 draws Philox counter
+   (p lo, p hi, 0x20000 + step, 7); word 0 gives the ring phase
+   theta = u 2 PI (PI = 3.14159, inputs.f:52) and v = (vsw + vsw cos theta,
+   vsw sin theta, 0) (:570-573); words 1..3 give x_a = q(1) + (1 - u)(q(n-1) -
+   q(1)) (:575-577), i.e. t = (1 - u) nc_a cells above q(1), split
+   right-closed as get_interp_weights' search (:579-593) does: cell
+   ceil(t) - 1, fraction t - cell in (0, 1].  mrat = 1 and beta_p = beta_pu
+   (:606-608) are the caller's; the re-deposit of :641-643 is the caller's
+   next update_np/update_up.  Returns dNi; writes at most cap ions
+   (cell packed i | j<<11 | k<<21, frac and v as [3][cap]). */
+int64_t or_ionize_sw(const int nc[3], double dx, double dy, double dz, double dt, double beta,
+                     double ndot, double vsw, double beta_pu, uint64_t seed, uint32_t step,
+                     uint32_t *cell, double *frac, double *v, int64_t cap) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  double dNi = ndot * dt * beta * beta_pu * dx * dy * dz * (double)(nc[2] + 2);
+  if (dNi < 1.0) {
+    uint32_t ctr[4] = {step, 0u, 0x10u, 0u}, r[4];
+    or_philox4x32_10(ctr, key, r);
+    if (dNi > or_u24(r[0])) dNi = 1.0;
+  }
+  int64_t m = (int64_t)llround(dNi);
+  if (m < 0) m = 0;
+  for (int64_t p = 0; p < m && p < cap; p++) {
+    uint32_t ctr[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), 0x20000u + step, 7u}, r[4];
+    or_philox4x32_10(ctr, key, r);
+    double th = or_u24(r[0]) * 2.0 * 3.14159;
+    int ijk[3];
+    for (int a = 0; a < 3; a++) {
+      double t = (1.0 - or_u24(r[1 + a])) * (double)nc[a];
+      double i = ceil(t) - 1.0;
+      ijk[a] = (int)i;
+      frac[a * cap + p] = t - i;
+    }
+    cell[p] = (uint32_t)ijk[0] | ((uint32_t)ijk[1] << 11) | ((uint32_t)ijk[2] << 21);
+    v[p] = vsw + vsw * cos(th);
+    v[cap + p] = vsw * sin(th);
+    v[2 * cap + p] = 0.0;
+  }
+  return m;
+}
+
 /* pickup-ion cloud: photoionization counts and new ions per interior node,
    and charge exchange, following Ionize_pluto_mp (chem_rates.f:290-503) and
    res_chex (chem_rates.f:805-843) with the neutral profile

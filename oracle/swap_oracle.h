@@ -150,6 +150,13 @@ void or_stable_sort(int64_t n, const uint32_t *key, uint32_t nkeys,
 void or_load_maxwellian(int64_t n, uint64_t seed, uint32_t stream, const int nc[3], double vsw,
                         double vth, uint32_t *cell, double *frac, double *v);
 
+/* Ionize_sw_mp (chem_rates.f:508-646) with the build's RNG: returns the
+   count dNi and writes the first min(dNi, cap) new ions (cell, frac [3][cap],
+   v [3][cap]); see swap_oracle.c */
+int64_t or_ionize_sw(const int nc[3], double dx, double dy, double dz, double dt, double beta,
+                     double ndot, double vsw, double beta_pu, uint64_t seed, uint32_t step,
+                     uint32_t *cell, double *frac, double *v, int64_t cap);
+
 /* pickup-ion cloud (chem_rates.f:290-503, 805-843 restated with the
    build's RNG): cl = {cx, cy, cz, N0, r0, H, tau_photo, rmin, sigma_chex,
    mrat_pu, beta_min}.  Counts and weights per interior node (i fastest), the new

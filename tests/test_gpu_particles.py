@@ -20,6 +20,17 @@ def _face_ambiguous(frac):
     return np.any((frac > 1 - 1e-9) | (frac < 1e-9), axis=0)
 
 
+def _face_ambiguous_bound(n, nc):
+    # The seeded load puts t = (1 - k 2^-24) nc_a (k an integer) and stores
+    # the fraction t - ceil(t) + 1 in FP32: a fraction is either a multiple
+    # of nc_a 2^-24 >= 6e-8 or exactly 1 (t an integer, which for nc_a a
+    # power of two has probability nc_a 2^-24 per axis), so within 1e-9 of
+    # a face means exactly 1.0.  Expected count n sum_a nc_a 2^-24; the
+    # bound allows the Poisson tail to about 1e-6.
+    lam = n * sum(nc) * 2.0 ** -24
+    return max(2, int(np.ceil(lam + 6 * np.sqrt(lam) + 3)))
+
+
 def test_seeded_load_matches_oracle_sampler():
     pr = Pair(load=False)
     p = pr.p
@@ -46,6 +57,7 @@ def test_get_interp_weights_and_deposits():
     o.get_interp_weights(pr.n, pr.n, st["xp"], st["ijkp"], st["wght"])
     ijk_g = s.particle_array(api.P_IJKP)
     amb = _face_ambiguous(s.particle_array(api.P_FRAC))
+    assert amb.sum() <= _face_ambiguous_bound(pr.n, (pr.p["ncx"], pr.p["ncy"], pr.p["ncz"]))
     same = np.all(ijk_g == st["ijkp"], axis=0)
     assert np.all(same | amb), "cell index mismatch away from faces"
     w = s.particle_array(api.P_WGHT)

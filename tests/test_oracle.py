@@ -278,6 +278,105 @@ def test_vplus_vminus_exact_rotation():
     assert np.allclose(ang, want, rtol=1e-10)
 
 
+# corner order of the 8 weights, gutsp.f:1415-1429: (i,j,k) (i+1,j,k)
+# (i,j,k+1) (i+1,j,k+1) (i,j+1,k) (i+1,j+1,k) (i,j+1,k+1) (i+1,j+1,k+1)
+CORNER_DI = np.array([0, 1, 0, 1, 0, 1, 0, 1])
+CORNER_DJ = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+CORNER_DK = np.array([0, 0, 1, 1, 0, 0, 1, 1])
+
+
+def _interior_positions(o, n, rng, margin=2):
+    """positions whose cells' corners and their face_to_center stencils are
+    all away from the wrapped ghosts"""
+    xp = o.parts(n)
+    for a, ax in enumerate("xyz"):
+        q = o.coord("q" + ax)
+        lo, hi = q[margin], q[-2 - margin]
+        xp[a] = lo + (hi - lo) * (1.0 - rng.random(n))
+    return xp
+
+
+def test_get_Ep_gathers_linear_fields_exactly():
+    """Non-uniform linear aj, up (on the staggered edge positions) and btc
+    (nodes): face_to_center is exact on them, and the trilinear gather of
+    gutsp.f:217-251 reproduces a linear field at the particle, so Ep =
+    ((aj - up)(x_p) x B(x_p)) mrat exactly.  A permuted corner table, a
+    swapped aj/up or a wrong cross product fails."""
+    o = make(nc=(10, 9, 8))
+    x, y, z, d = _node_coords(o)
+    h = d / 2
+    rng = np.random.default_rng(16)
+    n = 2000
+    xp = _interior_positions(o, n, rng)
+    ijkp, w = weights(o, xp)
+    LA = rng.standard_normal((3, 4))
+    LU = rng.standard_normal((3, 4)) * 50
+    LB = rng.standard_normal((3, 4)) * 0.01
+
+    def lin(L, c, X, Y, Z):
+        return L[c, 0] + (L[c, 1] * X + L[c, 2] * Y + L[c, 3] * Z) / 1e3
+    stag = ((x + h, y, z), (x, y + h, z), (x, y, z + h))  # face_to_center input positions
+    aj = np.ascontiguousarray(np.stack([lin(LA, c, *stag[c]) for c in range(3)]))
+    up = np.ascontiguousarray(np.stack([lin(LU, c, *stag[c]) for c in range(3)]))
+    btc = np.ascontiguousarray(np.stack([lin(LB, c, x, y, z) for c in range(3)]))
+    for mrat in (1.0, 1.0 / 28):
+        Ep = o.parts(n)
+        o.get_Ep(n, n, ijkp, w, np.full(n, mrat), aj.copy(), up.copy(), btc, Ep)
+        A = np.stack([lin(LA, c, *xp) - lin(LU, c, *xp) for c in range(3)])
+        B = np.stack([lin(LB, c, *xp) for c in range(3)])
+        want = np.cross(A.T, B.T).T * mrat
+        assert np.allclose(Ep, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+    # get_vplus_vminus re-gathers btc (gutsp.f:359-369): its rotation uses the
+    # linear B at the particle -- |v+| = |v-| and the parallel part kept about
+    # each particle's own B(x_p)
+    vp = rng.standard_normal((3, n)) * 50
+    Ep = rng.standard_normal((3, n)) * 1e-2
+    vpl, vmi = o.parts(n), o.parts(n)
+    o.get_vplus_vminus(n, n, ijkp, w, np.ones(n), Ep, btc, vp, vpl, vmi)
+    B = np.stack([lin(LB, c, *xp) for c in range(3)])
+    bh = B / np.linalg.norm(B, axis=0)
+    assert np.allclose(np.linalg.norm(vpl, axis=0), np.linalg.norm(vmi, axis=0), rtol=1e-12)
+    assert np.allclose(np.sum(bh * vpl, 0), np.sum(bh * vmi, 0), rtol=1e-10, atol=1e-9)
+    # and the rotation angle about B(x_p): -2 atan(|B(x_p)| dt / 2)
+    pm = vmi - bh * np.sum(bh * vmi, 0)
+    pp = vpl - bh * np.sum(bh * vpl, 0)
+    ang = np.arctan2(np.sum(bh * np.cross(pm.T, pp.T).T, 0), np.sum(pm * pp, 0))
+    assert np.allclose(ang, -2 * np.arctan(np.linalg.norm(B, axis=0) * PAR["dt"] / 2),
+                       rtol=1e-9)
+
+
+def test_update_np_is_the_adjoint_of_the_gather():
+    """For any periodic node function g: sum over unique nodes of
+    np g V beta = sum over particles of (1/beta_p) sum_c w_c g(corner_c), with
+    the reference's corner order -- the deposit puts each weight on the
+    corner the gather reads it from (gutsp.f:1480-1498, 1415-1429)"""
+    nc = (7, 6, 5)
+    n = 3000
+    o = make(nc=nc, npart=n)
+    rng = np.random.default_rng(17)
+    xp, _ = load(o, n, rng)
+    ijkp, w = weights(o, xp)
+    bp = rng.uniform(0.3, 1.0, n)
+    npg = o.grid(1)
+    o.update_np(n, n, ijkp, w, bp, npg)
+    g = rng.standard_normal((nc[2], nc[1], nc[0]))  # unique nodes 1..nc (storage)
+    gext = np.zeros((o.nz, o.ny, o.nx))
+    gext[1:-1, 1:-1, 1:-1] = g
+    for ax in range(3):  # periodic images: storage 0 == nc, nc+1 == 1
+        gext = np.moveaxis(gext, 2 - ax, 0)
+        gext[0] = gext[-2]
+        gext[-1] = gext[1]
+        gext = np.moveaxis(gext, 0, 2 - ax)
+    beta = n / (np.prod(nc) * PAR["dx"] ** 3 * 0.1e15)
+    lhs = np.sum(npg[1:-1, 1:-1, 1:-1] * g) * PAR["dx"] ** 3 * beta
+    i, j, k = ijkp  # 1-based: storage index of the cell's lower corner
+    rhs = 0.0
+    for c in range(8):
+        rhs += np.sum(w[c] / bp * gext[k + CORNER_DK[c] - 1, j + CORNER_DJ[c] - 1,
+                                       i + CORNER_DI[c] - 1])
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
 def test_vp_final_and_move_half():
     """get_vp_final: vp1 <- vp, vp <- v+ + dt/2 Ep (gutsp.f:433-449);
     move_ion_half: x += dt/2 v, one periodic wrap into (q(1), q(n-1)]"""
