@@ -131,6 +131,10 @@ typedef struct {
   int64_t escaped;      /* cumulative outflow removals */
   int64_t injected;     /* cumulative injections */
   int64_t window_misses;/* particles that took the global-atomic deposit path */
+  int64_t shrunk_windows;/* P3 work items whose deposit window gave up part of
+                            its shift range to fit LDS (their fastest
+                            particles become window misses) */
+  int64_t sorts;        /* counting sorts done by swap_sort / swap_step */
 } swap_diag;
 
 /* ---- lifecycle (replaces dimensions.f/global.f static state) ---- */
@@ -151,7 +155,8 @@ int swap_get_coords(swap_ctx *ctx, int axis, double *q /* n_axis+2 */);
    get_interp_weights (gutsp.f:1391-1412): q(i) < x <= q(i+1). */
 int swap_load_particles(swap_ctx *ctx, int64_t n, const double *xp, const double *vp,
                         const double *vp1, const double *beta_p, const double *mrat);
-/* split form: packed cells and fractions as stored on the device */
+/* split form: packed cells and fractions as stored on the device; a cell
+   outside the grid or a fraction outside (0, 1] is refused (SWAP_EINVAL) */
 int swap_load_particles_split(swap_ctx *ctx, int64_t n, const uint32_t *cell,
                               const void *frac, const void *vp, const void *vp1,
                               const float *beta_p, const float *mrat);
@@ -332,6 +337,22 @@ int swap_dn_grad(swap_ctx *ctx, const float *x, const float *target, double *los
 int swap_dn_infer(swap_ctx *ctx, const float *x, float *y);
 int swap_dn_get_params(swap_ctx *ctx, float *params);
 int swap_dn_set_params(swap_ctx *ctx, const float *params);
+
+/* ---- run state for a checkpoint (no reference counterpart: the reference
+   restarts from restart.vars, maind.f:588-607, which holds its module
+   state).  Everything swap_step reads beside the particles and the B levels:
+   the step count (it keys the inflow draws), the sort schedule (particles
+   sorted, steps since the sort, the work items with their cell boxes and the
+   velocity bounds of the last pass, which fix how the FP32 run sums group
+   particles), the id counter, the injected/escaped counts, the inflow
+   parameters, the input-energy account and the counters of swap_diag.
+   Restore after loading the same particles (swap_load_particles_split in
+   the saved order, ids with swap_set_particle_array) into a context created
+   with the same configuration: the resumed run then continues bit for bit.
+   The blob is opaque host memory of swap_run_state_size bytes. ---- */
+int swap_run_state_size(swap_ctx *ctx, int64_t *bytes);
+int swap_save_run_state(swap_ctx *ctx, void *host);
+int swap_restore_run_state(swap_ctx *ctx, const void *host);
 
 /* ---- build-only instrumentation: HIP events on the context stream around
    each stage of swap_step; no reference counterpart (the reference times

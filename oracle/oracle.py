@@ -65,6 +65,7 @@ def _load(width):
         "or_move_ion_half": ([vp, i32, i32, vp, vp], None),
         "or_step": ([vp, vp], None),
         "or_set_open_x": ([vp, i32], None),
+        "or_set_threads": ([vp, i32], None),
         "or_read_load": ([C.c_char_p, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
         "or_energy_diag": ([vp, vp, dbl, vp, vp], None),
         "or_ke": ([vp, vp, i32, i32], dbl),
@@ -209,6 +210,10 @@ class Oracle:
         b0 = np.asarray(b0, np.float64)
         self.lib.or_energy_diag(self.h, C.byref(s), mion, b0.ctypes.data, out.ctypes.data)
         return dict(zip(ENERGY_KEYS, out.tolist()))
+
+    def set_threads(self, n):
+        """OpenMP threads of the oracle's loops (1: serial)"""
+        self.lib.or_set_threads(self.h, int(n))
 
     def set_open_x(self, on=True):
         """build-defined inflow/outflow x instead of periodic"""

@@ -16,8 +16,14 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define R(x) ((real)(x))
+/* parallel loop over particles or k-planes (every such loop writes only its
+   own particle or plane) */
+#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(h->nthreads) if (h->nthreads > 1)")
 
 struct orc {
   int nx, ny, nz;
@@ -36,6 +42,11 @@ struct orc {
   double in_ppc, in_vsw, in_vth, in_bp;
   uint32_t next_id;
   int64_t step;
+  /* OpenMP threads of the particle and grid loops (or_set_threads; 1: the
+     serial restatement).  Deposits then sum per-thread grids in thread
+     order, so a given thread count is deterministic; counts differ in the
+     rounding of the deposit sums only. */
+  int nthreads;
 };
 
 /* Fortran (i,j,k) / (i,j,k,m), 1-based, column-major */
@@ -109,6 +120,7 @@ orc *or_create(int nx, int ny, int nz, double dx, double dy, double delz,
   h->dtsub = h->dt / (real)ntsub; /* inputs.f:145 */
   h->ntf = (real)ntsub;
   h->alpha = (real)alpha; h->beta = (real)beta;
+  h->nthreads = 1;
   real **c1[] = {&h->qx, &h->qy, &h->qz, &h->dx_grid, &h->dy_grid, &h->dz_grid,
                  &h->dx_cell, &h->dy_cell, &h->dz_cell, &h->xrat, &h->yrat, &h->zrat};
   int n1[] = {nx, ny, nz, nx, ny, nz, nx, ny, nz, nx, ny, nz};
@@ -139,6 +151,7 @@ void or_get_coord(const orc *h, int which, real *out) {
 
 void or_set_dtsub(orc *h, double dtsub, double ntf) { h->dtsub = (real)dtsub; h->ntf = (real)ntf; }
 void or_set_open_x(orc *h, int on) { h->ox = on ? 1 : 0; }
+void or_set_threads(orc *h, int n) { h->nthreads = n > 1 ? n : 1; }
 void or_get_dtsub(const orc *h, double *dtsub, double *ntf) { *dtsub = h->dtsub; *ntf = h->ntf; }
 
 /* initparameters, inputs.f:125-172 (constants inputs.f:19-67) */
@@ -235,6 +248,7 @@ static void e2c_point(const orc *h, const real *bt, int i, int j, int k, real *o
 /* edge_to_center, grid_interp.f:10-62 */
 void or_edge_to_center(const orc *h, real *bt, real *btc) {
   or_periodic(h, bt);
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++) {
@@ -254,6 +268,7 @@ void or_edge_to_face(const orc *h, real *bt, real *btmf) {
   size_t n4 = NN * 3;
   real *btc = (real *)calloc(n4, sizeof(real));
   or_periodic(h, bt);
+  OMP_FOR
   for (int k = 2; k <= h->nz; k++)
     for (int j = 2; j <= h->ny; j++)
       for (int i = 2; i <= h->nx; i++) {
@@ -262,6 +277,7 @@ void or_edge_to_face(const orc *h, real *bt, real *btmf) {
         for (int m = 1; m <= 3; m++) btc[F4(i, j, k, m)] = o[m - 1];
       }
   or_periodic(h, btc);
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++) {
@@ -276,6 +292,7 @@ void or_edge_to_face(const orc *h, real *bt, real *btmf) {
 /* face_to_center, grid_interp.f:136-176 */
 void or_face_to_center(const orc *h, real *v, real *vc) {
   or_periodic(h, v);
+  OMP_FOR
   for (int k = 2; k <= h->nz; k++)
     for (int j = 2; j <= h->ny; j++)
       for (int i = 2; i <= h->nx; i++) {
@@ -292,6 +309,7 @@ void or_crossf2(orc *h, real *aa, real *btc, real *cc) {
   real *ct = h->ct;
   or_periodic(h, aa);
   or_periodic(h, btc);
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++) {
@@ -303,6 +321,7 @@ void or_crossf2(orc *h, real *aa, real *btc, real *cc) {
         ct[F4(i, j, k, 3)] = ax * by - ay * bx;
       }
   or_periodic(h, ct);
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++) {
@@ -317,6 +336,7 @@ void or_crossf2(orc *h, real *aa, real *btc, real *cc) {
    aj ghosts are not written (gutsf.f:241). */
 void or_curlB(const orc *h, real *b1, const real *np, real *aj) {
   or_periodic(h, b1);
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++) {
@@ -338,6 +358,7 @@ void or_curlB(const orc *h, real *b1, const real *np, real *aj) {
 
 /* curlE, gutsf.f:249-308 (interior only; callers never read its ghosts) */
 void or_curlE(const orc *h, const real *E, real *cE) {
+  OMP_FOR
   for (int i = 2; i <= h->nx - 1; i++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int k = 2; k <= h->nz - 1; k++) {
@@ -354,6 +375,7 @@ void or_curlE(const orc *h, const real *E, real *cE) {
 /* E = c + nu*aj on the interior, then periodic (gutsf.f:1078-1089,
    :1278-1288) */
 static void e_from_c(orc *h, real *E, const real *aj, const real *nu) {
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++)
@@ -364,6 +386,7 @@ static void e_from_c(orc *h, real *E, const real *aj, const real *nu) {
 
 /* a = aj - up on the interior, into module scratch a (gutsf.f:1058-1064) */
 static void a_from_aj_up(orc *h, const real *aj, const real *up) {
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++)
@@ -395,6 +418,7 @@ void or_predict_B(orc *h, real *b1, real *b12, real *b1p2, real *bt, real *E,
   real *cE = (real *)calloc(n4, sizeof(real));
   or_get_E(h, E, bt, aj, up, np, nu);
   or_curlE(h, E, cE);
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++)
@@ -413,6 +437,7 @@ void or_get_Ep1(orc *h, real *E, real *b1, real *b1p2, real *aj, real *up,
   real *btp1mf = (real *)calloc(n4, sizeof(real));
   real *btc = (real *)calloc(n4, sizeof(real));
   real *aa = (real *)calloc(n4, sizeof(real));
+  OMP_FOR
   for (size_t n = 0; n < n4; n++) btp1[n] = R(0.5) * (b1p2[n] + b1[n]);
   or_curlB(h, btp1, np, aj);
   a_from_aj_up(h, aj, up);
@@ -431,6 +456,7 @@ void or_correct_B(orc *h, real *b1, real *b1p2, real *E, real *aj, real *up,
   real *cE = (real *)calloc(n4, sizeof(real));
   or_get_Ep1(h, E, b1, b1p2, aj, up, np, nu);
   or_curlE(h, E, cE);
+  OMP_FOR
   for (int k = 2; k <= h->nz - 1; k++)
     for (int j = 2; j <= h->ny - 1; j++)
       for (int i = 2; i <= h->nx - 1; i++)
@@ -443,6 +469,7 @@ void or_correct_B(orc *h, real *b1, real *b1p2, real *E, real *aj, real *up,
 /* f_update_tlev, gutsf.f:10-35 (the B0 add is commented out there) */
 void or_f_update_tlev(const orc *h, real *b1, real *b12, const real *b1p2, real *bt) {
   size_t n4 = NN * 3;
+  OMP_FOR
   for (size_t n = 0; n < n4; n++) {
     bt[n] = b1p2[n];
     b12[n] = b1[n];
@@ -493,6 +520,7 @@ static int search_cell(const real *q, int n, real x) {
 /* get_interp_weights, gutsp.f:1378-1435 */
 void or_get_interp_weights(const orc *h, int Ni_tot, int Ni_max, const real *xp,
                            int32_t *ijkp, real *wght) {
+  OMP_FOR
   for (int l = 1; l <= Ni_tot; l++) {
     real x = P(xp, l, 1), y = P(xp, l, 2), z = P(xp, l, 3);
     int i = search_cell(h->qx, h->nx, x);
@@ -548,33 +576,79 @@ static void fold(const orc *h, real *g, int ncomp) {
   }
 }
 
-/* update_np, gutsp.f:1441-1512 (one rank: the allreduce is the identity) */
-void or_update_np(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
-                  const real *wght, const real *beta_p, real *np) {
-  memset(np, 0, sizeof(real) * NN);
-  for (int l = 1; l <= Ni_tot; l++) {
+/* Deposit particles l0..l1 (1-based, inclusive) of update_np into g. */
+static void np_deposit(const orc *h, int l0, int l1, int Ni_max, const int32_t *ijkp,
+                       const real *wght, const real *beta_p, real *g) {
+  for (int l = l0; l <= l1; l++) {
     int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
     real bp = beta_p ? beta_p[l - 1] : R(1.0);
     real volb = R(1.0) / (h->dx_grid[i - 1] * h->dy_grid[j - 1] * h->dz_grid[k - 1] * h->beta * bp);
     for (int c = 1; c <= 8; c++) {
       int ci, cj, ck;
       corner(c, i, j, k, &ci, &cj, &ck);
-      np[F3(ci, cj, ck)] += P(wght, l, c) * volb;
+      g[F3(ci, cj, ck)] += P(wght, l, c) * volb;
     }
+  }
+}
+
+/* Particle deposits with h->nthreads > 1: thread t sums particles
+   [1 + t N/T, (t+1) N/T] into its own zeroed copy of the ncomp-component grid,
+   then out = the copies added in thread order. */
+typedef void (*deposit_fn)(const orc *h, int l0, int l1, const void *args, real *g);
+static void deposit_threads(const orc *h, int N, int ncomp, deposit_fn fn, const void *args,
+                            real *out) {
+  const int T = h->nthreads;
+  const size_t len = NN * (size_t)ncomp;
+  real *priv = (real *)calloc((size_t)T * len, sizeof(real));
+#pragma omp parallel num_threads(T)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num();
+#else
+    const int t = 0;
+#endif
+    const int l0 = 1 + (int)((int64_t)t * N / T), l1 = (int)((int64_t)(t + 1) * N / T);
+    fn(h, l0, l1, args, priv + (size_t)t * len);
+  }
+#pragma omp parallel for schedule(static) num_threads(T)
+  for (size_t q = 0; q < len; q++) {
+    real a = priv[q];
+    for (int t = 1; t < T; t++) a += priv[(size_t)t * len + q];
+    out[q] = a;
+  }
+  free(priv);
+}
+
+typedef struct {
+  int Ni_max;
+  const int32_t *ijkp;
+  const real *wght, *beta_p, *vp, *np;
+} dep_args;
+static void np_deposit_fn(const orc *h, int l0, int l1, const void *a, real *g) {
+  const dep_args *d = (const dep_args *)a;
+  np_deposit(h, l0, l1, d->Ni_max, d->ijkp, d->wght, d->beta_p, g);
+}
+
+/* update_np, gutsp.f:1441-1512 (one rank: the allreduce is the identity) */
+void or_update_np(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
+                  const real *wght, const real *beta_p, real *np) {
+  memset(np, 0, sizeof(real) * NN);
+  if (h->nthreads > 1) {
+    dep_args a = {Ni_max, ijkp, wght, beta_p, NULL, NULL};
+    deposit_threads(h, Ni_tot, 1, np_deposit_fn, &a, np);
+  } else {
+    np_deposit(h, 1, Ni_tot, Ni_max, ijkp, wght, beta_p, np);
   }
   fold(h, np, 1);
   or_periodic_scalar(h, np);
 }
 
-/* update_up, gutsp.f:1649-1842 */
-void or_update_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
-                  const real *wght, const real *beta_p, const real *vp, real *np,
-                  real *up) {
-  real *ct = h->ct;
-  (void)np;
-  memset(up, 0, sizeof(real) * NN * 3);
-  memset(ct, 0, sizeof(real) * NN * 3);
-  for (int l = 1; l <= Ni_tot; l++) {
+/* Deposit particles l0..l1 of update_up into ct (gutsp.f:1704-1756: a
+   division by np per particle and corner) */
+static void up_deposit(const orc *h, int l0, int l1, int Ni_max, const int32_t *ijkp,
+                       const real *wght, const real *beta_p, const real *vp, const real *np,
+                       real *ct) {
+  for (int l = l0; l <= l1; l++) {
     int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
     real bp = beta_p ? beta_p[l - 1] : R(1.0);
     real volb = (QX(i + 1) - QX(i)) * (QY(j + 1) - QY(j)) * (QZ(k + 1) - QZ(k)) * h->beta * bp;
@@ -586,8 +660,28 @@ void or_update_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
         ct[F4(ci, cj, ck, m)] += P(vp, l, m) * P(wght, l, c) * nvolb;
     }
   }
+}
+static void up_deposit_fn(const orc *h, int l0, int l1, const void *a, real *g) {
+  const dep_args *d = (const dep_args *)a;
+  up_deposit(h, l0, l1, d->Ni_max, d->ijkp, d->wght, d->beta_p, d->vp, d->np, g);
+}
+
+/* update_up, gutsp.f:1649-1842 */
+void or_update_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
+                  const real *wght, const real *beta_p, const real *vp, real *np,
+                  real *up) {
+  real *ct = h->ct;
+  memset(up, 0, sizeof(real) * NN * 3);
+  memset(ct, 0, sizeof(real) * NN * 3);
+  if (h->nthreads > 1) {
+    dep_args a = {Ni_max, ijkp, wght, beta_p, vp, np};
+    deposit_threads(h, Ni_tot, 3, up_deposit_fn, &a, ct);
+  } else {
+    up_deposit(h, 1, Ni_tot, Ni_max, ijkp, wght, beta_p, vp, np, ct);
+  }
   fold(h, ct, 3);
   or_periodic(h, ct);
+  OMP_FOR
   for (int k = 1; k <= h->nz - 1; k++)
     for (int j = 1; j <= h->ny - 1; j++)
       for (int i = 1; i <= h->nx - 1; i++) {
@@ -603,8 +697,10 @@ void or_extrapol_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp,
                     const real *wght, const real *beta_p, const real *vp,
                     const real *vp1, real *np, real *up) {
   real *v = (real *)malloc(sizeof(real) * (size_t)Ni_max * 3);
-  for (int m = 1; m <= 3; m++)
+  for (int m = 1; m <= 3; m++) {
+    OMP_FOR
     for (int l = 1; l <= Ni_tot; l++) P(v, l, m) = R(1.5) * P(vp, l, m) - R(0.5) * P(vp1, l, m);
+  }
   or_update_up(h, Ni_tot, Ni_max, ijkp, wght, beta_p, v, np, up);
   free(v);
 }
@@ -626,6 +722,7 @@ void or_get_Ep(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp, const real *
   real *upc = (real *)calloc(n4, sizeof(real));
   or_face_to_center(h, aj, ajc);
   or_face_to_center(h, up, upc);
+  OMP_FOR
   for (int l = 1; l <= Ni_tot; l++) {
     int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
     real w8[8];
@@ -651,8 +748,11 @@ void or_get_vplus_vminus(const orc *h, int Ni_tot, int Ni_max, const int32_t *ij
                          const real *wght, const real *mrat, const real *Ep,
                          const real *btc, const real *vp, real *vplus, real *vminus) {
   real dt = h->dt;
-  for (int m = 1; m <= 3; m++)
+  for (int m = 1; m <= 3; m++) {
+    OMP_FOR
     for (int l = 1; l <= Ni_tot; l++) P(vminus, l, m) = P(vp, l, m) + R(0.5) * dt * P(Ep, l, m);
+  }
+  OMP_FOR
   for (int l = 1; l <= Ni_tot; l++) {
     int i = P(ijkp, l, 1), j = P(ijkp, l, 2), k = P(ijkp, l, 3);
     real w8[8];
@@ -682,20 +782,24 @@ void or_get_vplus_vminus(const orc *h, int Ni_tot, int Ni_max, const int32_t *ij
 void or_improve_up(orc *h, int Ni_tot, int Ni_max, const int32_t *ijkp, const real *wght,
                    const real *beta_p, real *vp1, const real *vplus, const real *vminus,
                    real *np, real *up) {
-  for (int m = 1; m <= 3; m++)
+  for (int m = 1; m <= 3; m++) {
+    OMP_FOR
     for (int l = 1; l <= Ni_tot; l++)
       P(vp1, l, m) = R(0.5) * (P(vplus, l, m) + P(vminus, l, m));
+  }
   or_update_up(h, Ni_tot, Ni_max, ijkp, wght, beta_p, vp1, np, up);
 }
 
 /* get_vp_final, gutsp.f:433-449 */
 void or_get_vp_final(const orc *h, int Ni_tot, int Ni_max, const real *Ep, real *vp,
                      real *vp1, const real *vplus) {
-  for (int m = 1; m <= 3; m++)
+  for (int m = 1; m <= 3; m++) {
+    OMP_FOR
     for (int l = 1; l <= Ni_tot; l++) {
       P(vp1, l, m) = P(vp, l, m);
       P(vp, l, m) = P(vplus, l, m) + R(0.5) * h->dt * P(Ep, l, m);
     }
+  }
 }
 
 /* move_ion_half, gutsp.f:454-570: x += dt/2 v, one periodic wrap per axis
@@ -705,6 +809,7 @@ void or_move_ion_half(const orc *h, int Ni_tot, int Ni_max, real *xp, const real
   real dth = h->dt / R(2.);
   const real *q[3] = {h->qx, h->qy, h->qz};
   int n[3] = {h->nx, h->ny, h->nz};
+  OMP_FOR
   for (int l = 1; l <= Ni_tot; l++)
     for (int m = 1; m <= 3; m++) {
       real x = P(xp, l, m) + dth * P(vp, l, m);

@@ -126,6 +126,11 @@ void or_energy_diag(const orc *h, const or_state *s, double mion, const double b
 /* kinetic energy / mion of ions [l0, l1) (1-based l0, exclusive l1) */
 double or_ke(const orc *h, const or_state *s, int l0, int l1);
 void or_set_open_x(orc *h, int on);
+/* OpenMP threads of the particle, grid and deposit loops (default 1: the
+   serial restatement).  With more threads each deposits its share of the
+   particles into its own grid and the grids are added in thread order:
+   deterministic for a given count, different rounding across counts. */
+void or_set_threads(orc *h, int n);
 int or_remove_escaped(const orc *h, or_state *s);
 /* inflow through the open x face in every later or_step (ppc = 0: none);
    next_id: the id of the first ion it appends */

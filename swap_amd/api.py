@@ -56,7 +56,8 @@ class Diag(C.Structure):
     _fields_ = [("charge", C.c_double), ("momentum", C.c_double * 3), ("kinetic", C.c_double),
                 ("divb_l2", C.c_double), ("b_energy", C.c_double), ("nparticles", C.c_int64),
                 ("escaped", C.c_int64), ("injected", C.c_int64),
-                ("window_misses", C.c_int64)]
+                ("window_misses", C.c_int64), ("shrunk_windows", C.c_int64),
+                ("sorts", C.c_int64)]
 
 
 class Cloud(C.Structure):
@@ -133,6 +134,9 @@ def load_library():
         "swap_diagnostics": ([vp, C.POINTER(Diag)], i32),
         "swap_energy_diag": ([vp, dbl, vp, C.POINTER(Energy)], i32),
         "swap_set_input_energy": ([vp, dbl, dbl], i32),
+        "swap_run_state_size": ([vp, C.POINTER(i64)], i32),
+        "swap_save_run_state": ([vp, vp], i32),
+        "swap_restore_run_state": ([vp, vp], i32),
         "swap_profile": ([vp, i32], i32),
         "swap_profile_read": ([vp, vp, vp, i32], i32),
         "swap_set_seed": ([vp, C.c_uint64], i32),
@@ -490,10 +494,23 @@ class Swap:
     def set_input_energy(self, mion, input_E):
         self._ck(self.lib.swap_set_input_energy(self.h, mion, input_E), "set_input_energy")
 
+    def run_state(self):
+        """the run state a checkpoint needs beside particles and B (opaque uint8)"""
+        nb = C.c_int64()
+        self._ck(self.lib.swap_run_state_size(self.h, C.byref(nb)), "run_state_size")
+        buf = np.zeros(nb.value, np.uint8)
+        self._ck(self.lib.swap_save_run_state(self.h, buf.ctypes.data), "save_run_state")
+        return buf
+
+    def restore_run_state(self, buf):
+        buf = np.ascontiguousarray(buf, np.uint8)
+        self._ck(self.lib.swap_restore_run_state(self.h, buf.ctypes.data), "restore_run_state")
+
     def diagnostics(self):
         d = Diag()
         self._ck(self.lib.swap_diagnostics(self.h, C.byref(d)), "diagnostics")
         return {"charge": d.charge, "momentum": list(d.momentum), "kinetic": d.kinetic,
                 "divb_l2": d.divb_l2, "b_energy": d.b_energy, "nparticles": d.nparticles,
                 "escaped": d.escaped, "injected": d.injected,
-                "window_misses": d.window_misses}
+                "window_misses": d.window_misses, "shrunk_windows": d.shrunk_windows,
+                "sorts": d.sorts}

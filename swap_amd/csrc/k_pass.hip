@@ -239,7 +239,9 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     // a P3 window over the LDS budget (boxes grown by a few steps of drift)
     // gives up cells of its shift range one at a time, the widest axis's
     // side farther from the range's centre first: the bulk keeps the window
-    // and only the fastest particles deposit straight to HBM
+    // and only the fastest particles deposit straight to HBM (counted in
+    // dcnt[5] per item, beside the misses in dcnt[0])
+    if (tid == 0 && (int64_t)W[0] * W[1] * W[2] * M > WIN_VALUES) atomicAdd(dcnt + 5, 1ull);
     for (int it = 0; it < 24 && (int64_t)W[0] * W[1] * W[2] * M > WIN_VALUES; it++) {
       int a = 0;
       for (int b = 1; b < 3; b++)

@@ -505,5 +505,6 @@ int so_sort(swap_ctx* c) {
   if (int e = post(c)) return e;
   c->n_sorted = n;
   c->steps_since_sort = 0;
+  c->sorts++;
   return 0;
 }

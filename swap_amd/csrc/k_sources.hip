@@ -519,7 +519,7 @@ int sr_diagnostics(swap_ctx* c, swap_diag* d) {
                      c->cfg.dy, c->cfg.dz, c->dred);
   if (int e = post(c)) return e;
   double h[8];
-  unsigned long long cnt[3];
+  unsigned long long cnt[6];
   HIPCHK(c, hipMemcpyAsync(h, c->dred, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(cnt, c->dcnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -532,6 +532,8 @@ int sr_diagnostics(swap_ctx* c, swap_diag* d) {
   d->escaped = c->escaped;
   d->injected = c->injected;
   d->window_misses = (int64_t)cnt[0];
+  d->shrunk_windows = (int64_t)cnt[5];
+  d->sorts = c->sorts;
   return 0;
 }
 

@@ -2,6 +2,7 @@
 // the routine-level entry points and the fused time step.
 #include <stdarg.h>
 #include <stdio.h>
+#include <string.h>
 #include <algorithm>
 #include <cmath>
 #include <new>
@@ -412,6 +413,14 @@ int swap_load_particles_split(swap_ctx* c, int64_t n, const uint32_t* cell, cons
     if (cell_i(v) >= c->cfg.ncx || cell_j(v) >= c->cfg.ncy || cell_k(v) >= c->cfg.ncz)
       return set_err(c, SWAP_EINVAL, "load: particle %lld has a cell outside the grid",
                      (long long)q);
+  }
+  // fractions lie in (0, 1] (right-closed cells); anything else would give
+  // negative or extrapolated weights
+  for (int64_t q = 0; q < 3 * n; q++) {
+    const double f = c->prec == 64 ? ((const double*)frac)[q] : (double)((const float*)frac)[q];
+    if (!(f > 0.0 && f <= 1.0))
+      return set_err(c, SWAP_EINVAL, "load: particle %lld has a fraction %g outside (0, 1]",
+                     (long long)(q % n), f);
   }
   c->n = n;
   c->next_id = n;
@@ -949,6 +958,118 @@ int swap_set_input_energy(swap_ctx* c, double mion, double input_E) {
   HIPCHK(c, hipMemcpyAsync(c->dred + DRED_INPUT, &k, sizeof(double), hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ---------------- run state (checkpoint / resume) ----------------
+
+namespace {
+constexpr char RS_MAGIC[8] = {'S', 'W', 'A', 'P', 'R', 'S', '0', '1'};
+struct RunStateHead {
+  char magic[8];
+  int64_t max_items, vb_ints, ntiles;  // shape of the arrays that follow
+  int64_t n, step_count, steps_since_sort, n_sorted, next_id, injected, escaped, sorts;
+  uint64_t box_over;
+  double in_ppc, in_vsw, in_vth, in_bp, input_k;
+  int32_t vb_valid, nitems;
+  uint64_t counters[8];  // dcnt: window misses, escapes, scratch, chex, box overflows, ...
+};
+size_t run_state_bytes(const swap_ctx* c) {
+  return sizeof(RunStateHead) + sizeof(Item) * (size_t)c->max_items +
+         24 * (size_t)c->max_items + sizeof(int) * (size_t)vb_ints(c);
+}
+}  // namespace
+
+int swap_run_state_size(swap_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return SWAP_EINVAL;
+  *bytes = (int64_t)run_state_bytes(c);
+  return 0;
+}
+
+int swap_save_run_state(swap_ctx* c, void* host) {
+  if (!c || !host) return SWAP_EINVAL;
+  RunStateHead h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, RS_MAGIC, 8);
+  h.max_items = c->max_items;
+  h.vb_ints = vb_ints(c);
+  h.ntiles = c->ntiles;
+  h.n = c->n;
+  h.step_count = c->step_count;
+  h.steps_since_sort = c->steps_since_sort;
+  h.n_sorted = c->n_sorted;
+  h.next_id = c->next_id;
+  h.injected = c->injected;
+  h.escaped = c->escaped;
+  h.sorts = c->sorts;
+  h.box_over = c->box_over;
+  h.in_ppc = c->in_ppc;
+  h.in_vsw = c->in_vsw;
+  h.in_vth = c->in_vth;
+  h.in_bp = c->in_bp;
+  h.vb_valid = c->vb_valid ? 1 : 0;
+  char* b = (char*)host + sizeof h;
+  HIPCHK(c, hipMemcpyAsync(&h.input_k, c->dred + DRED_INPUT, sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&h.nitems, c->d_nitems, sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(h.counters, c->dcnt, sizeof h.counters, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(b, c->items, sizeof(Item) * (size_t)c->max_items,
+                           hipMemcpyDeviceToHost, c->stream));
+  b += sizeof(Item) * (size_t)c->max_items;
+  HIPCHK(c, hipMemcpyAsync(b, c->boxes, 24 * (size_t)c->max_items, hipMemcpyDeviceToHost,
+                           c->stream));
+  b += 24 * (size_t)c->max_items;
+  HIPCHK(c, hipMemcpyAsync(b, c->vb, sizeof(int) * (size_t)vb_ints(c), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(host, &h, sizeof h);
+  return 0;
+}
+
+int swap_restore_run_state(swap_ctx* c, const void* host) {
+  if (!c || !host) return SWAP_EINVAL;
+  RunStateHead h;
+  memcpy(&h, host, sizeof h);
+  if (memcmp(h.magic, RS_MAGIC, 8) != 0)
+    return set_err(c, SWAP_EINVAL, "restore_run_state: not a run state");
+  if (h.max_items != c->max_items || h.vb_ints != vb_ints(c) || h.ntiles != c->ntiles)
+    return set_err(c, SWAP_EINVAL, "restore_run_state: saved from a context of another shape");
+  if (h.n != c->n)
+    return set_err(c, SWAP_ESTATE,
+                   "restore_run_state: saved with %lld particles, %lld loaded (load the "
+                   "particles first)", (long long)h.n, (long long)c->n);
+  if (h.n_sorted > c->n || h.n_sorted < -1 || h.nitems < 0 || h.nitems > c->max_items)
+    return set_err(c, SWAP_EINVAL, "restore_run_state: inconsistent sort state");
+  if (h.in_ppc > 0 || c->in_ppc > 0)
+    TRY(sr_set_inflow(c, h.in_ppc, h.in_vsw, h.in_vth, h.in_bp));
+  const char* b = (const char*)host + sizeof h;
+  HIPCHK(c, hipMemcpyAsync(c->items, b, sizeof(Item) * (size_t)c->max_items,
+                           hipMemcpyHostToDevice, c->stream));
+  b += sizeof(Item) * (size_t)c->max_items;
+  HIPCHK(c, hipMemcpyAsync(c->boxes, b, 24 * (size_t)c->max_items, hipMemcpyHostToDevice,
+                           c->stream));
+  b += 24 * (size_t)c->max_items;
+  HIPCHK(c, hipMemcpyAsync(c->vb, b, sizeof(int) * (size_t)vb_ints(c), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_nitems, &h.nitems, sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->dcnt, h.counters, sizeof h.counters, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->dred + DRED_INPUT, &h.input_k, sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->step_count = h.step_count;
+  c->steps_since_sort = (int)h.steps_since_sort;
+  c->n_sorted = h.n_sorted;
+  c->next_id = h.next_id;
+  c->injected = h.injected;
+  c->escaped = h.escaped;
+  c->sorts = h.sorts;
+  c->box_over = h.box_over;
+  c->vb_valid = h.vb_valid != 0;
+  c->mgen++;
   return 0;
 }
 

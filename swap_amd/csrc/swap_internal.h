@@ -94,7 +94,8 @@ struct swap_ctx {
   float* cbp = nullptr;
   // counters
   // [0] window misses [1] escapes [2] scratch [3] chex [4] items whose next
-  // box (reduced by the last P3) outgrows a P1 window
+  // box (reduced by the last P3) outgrows a P1 window [5] P3 windows that
+  // gave up part of their shift range to fit the LDS budget
   unsigned long long* dcnt = nullptr;
   unsigned long long box_over = 0;  // dcnt[4], read back with check_time_step
   int64_t injected = 0, escaped = 0;
@@ -113,6 +114,7 @@ struct swap_ctx {
   void* scratch = nullptr;    // general device scratch
   size_t scratch_bytes = 0;
   int64_t step_count = 0;
+  int64_t sorts = 0;          // counting sorts done
   // learned moment denoiser (k_denoise.hip); null until swap_dn_create
   struct DnState* dn = nullptr;
   // stage timing (swap_profile): event pairs recorded on `stream`

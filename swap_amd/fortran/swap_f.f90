@@ -25,6 +25,15 @@
 !   edge_to_face(bt,btmf)                      grid_interp.f:67
 !   face_to_center(v,vc)                       grid_interp.f:136
 !   periodic(b), periodic_scalar(b)            boundary.f:9, 167
+!   Ionize_sw_mp(np,vp,vp1,xp,m_tstep,input_p,up)     chem_rates.f:508
+!   Ionize_pluto_mp(np,vp,vp1,xp,m_tstep,input_p,up)  chem_rates.f:290
+!   res_chex(xp,vp)                                   chem_rates.f:805
+!
+! The sources read the reference's module inputs (nf_init, omega_p, vsw,
+! beta_pu of inputs.f; the cloud's Qo, vrad, tau_photo, which the reference
+! leaves undefined) from swap_ref_set_sw and swap_ref_set_cloud; they append
+! to the host arrays, raise Ni_tot and keep the library's per-ion beta_p
+! and mrat (swap_ref_get_species copies them to the host).
 !
 ! These are the strict forms: every call moves its REAL*4 host arguments to
 ! the device, runs the routine there and moves the results (and the ghost
@@ -58,6 +67,13 @@ module swap_hip_c
      integer(c_int) :: precision, flags, sort_every
      integer(c_int64_t) :: seed
   end type swap_config
+
+  ! the neutral cloud of swap_ionize_cloud
+  type, bind(C) :: swap_cloud
+     real(c_double) :: cx, cy, cz, N0, r0, H, tau_photo, sigma_chex, mrat_pu, beta_min
+     integer(c_int) :: max_per_cell
+     real(c_double) :: rmin
+  end type swap_cloud
 
   ! Energy_diag's record (part_init.f:14-139), J
   type, bind(C) :: swap_energy
@@ -228,6 +244,22 @@ module swap_hip_c
        type(c_ptr), value :: ctx
        real(c_double), value :: mion, input_E
      end function swap_set_input_energy
+     integer(c_int) function swap_ionize_sw(ctx, step, ndot, vsw, beta_pu, injected) &
+          bind(C, name="swap_ionize_sw")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: step
+       real(c_double), value :: ndot, vsw, beta_pu
+       integer(c_int64_t), intent(out) :: injected
+     end function swap_ionize_sw
+     integer(c_int) function swap_ionize_cloud(ctx, step, cloud, injected, chex) &
+          bind(C, name="swap_ionize_cloud")
+       import :: c_int, c_ptr, c_int64_t, swap_cloud
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: step
+       type(swap_cloud), intent(in) :: cloud
+       integer(c_int64_t), intent(out) :: injected, chex
+     end function swap_ionize_cloud
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
@@ -240,12 +272,18 @@ module swap_ref
   use iso_c_binding
   use swap_hip_c
   implicit none
-  private :: chk, fset, fget, pset, pget, xp_set, xp_get
+  private :: chk, fset, fget, pset, pget, xp_set, xp_get, pull_new, redeposit
 
   type(c_ptr), save :: swap_ctx_h = c_null_ptr
   integer, save :: nx = 0, ny = 0, nz = 0   ! grid incl. ghosts (dimensions.f:5)
   integer, save :: Ni_tot = 0               ! particles on the device
   real, save :: dtsub = 0.0, ntf = 0.0      ! subcycle step and count (gutsf.f:1776)
+  integer, save :: Ni_max = 0               ! particle capacity (dimensions.f)
+  real(8), save :: beta_s = 0d0, mion_s = 1.67d-27   ! beta (misc.f:462) and mion (inputs.f)
+  ! Ionize_sw_mp's inputs: nf_init, omega_p, vsw, beta_pu (inputs.f)
+  real(8), save :: sw_nf_init = 0d0, sw_omega_p = 0d0, sw_vsw = 0d0, sw_beta_pu = 1d0
+  type(swap_cloud), save :: cloud_s
+  integer, save :: src_step = 0             ! m_tstep of the last source call
 
 contains
 
@@ -269,21 +307,64 @@ contains
   end subroutine chk
 
   ! -------- set-up --------
+  ! flags: SWAP_SPECIES keeps per-ion beta_p and mrat (needed by the
+  ! sources), SWAP_TRACK_IDS per-ion ids; capacity is the reference's Ni_max
   subroutine swap_ref_init(ncx, ncy, ncz, dx, dy, delz, dt, ntsub, alpha, beta, nu, &
-       capacity, sort_every, seed)
+       capacity, sort_every, seed, flags)
     integer, intent(in) :: ncx, ncy, ncz, ntsub, sort_every
     real(8), intent(in) :: dx, dy, delz, dt, alpha, beta, nu
     integer(8), intent(in) :: capacity, seed
+    integer, intent(in), optional :: flags
     type(swap_config) :: cfg
     cfg%ncx = ncx; cfg%ncy = ncy; cfg%ncz = ncz
     cfg%dx = dx; cfg%dy = dy; cfg%dz = delz; cfg%dt = dt
     cfg%ntsub = ntsub; cfg%alpha = alpha; cfg%beta = beta; cfg%nu = nu
     cfg%capacity = capacity; cfg%precision = SWAP_FP32; cfg%flags = 0
+    if (present(flags)) cfg%flags = flags
     cfg%sort_every = sort_every; cfg%seed = seed
     call chk(swap_create(cfg, swap_ctx_h), 'swap_create')
     nx = ncx + 2; ny = ncy + 2; nz = ncz + 2
     dtsub = real(dt / ntsub); ntf = real(ntsub)
+    Ni_max = int(capacity)
+    beta_s = beta
   end subroutine swap_ref_init
+
+  ! the inputs of Ionize_sw_mp (inputs.f: nf_init, omega_p, vsw, beta_pu; the
+  ! ion mass mion of the input_p account)
+  subroutine swap_ref_set_sw(nf_init, omega_p, vsw, beta_pu, mion)
+    real(8), intent(in) :: nf_init, omega_p, vsw, beta_pu, mion
+    sw_nf_init = nf_init; sw_omega_p = omega_p; sw_vsw = vsw; sw_beta_pu = beta_pu
+    mion_s = mion
+  end subroutine swap_ref_set_sw
+
+  ! the neutral cloud of Ionize_pluto_mp and res_chex: n_n(r) = N0 (r0/r)^2
+  ! exp(-r/H) about (cx, cy, cz), r >= rmin (the reference's Qo, vrad,
+  ! tau_photo and S_radius are undefined, chem_rates.f:350-352)
+  subroutine swap_ref_set_cloud(cx, cy, cz, N0, r0, H, tau_photo, sigma_chex, mrat_pu, &
+       beta_min, max_per_cell, rmin)
+    real(8), intent(in) :: cx, cy, cz, N0, r0, H, tau_photo, sigma_chex, mrat_pu, beta_min, rmin
+    integer, intent(in) :: max_per_cell
+    cloud_s%cx = cx; cloud_s%cy = cy; cloud_s%cz = cz; cloud_s%N0 = N0; cloud_s%r0 = r0
+    cloud_s%H = H; cloud_s%tau_photo = tau_photo; cloud_s%sigma_chex = sigma_chex
+    cloud_s%mrat_pu = mrat_pu; cloud_s%beta_min = beta_min
+    cloud_s%max_per_cell = max_per_cell; cloud_s%rmin = rmin
+  end subroutine swap_ref_set_cloud
+
+  ! the library's per-ion beta_p and mrat (global.f) into host arrays
+  subroutine swap_ref_get_species(beta_p, mrat)
+    real, intent(inout) :: beta_p(:), mrat(:)
+    real(c_float), allocatable, target :: t(:)
+    allocate(t(Ni_tot))
+    call chk(swap_get_particle_array(swap_ctx_h, SWAP_P_BETA, c_loc(t)), 'get beta_p')
+    beta_p(1:Ni_tot) = t
+    call chk(swap_get_particle_array(swap_ctx_h, SWAP_P_MRAT, c_loc(t)), 'get mrat')
+    mrat(1:Ni_tot) = t
+  end subroutine swap_ref_get_species
+
+  ! ions on the device (the reference's Ni_tot)
+  integer function swap_ref_ni_tot()
+    swap_ref_ni_tot = Ni_tot
+  end function swap_ref_ni_tot
 
   subroutine swap_ref_finalize()
     if (c_associated(swap_ctx_h)) call chk(swap_destroy(swap_ctx_h), 'swap_destroy')
@@ -567,6 +648,110 @@ contains
     call pget(SWAP_P_VP, vp)
     call pget(SWAP_P_VP1, vp1)
   end subroutine get_vp_final
+
+  ! -------- chem_rates.f --------
+  ! the host rows l0..Ni_tot of xp, vp, vp1 from the device (ions just added)
+  subroutine pull_new(l0, xp, vp, vp1)
+    integer, intent(in) :: l0
+    real, intent(inout) :: xp(:,:), vp(:,:), vp1(:,:)
+    real(c_double), allocatable, target :: x(:,:)
+    real(c_float), allocatable, target :: v(:,:)
+    allocate(x(Ni_tot, 3), v(Ni_tot, 3))
+    call chk(swap_get_particle_array(swap_ctx_h, SWAP_P_XP, c_loc(x)), 'get xp')
+    xp(l0:Ni_tot, :) = real(x(l0:Ni_tot, :))
+    call chk(swap_get_particle_array(swap_ctx_h, SWAP_P_VP, c_loc(v)), 'get vp')
+    vp(l0:Ni_tot, :) = v(l0:Ni_tot, :)
+    call chk(swap_get_particle_array(swap_ctx_h, SWAP_P_VP1, c_loc(v)), 'get vp1')
+    vp1(l0:Ni_tot, :) = v(l0:Ni_tot, :)
+  end subroutine pull_new
+
+  ! the re-deposit that ends the sources (chem_rates.f:641-643)
+  subroutine redeposit(np, up)
+    real, intent(out) :: np(nx,ny,nz), up(nx,ny,nz,3)
+    call chk(swap_get_interp_weights(swap_ctx_h), 'get_interp_weights')
+    call chk(swap_update_np(swap_ctx_h), 'update_np')
+    call fget(SWAP_F_NP, np)
+    call chk(swap_update_up(swap_ctx_h, SWAP_V_CUR), 'update_up')
+    call fget(SWAP_F_UP, up)
+  end subroutine redeposit
+
+  ! Ionize_sw_mp: dNi = ndot dt beta beta_pu dx dy delz nz with
+  ! ndot = 1e-6 nf_init omega_p (chem_rates.f:551-562), ring-distribution
+  ! protons appended (:567-628), input_p += m v / beta * beta_p as the
+  ! reference accumulates it (:614-619), then the re-deposit (:641-643)
+  subroutine Ionize_sw_mp(np, vp, vp1, xp, m_tstep, input_p, up)
+    real, intent(inout) :: np(nx,ny,nz), vp(:,:), vp1(:,:), xp(:,:), input_p(3)
+    real, intent(inout) :: up(nx,ny,nz,3)
+    integer, intent(in) :: m_tstep
+    integer(c_int64_t) :: inj
+    integer :: l0, m
+    real(8) :: ndot
+    call xp_set(xp)
+    call pset(SWAP_P_VP, vp)
+    call pset(SWAP_P_VP1, vp1)
+    ndot = 1d-6 * sw_nf_init * sw_omega_p
+    src_step = m_tstep
+    call chk(swap_ionize_sw(swap_ctx_h, int(m_tstep, c_int), ndot, sw_vsw, sw_beta_pu, inj), &
+         'Ionize_sw_mp')
+    l0 = Ni_tot + 1
+    Ni_tot = Ni_tot + int(inj)
+    if (inj > 0) then
+       call pull_new(l0, xp, vp, vp1)
+       do m = 1, 3
+          input_p(m) = input_p(m) + real(mion_s * sum(real(vp(l0:Ni_tot, m), 8)) / beta_s * &
+               sw_beta_pu)
+       end do
+    end if
+    call redeposit(np, up)
+  end subroutine Ionize_sw_mp
+
+  ! Ionize_pluto_mp, the cloud form (chem_rates.f:290-503): photoionization
+  ! of the neutral cloud, ions at rest in their node's cell with variable
+  ! weights, then the re-deposit (:496-498); no charge exchange here
+  subroutine Ionize_pluto_mp(np, vp, vp1, xp, m_tstep, input_p, up)
+    real, intent(inout) :: np(nx,ny,nz), vp(:,:), vp1(:,:), xp(:,:), input_p(3)
+    real, intent(inout) :: up(nx,ny,nz,3)
+    integer, intent(in) :: m_tstep
+    integer(c_int64_t) :: inj, nchex
+    type(swap_cloud) :: cl
+    integer :: l0
+    call xp_set(xp)
+    call pset(SWAP_P_VP, vp)
+    call pset(SWAP_P_VP1, vp1)
+    src_step = m_tstep
+    cl = cloud_s
+    cl%sigma_chex = 0d0
+    call chk(swap_ionize_cloud(swap_ctx_h, int(m_tstep, c_int), cl, inj, nchex), &
+         'Ionize_pluto_mp')
+    l0 = Ni_tot + 1
+    Ni_tot = Ni_tot + int(inj)
+    if (inj > 0) call pull_new(l0, xp, vp, vp1)
+    call redeposit(np, up)
+  end subroutine Ionize_pluto_mp
+
+  ! res_chex (chem_rates.f:805-843): an ion hit by charge exchange, with
+  ! probability dt n_n sigma |v|, stops and becomes a pickup ion (mrat of the
+  ! cloud's ions).  The neutral density is the cloud's profile (the
+  ! reference's constant 10000e15 stands where its profile is commented out).
+  ! The draws are keyed by the step: m_tstep, by default the step of the last
+  ! source call (the reference draws from its running pad_ranf stream).
+  subroutine res_chex(xp, vp, m_tstep)
+    real, intent(in) :: xp(:,:)
+    real, intent(inout) :: vp(:,:)
+    integer, intent(in), optional :: m_tstep
+    integer(c_int64_t) :: inj, nchex
+    type(swap_cloud) :: cl
+    integer :: st
+    st = src_step
+    if (present(m_tstep)) st = m_tstep
+    call xp_set(xp)
+    call pset(SWAP_P_VP, vp)
+    cl = cloud_s
+    cl%tau_photo = huge(1d0)   ! no photoionization
+    call chk(swap_ionize_cloud(swap_ctx_h, int(st, c_int), cl, inj, nchex), 'res_chex')
+    if (inj /= 0) call chk(-1_c_int, 'res_chex: unexpected new ions')
+    call pget(SWAP_P_VP, vp)
+  end subroutine res_chex
 
   subroutine move_ion_half(xp, vp, vp1, input_p)
     real, intent(inout) :: xp(:,:)

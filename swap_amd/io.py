@@ -35,10 +35,13 @@ files of maind.f:268-333 hold quantities outside the time-step path
 
 Checkpoint (replaces restart.vars, maind.f:588-607): an .npz of the
 configuration, the B levels b1 and b12, the split particle arrays (cell,
-fraction, vp, vp1, beta_p, mrat, id) and the input-energy account.
-Everything else a step reads is recomputed by the step, so a run resumed
-from a checkpoint continues bit for bit (the moment sums are exact and
-independent of particle order).
+fraction, vp, vp1, beta_p, mrat, id) in the device's order, and the
+library's run state (swap_save_run_state: step count, sort schedule and work
+items, id counter, injected/escaped counts, inflow parameters, input-energy
+account).  The moments, E and the node sources are recomputed by the next
+step, so a run resumed from a checkpoint continues bit for bit, also in the
+middle of a sort interval and with an open x and inflow.  The learned
+denoiser's weights are not part of it (swap_dn_get_params / set_params).
 """
 import os
 import struct
@@ -303,8 +306,7 @@ def save_checkpoint(path, s):
         arrays["mrat"] = s.particle_array(api.P_MRAT)
     if s.cfg.flags & api.TRACK_IDS:
         arrays["id"] = s.particle_array(api.P_ID)
-    # the input-energy account without the ion mass (mion = 1)
-    arrays["input_k"] = np.array([s.energy_diag(1.0, (0.0, 0.0, 0.0))["input_E"]])
+    arrays["run_state"] = s.run_state()
     np.savez(path, cfg=np.array([cfg[k] for k in _CFG], np.float64),
              cfg_int=np.array([int(cfg[k]) for k in ("ncx", "ncy", "ncz", "ntsub", "capacity",
                                                     "precision", "flags", "sort_every")],
@@ -327,6 +329,5 @@ def load_checkpoint(path):
                  z["mrat"] if "mrat" in z.files else None)
     if "id" in z.files:
         s.set_particle_array(api.P_ID, z["id"])
-    if "input_k" in z.files:
-        s.set_input_energy(1.0, float(z["input_k"][0]))
+    s.restore_run_state(z["run_state"])
     return s

@@ -66,6 +66,68 @@ def test_checkpoint_resume_is_bit_exact(tmp_path):
         assert np.array_equal(xa[..., oa], xb[..., ob]), which
 
 
+@pytest.mark.parametrize("open_x", [True, False])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_checkpoint_mid_sort_interval_resumes_bit_for_bit(tmp_path, precision, open_x):
+    """Sorting every 4 steps, periodic or an open x with inflow: a checkpoint
+    after step 2 holds the step count (it keys the inflow draws), the sort
+    schedule with its work items, boxes and velocity bounds, the id counter
+    and the counts, so the resumed run takes the same steps as the
+    uninterrupted one: the same arrays in the same order, the same fields,
+    the same books."""
+    p = physics.workload(16, 8, 8, 8)
+    n = p["npart"]
+    a = physics.make_swap(p, capacity=2 * n, precision=precision, sort_every=4,
+                          flags=api.TRACK_IDS | (api.OUTFLOW_X if open_x else 0))
+    a.load_maxwellian(n, p["vsw"], p["vth"])
+    if open_x:
+        a.set_inflow(p["ppc"], p["vsw"], p["vth"])
+    a.step(2)
+    ck = str(tmp_path / "ck.npz")
+    io.save_checkpoint(ck, a)
+    a.step(3)
+    b = io.load_checkpoint(ck)
+    b.step(3)
+    da, db = a.diagnostics(), b.diagnostics()
+    for k in ("nparticles", "escaped", "injected", "sorts", "window_misses"):
+        assert da[k] == db[k], k
+    assert da["sorts"] >= 2
+    assert (da["injected"] > 0 and da["escaped"] > 0) == open_x
+    for f in ("b1", "b12", "np", "up", "E", "aj"):
+        assert np.array_equal(a.get_field(f), b.get_field(f)), f
+    for which in (api.P_ID, api.P_CELL, api.P_FRAC, api.P_VP, api.P_VP1):
+        assert np.array_equal(a.particle_array(which), b.particle_array(which)), which
+    ea, eb = a.energy_diag(p["mion"], (0, 0, p["b0z"])), b.energy_diag(p["mion"], (0, 0, p["b0z"]))
+    assert ea["input_E"] == eb["input_E"]
+
+
+def test_run_state_refuses_another_load():
+    p, a = _run()
+    st = a.run_state()
+    _, b = _run(nc=(12, 8, 6), ppc=4)
+    with pytest.raises(api.SwapError, match="particles"):
+        b.restore_run_state(st)
+    bad = st.copy()
+    bad[:8] = 0
+    with pytest.raises(api.SwapError, match="not a run state"):
+        a.restore_run_state(bad)
+
+
+@pytest.mark.parametrize("f", [0.0, 1.5, -0.25, float("nan")])
+def test_load_split_refuses_fractions_outside_the_cell(f):
+    """fractions lie in (0, 1]: anything else would give negative or
+    extrapolated weights and silently wrong deposits"""
+    p = physics.workload(8, 8, 8, 1)
+    s = physics.make_swap(p, capacity=16)
+    cell = np.zeros(4, np.uint32)
+    frac = np.full((3, 4), 0.5, np.float32)
+    v = np.zeros((3, 4), np.float32)
+    s.load_split(cell, frac, v, v)
+    frac[1, 2] = f
+    with pytest.raises(api.SwapError, match="fraction"):
+        s.load_split(cell, frac, v, v)
+
+
 @pytest.mark.parametrize("precision", [32, 64])
 def test_load_file_is_the_identical_seeded_load(tmp_path, precision):
     """the device's seeded load, saved as a load file, is the oracle's seeded
