@@ -48,34 +48,65 @@ def measured_traffic(kernel, cfg):
     return None if k is None else k["fetch_bytes"] + k["write_bytes"]
 
 
-def cpu_baseline(steps=5):
-    """FP64 oracle (1 thread) on a bounded sample of the workload: the same
-    64 ppc drifting Maxwellian on 64x32x32 cells, `steps` full steps."""
+def _host_cores():
+    """cores this process may use on the host (its affinity set), at most 16
+    (the GPU box's CPU share per GPU), and the CPU model"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return min(n, 16), n, model
+
+
+def cpu_baseline(steps=20):
+    """The oracle (oracle/swap_oracle.c, OpenMP over the host's cores) on a
+    bounded sample of the workload: the same 64 ppc drifting Maxwellian on
+    64x32x32 cells (4.2M protons), `steps` full steps, in the reference's own
+    width (FP32, makefile:1 -real-size 32) and in FP64."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
     from swap_amd import physics
+    cores, ncpu, model = _host_cores()
     p = physics.workload(64, 32, 32, 64)
     n = p["npart"]
     cell, frac, v = O.load_maxwellian(n, 20261203, (64, 32, 32), p["vsw"], p["vth"])
     i, j, k = cell & 2047, (cell >> 11) & 1023, cell >> 21
     xp = np.stack([(i + 1 + frac[0]) * p["dx"], (j + 1 + frac[1]) * p["dx"],
                    (k + 1 + frac[2]) * p["dx"]])
-    o = O.Oracle(66, 34, 34, p["dx"], p["dx"], p["dx"], p["dt"], 10, p["alpha"], p["beta"])
-    st = {nm: o.grid() for nm in ("b1", "b12", "b1p2", "bt", "btc", "up", "aj", "E")}
-    for b in ("b1", "b12", "b1p2", "bt"):
-        st[b][2] = p["b0z"]
-    st.update(np=o.grid(1), nu=np.full((34, 34, 66), p["nu"]), xp=np.ascontiguousarray(xp),
-              vp=np.ascontiguousarray(v), vp1=v.copy(), vplus=o.parts(n), vminus=o.parts(n),
-              Ep=o.parts(n), wght=o.parts(n, 8), ijkp=np.zeros((3, n), np.int32), Ni_tot=n)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        o.step(st)
-    dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle (oracle/swap_oracle.c, 1 thread), 64x32x32 cells x 64 ppc "
-                      "(%s protons), %d full steps incl. 10 field subcycles, %.1f s"
-                      % (format(n, ","), steps, dt)}
+    rate = {}
+    secs = {}
+    for width in (32, 64):
+        o = O.Oracle(66, 34, 34, p["dx"], p["dx"], p["dx"], p["dt"], 10, p["alpha"], p["beta"],
+                     width=width)
+        o.set_threads(cores)
+        dt = o.dtype
+        st = {nm: o.grid() for nm in ("b1", "b12", "b1p2", "bt", "btc", "up", "aj", "E")}
+        for b in ("b1", "b12", "b1p2", "bt"):
+            st[b][2] = p["b0z"]
+        st.update(np=o.grid(1), nu=np.full((34, 34, 66), p["nu"], dt),
+                  xp=np.ascontiguousarray(xp, dt), vp=np.ascontiguousarray(v, dt),
+                  vp1=np.ascontiguousarray(v, dt), vplus=o.parts(n), vminus=o.parts(n),
+                  Ep=o.parts(n), wght=o.parts(n, 8), ijkp=np.zeros((3, n), np.int32), Ni_tot=n)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.step(st)
+        secs[width] = time.perf_counter() - t0
+        rate[width] = n * steps / secs[width]
+    return {"value": rate[32], "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "fp64_value": rate[64], "cpu_model": model, "host_cpus": ncpu,
+            "sample": "oracle/swap_oracle.c with OpenMP on %d of the host's %d cores (%s), "
+                      "64x32x32 cells x 64 ppc (%s protons), %d full steps incl. 10 field "
+                      "subcycles each: FP32 %.1f s (value), FP64 %.1f s (fp64_value)"
+                      % (cores, ncpu, model, format(n, ","), steps, secs[32], secs[64])}
 
 
 def main():

@@ -39,8 +39,11 @@ fraction, vp, vp1, beta_p, mrat, id) in the device's order, and the
 library's run state (swap_save_run_state: step count, sort schedule and work
 items, id counter, injected/escaped counts, inflow parameters, input-energy
 account).  The moments, E and the node sources are recomputed by the next
-step, so a run resumed from a checkpoint continues bit for bit, also in the
-middle of a sort interval and with an open x and inflow.  The learned
+step, so a run resumed from a checkpoint continues as the uninterrupted run
+does, also in the middle of a sort interval and with an open x and inflow:
+bit for bit in the FP32 build (exact fixed-point moment sums), to rounding in
+the FP64 build (its FP64 atomics add in a varying order, so no two FP64 runs
+are bit-identical).  The learned
 denoiser's weights are not part of it (swap_dn_get_params / set_params).
 """
 import os

@@ -73,8 +73,10 @@ def test_checkpoint_mid_sort_interval_resumes_bit_for_bit(tmp_path, precision, o
     after step 2 holds the step count (it keys the inflow draws), the sort
     schedule with its work items, boxes and velocity bounds, the id counter
     and the counts, so the resumed run takes the same steps as the
-    uninterrupted one: the same arrays in the same order, the same fields,
-    the same books."""
+    uninterrupted one: the same books, and in the FP32 build the same arrays
+    in the same order and the same fields bit for bit.  The FP64 build sums
+    its moments with FP64 atomics, whose order varies from run to run, so two
+    FP64 runs -- resumed or not -- agree to rounding (1e-12 here)."""
     p = physics.workload(16, 8, 8, 8)
     n = p["npart"]
     a = physics.make_swap(p, capacity=2 * n, precision=precision, sort_every=4,
@@ -93,12 +95,23 @@ def test_checkpoint_mid_sort_interval_resumes_bit_for_bit(tmp_path, precision, o
         assert da[k] == db[k], k
     assert da["sorts"] >= 2
     assert (da["injected"] > 0 and da["escaped"] > 0) == open_x
-    for f in ("b1", "b12", "np", "up", "E", "aj"):
-        assert np.array_equal(a.get_field(f), b.get_field(f)), f
-    for which in (api.P_ID, api.P_CELL, api.P_FRAC, api.P_VP, api.P_VP1):
-        assert np.array_equal(a.particle_array(which), b.particle_array(which)), which
+    if precision == 32:
+        for f in ("b1", "b12", "np", "up", "E", "aj"):
+            assert np.array_equal(a.get_field(f), b.get_field(f)), f
+        for which in (api.P_ID, api.P_CELL, api.P_FRAC, api.P_VP, api.P_VP1):
+            assert np.array_equal(a.particle_array(which), b.particle_array(which)), which
+    else:
+        for f in ("b1", "b12", "np", "up", "E", "aj"):
+            fa, fb = a.get_field(f), b.get_field(f)
+            assert np.max(np.abs(fa - fb)) <= 1e-12 * np.max(np.abs(fa)), f
+        ia, ib = a.particle_array(api.P_ID), b.particle_array(api.P_ID)
+        oa, ob = np.argsort(ia), np.argsort(ib)
+        assert np.array_equal(ia[oa], ib[ob])
+        for arr_a, arr_b in ((a.xp(), b.xp()), (a.vp(), b.vp())):
+            d = np.abs(arr_a[:, oa] - arr_b[:, ob])
+            assert np.max(d) <= 1e-12 * np.max(np.abs(arr_a)), d.max()
     ea, eb = a.energy_diag(p["mion"], (0, 0, p["b0z"])), b.energy_diag(p["mion"], (0, 0, p["b0z"]))
-    assert ea["input_E"] == eb["input_E"]
+    assert ea["input_E"] == pytest.approx(eb["input_E"], rel=1e-14)
 
 
 def test_run_state_refuses_another_load():
