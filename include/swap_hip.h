@@ -167,6 +167,17 @@ int swap_load_maxwellian(swap_ctx *ctx, int64_t n, double vsw, double vth,
 /* host copy of one particle array (see SWAP_P_*); dtype: FP64 for XP, the
    context precision for real arrays, uint32/int32 for integer arrays */
 int swap_get_particle_array(swap_ctx *ctx, int which, void *host);
+/* No counterpart in the reference: swap_get_particle_array for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, hosts that forbid hipMalloc).  SWAP_P_XP is staged through a fixed
+   device buffer of the library -- SWAP_NOALLOC_MAX particles, 24 MiB on each
+   GPU it runs on, held until the process ends -- instead of a hipMalloc per
+   call; a context with more particles is refused with SWAP_EINVAL.  Every
+   call on one GPU shares that buffer: calls that overlap on one GPU (two host
+   threads, two contexts) race.  Other arrays are copied as
+   swap_get_particle_array copies them, which allocates nothing. */
+#define SWAP_NOALLOC_MAX (1 << 20)
+int swap_get_particle_array_noalloc(swap_ctx *ctx, int which, void *host);
 /* host -> device for one array of the n loaded particles, with the layouts
    and dtypes of swap_get_particle_array.  XP re-splits the positions into
    cell + fraction.  EP, VPLUS, VMINUS, WGHT and IJKP fill the routine-level

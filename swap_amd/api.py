@@ -134,6 +134,7 @@ def load_library():
         "swap_diagnostics": ([vp, C.POINTER(Diag)], i32),
         "swap_energy_diag": ([vp, dbl, vp, C.POINTER(Energy)], i32),
         "swap_set_input_energy": ([vp, dbl, dbl], i32),
+        "swap_get_particle_array_noalloc": ([vp, i32, vp], i32),
         "swap_run_state_size": ([vp, C.POINTER(i64)], i32),
         "swap_save_run_state": ([vp, vp], i32),
         "swap_restore_run_state": ([vp, vp], i32),

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libswap_hip.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["swap_api.hip", "k_fields.hip", "k_particles.hip", "k_pass.hip", "k_sort.hip", "k_sources.hip",
-           "k_denoise.hip"]
+           "k_denoise.hip", "swap_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

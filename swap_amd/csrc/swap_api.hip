@@ -304,6 +304,11 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   DMF(c->boxes, 24 * (size_t)c->max_items);
   DMF(c->boxes_next, 24 * (size_t)c->max_items);
 #undef DMF
+  // the fixed staging buffer of swap_get_particle_array_noalloc on this device
+  if ((e = na_setup(c))) {
+    fprintf(stderr, "swap_create: %s\n", c->err.c_str());
+    return fail(e);
+  }
   // nu = nu_init everywhere (grd6_setup, initial.f:79)
   std::vector<double> nuh(nn, cfg->nu);
   if ((e = host_to_real(c, c->fld[SWAP_F_NU], nn, nuh.data(), nullptr))) return fail(e);

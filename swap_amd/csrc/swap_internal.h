@@ -177,6 +177,9 @@ int pl_maxwellian_full(swap_ctx* c, int64_t off, int64_t n, double vsw, double v
                        float mrv, uint32_t stream);
 int pl_box_reset(swap_ctx* c);
 
+// ---- no-alloc staging (swap_noalloc.hip): device address of its buffer ----
+int na_setup(swap_ctx* c);
+
 // ---- sort (k_sort.hip) ----
 int so_sort(swap_ctx* c);
 // exclusive scan of n uint32 counts into out[0..n], out[n] = total; bsum: n / 4096 + 1
