@@ -1,8 +1,11 @@
 """Build the gfx950 HIP library libswap_hip.so in-tree (hipcc, no JIT).
 
 Every translation unit compiles for gfx950 only, with packed-FP32 VALU forms
-disabled and hardware FP atomics (-munsafe-fp-atomics: the compensated
-deposit relies on the returning hardware add; see swap_common.h).
+disabled and hardware FP atomics (-munsafe-fp-atomics).  The FP32 build's
+moment sums are exact fixed-point integer atomics (swap_common.h) and need
+no FP atomics; the flag serves the FP64 build's deposit (FP64 atomic adds
+into the LDS windows and HBM accumulators) and the FP64 diagnostic
+reductions, which would otherwise compile to compare-and-swap loops.
 """
 import concurrent.futures as cf
 import os

@@ -96,10 +96,18 @@ def test_training_beats_plain_low_ppc_and_gate():
         s.dn_add_pair()
     assert s.dn_num_pairs() == 24
     raw0, net0 = s.dn_eval(16, 24)
-    # untrained: the residual net starts near the identity, no better than raw
-    if not net0 < raw0:
-        with pytest.raises(api.SwapError):
-            s.dn_enable(True)
+    # a net that provably loses: all weights zero and an output bias of 1/2
+    # make the residual net f(x) = x + 1/2, whose error exceeds the raw one
+    # by 1/4 per value; the gate closes and the step keeps plain deposition
+    init = s.dn_params()
+    bad = np.zeros_like(init)
+    bad[-4:] = 0.5
+    s.dn_set_params(bad)
+    raw_b, net_b = s.dn_eval(16, 24)
+    assert raw_b == pytest.approx(raw0) and net_b > raw_b + 0.2
+    with pytest.raises(api.SwapError, match="gate"):
+        s.dn_enable(True)
+    s.dn_set_params(init)
     s.dn_train(400, 2e-3, 16)
     raw, net = s.dn_eval(16, 24)
     assert raw == pytest.approx(raw0)

@@ -172,10 +172,11 @@ def test_sort_permutation_is_stable_and_bit_exact():
     assert np.array_equal(s.particle_array(api.P_ID), perm)
 
 
-def test_compensated_deposit_accuracy():
-    """200k particles in one cell: the (hi, lo) FP32 deposit matches the exact
-    sum of the FP32 weights to a few ulps (plain FP32 accumulation would be
-    off by ~1e-5)."""
+def test_fixed_point_deposit_accuracy():
+    """200k particles in one cell: the fixed-point deposit (each FP32
+    contribution rounded to a 64-bit count of 2^-26, the counts added as
+    integers) matches the exact sum of the FP32 weights to a few ulps (plain
+    FP32 accumulation would be off by ~1e-5)."""
     from swap_amd import physics
     p = physics.workload(8, 8, 8, 1)
     n = 200_000

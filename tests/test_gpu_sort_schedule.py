@@ -4,10 +4,11 @@ cell has been summed, but that run sum is formed in the pass's precision and
 the runs follow the particle order, so schedules agree to FP32 rounding
 rather than bit for bit.  Contexts that sort every step, every 2 steps, and
 at most every 8 steps with the early sort (a work item's box outgrowing its
-deposit window, which a hot plasma reaches after one step) must agree to
-the tolerances of the FP32 step against the FP64 oracle, and the early sort
-must keep window misses rare where a fixed interval of 8 would lose every
-window."""
+deposit window, which a hot plasma reaches within a few steps) must agree to
+the tolerances of the FP32 step against the FP64 oracle, with window misses
+rare.  The early sort fires for the hot plasma (vth = 150 km/s: more than
+the first sort within 5 steps under a cap of 8) and not for the cold one
+(vth = 30 km/s: the first sort only)."""
 import numpy as np
 import pytest
 
@@ -32,7 +33,9 @@ def _run(sort_every, vth):
         a = s.particle_array(which)
         out[nm] = np.empty_like(a)
         out[nm][:, ids] = a
-    out["misses"] = s.diagnostics()["window_misses"]
+    d = s.diagnostics()
+    out["misses"] = d["window_misses"]
+    out["sorts"] = d["sorts"]
     s.close()
     return p, out
 
@@ -53,3 +56,10 @@ def test_sort_schedule_leaves_the_step_unchanged(vth):
             assert rel(r[f], ref[f]) < 2e-3, (se, f)
         # deposits (3 passes per step) that missed their window stay rare
         assert r["misses"] < 0.01 * 3 * p["npart"] * STEPS, (se, r["misses"])
+        if se == 2:  # steps 0, 2, 4, and for the hot plasma possibly early at 1, 3
+            assert (3 <= r["sorts"] <= 5) if vth > 100 else r["sorts"] == 3, r["sorts"]
+        elif vth > 100:
+            assert r["sorts"] >= 2, r["sorts"]  # the early sort fired
+        else:
+            assert r["sorts"] == 1, r["sorts"]  # cold: no early sort in 5 steps
+    assert ref["sorts"] == STEPS
