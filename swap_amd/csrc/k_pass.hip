@@ -51,11 +51,71 @@ __device__ __forceinline__ int unwrap(int c, int centre, int nc) {
   return c;
 }
 
-// window accumulation: exact fixed point in the FP32 build, FP64 otherwise
-__device__ __forceinline__ void win_add(unsigned long long* a, int i, float v, int bits) {
-  atomicAdd(a + i, to_fix(v, bits));
+// ---- moment accumulation into the LDS window ----
+// FP32 build: a pass multiplies each particle's moments by 2^bits of their
+// fixed-point unit once (exact: a power of two), so its FP32 products and
+// run sums are the unit counts themselves, rounded exactly as the products
+// and sums of the unscaled moments would be.  Adding a sum is then rounding
+// to the nearest integer and converting: three instructions for a sum below
+// 2^31 units; in general the 32-bit form or, for |sum| >= 2^31, an exact
+// split into two 32-bit halves, selected per value without a branch.  The
+// FP64 build adds doubles.
+constexpr float TWO31F = 2147483648.f, TWO32F = 4294967296.f;
+__device__ __forceinline__ unsigned long long fix_small(float x) {  // |rint(x)| < 2^31
+  return (unsigned long long)(long long)(int)rintf(x);
 }
-__device__ __forceinline__ void win_add(double* a, int i, double v, int) { atomicAdd(a + i, v); }
+__device__ __forceinline__ unsigned long long fix_any(float x) {  // |rint(x)| < 2^62
+  const float r = rintf(x);
+  const unsigned long long small = (unsigned long long)(long long)(int)r;
+  // |r| >= 2^31: r is a multiple of ulp(r) >= 2^8, and so is the low half
+  // r - hi 2^32 in [0, 2^32): it has at most 24 significant bits and the fma
+  // is exact.  (Below 2^31 the low half of a negative r may need more bits
+  // than FP32 holds; the 32-bit form covers those.)
+  const float hi = floorf(r * (1.f / TWO32F));
+  const float lo = fmaf(hi, -TWO32F, r);
+  const unsigned long long wide =
+      ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)(unsigned)lo;
+  return fabsf(r) < TWO31F ? small : wide;
+}
+// one contribution straight to the HBM accumulators (a window miss)
+__device__ __forceinline__ void gacc_add(unsigned long long* a, int64_t i, float v) {
+  if (v != 0.f) atomicAdd(a + i, fix_any(v));
+}
+__device__ __forceinline__ void gacc_add(double* a, int64_t i, double v) {
+  if (v != 0.0) atomicAdd(a + i, v);
+}
+// A lane's 8 corners x M sums into the window at node `key`.  The corners
+// (the reference's order, gutsp.f:1422-1429) are key + {0, 1} + {0, sz} +
+// {0, sy} (sy, sz: the window's y and z strides) and moment m lies WS values
+// further on (a compile-time stride), so every add is one DS atomic at one
+// of four base addresses plus an immediate offset.  mag bounds |sum| of
+// every value (FP32 build); below 2^30 the three-instruction conversion
+// holds for all 8 x M of them, with one branch.
+template <int M, int WS>
+__device__ __forceinline__ void win_put(unsigned long long* lacc, int key, int sy, int sz,
+                                        const float (&a)[8][M], float mag) {
+  unsigned long long* b[4] = {lacc + key, lacc + key + sz, lacc + key + sy, lacc + key + sy + sz};
+  if (mag < 1073741824.f) {
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+      for (int m = 0; m < M; m++) atomicAdd(b[cc >> 1] + (cc & 1) + m * WS, fix_small(a[cc][m]));
+  } else {
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+      for (int m = 0; m < M; m++) atomicAdd(b[cc >> 1] + (cc & 1) + m * WS, fix_any(a[cc][m]));
+  }
+}
+template <int M, int WS>
+__device__ __forceinline__ void win_put(double* lacc, int key, int sy, int sz,
+                                        const double (&a)[8][M], float) {
+  double* b[4] = {lacc + key, lacc + key + sz, lacc + key + sy, lacc + key + sy + sz};
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+    for (int m = 0; m < M; m++) atomicAdd(b[cc >> 1] + (cc & 1) + m * WS, a[cc][m]);
+}
 
 // ---- segmented inclusive scan over the 64 lanes of a wavefront ----
 // Lane L belongs to the run starting at lane s(L).  Hillis-Steele inside each
@@ -105,14 +165,20 @@ __device__ __forceinline__ void seg_step(T (&a)[8][M], bool cond) {
       a[cc][m] = cond ? a[cc][m] + t : a[cc][m];
     }
 }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step1(float& a, bool cond) {
+  const float t = dppf<CTRL, ROWS>(a);
+  a = cond ? a + t : a;
+}
+// the run sums a and their magnitude bound mag
 template <typename T, int M>
-__device__ __forceinline__ void segscan(T (&a)[8][M], const SegCond& c) {
-  if (c.lv & 1u) seg_step<0x111, 0xf>(a, c.c1);    // row_shr:1
-  if (c.lv & 2u) seg_step<0x112, 0xf>(a, c.c2);    // row_shr:2
-  if (c.lv & 4u) seg_step<0x114, 0xf>(a, c.c4);    // row_shr:4
-  if (c.lv & 8u) seg_step<0x118, 0xf>(a, c.c8);    // row_shr:8
-  if (c.lv & 16u) seg_step<0x142, 0xa>(a, c.c15);  // row_bcast:15
-  if (c.lv & 32u) seg_step<0x143, 0xc>(a, c.c31);  // row_bcast:31
+__device__ __forceinline__ void segscan(T (&a)[8][M], float& mag, const SegCond& c) {
+  if (c.lv & 1u) { seg_step<0x111, 0xf>(a, c.c1); seg_step1<0x111, 0xf>(mag, c.c1); }
+  if (c.lv & 2u) { seg_step<0x112, 0xf>(a, c.c2); seg_step1<0x112, 0xf>(mag, c.c2); }
+  if (c.lv & 4u) { seg_step<0x114, 0xf>(a, c.c4); seg_step1<0x114, 0xf>(mag, c.c4); }
+  if (c.lv & 8u) { seg_step<0x118, 0xf>(a, c.c8); seg_step1<0x118, 0xf>(mag, c.c8); }
+  if (c.lv & 16u) { seg_step<0x142, 0xa>(a, c.c15); seg_step1<0x142, 0xa>(mag, c.c15); }
+  if (c.lv & 32u) { seg_step<0x143, 0xc>(a, c.c31); seg_step1<0x143, 0xc>(mag, c.c31); }
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -147,8 +213,24 @@ __device__ __forceinline__ int xnode(const Geo& g, int u) {
 // KP particles per lane: PASS_KP right after a sort (cells in runs), 1
 // otherwise (the cells of neighbouring particles differ; the scan across the
 // wavefront then merges neighbours that still share a cell).  OX: open x.
+// Waves per SIMD each pass variant is compiled for (its register budget).
+// A 512-thread workgroup is 2 waves on each of a CU's 4 SIMDs, so occupancy
+// moves in whole workgroups: two of them (4 waves per SIMD) share a CU when
+// a wave needs at most 128 VGPRs and the LDS holds both 64 KB windows.  The
+// drifted P2 is held to that (127 VGPRs, no spills; left to itself it took
+// 132 and lost the second workgroup: 5.3 -> 6.7 ms per pass); the drifted P1
+// fits it unasked (104), and holding it there as well cost 0.2 ms per pass
+// (the scheduler's choices, same occupancy).  The drifted P3 would spill some
+// 200 VGPRs at 128 and, like the passes right after a sort (the P2/P3 gather
+// tile takes the LDS of a second workgroup), runs one workgroup per CU with
+// the registers it needs; 2 is the compiler's own default for 512 threads.
+template <typename R, int MODE, int KP>
+struct PassWaves {
+  static constexpr int value = (sizeof(R) == 4 && KP == 1 && MODE == 2) ? 4 : 2;
+};
+
 template <typename R, int MODE, int KP, bool OX>
-__global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
+__global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
                                                   const R* __restrict__ S,
                                                   typename AccOf<R>::T* __restrict__ gacc,
                                                   const Item* __restrict__ items,
@@ -162,6 +244,8 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   // after a sort); drifted boxes gather from HBM/L2
   constexpr bool LTILE = MODE != 1 && sizeof(R) == 4 && KP > 1;
   constexpr int GM0 = MODE == 2 ? 1 : 0;  // moment index of window moment 0
+  constexpr int WS = WIN_VALUES / M;       // window values per moment (LDS stride)
+  constexpr bool FIX = sizeof(R) == 4;     // fixed-point moment sums (FP32 build)
   using A = typename AccOf<R>::T;
   __shared__ A lacc[WIN_VALUES];
   __shared__ float4 gtile[LTILE ? 2 * GTILE_NODES : 1];
@@ -262,7 +346,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   const int Tn = T[0] * T[1] * T[2];
   const bool tiled = LTILE && boxed && Tn <= GTILE_NODES;
   if (windowed)
-    for (int q = tid; q < Wn * M; q += PASS_WG) lacc[q] = (A)0;
+    for (int q = tid; q < Wn * M; q += PASS_WG) lacc[q % Wn + (q / Wn) * WS] = (A)0;
   if constexpr (LTILE) {
     if (tiled)
       for (int q = tid; q < Tn; q += PASS_WG) {
@@ -276,8 +360,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
   if (MODE == 3 && tid < 12) sred[tid] = (tid % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;
   __syncthreads();
 
-  const int coff[8] = {0, 1, W[0] * W[1], 1 + W[0] * W[1], W[0], 1 + W[0],
-                       W[0] + W[0] * W[1], 1 + W[0] + W[0] * W[1]};
+  const int sy = W[0], sz = W[0] * W[1];  // window strides of the y and z corners
   const int toff[8] = {0, 1, T[0] * T[1], 1 + T[0] * T[1], T[0], 1 + T[0],
                        T[0] + T[0] * T[1], 1 + T[0] + T[0] * T[1]};
   int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
@@ -300,6 +383,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     int cur = -1;   // window cell of the running sums acc
     int hkey = -1;  // window cell of hd: the lane's first cell, once the lane left it
     R acc[8][M], hd[8][M];
+    float amag = 0.f, hmag = 0.f;  // bounds of |acc|, |hd| in fixed-point units
 #pragma unroll
     for (int cc = 0; cc < 8; cc++)
 #pragma unroll
@@ -374,6 +458,16 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
           q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
         }
       }
+      // fixed-point units (a power-of-two scale: exact), and the bound of
+      // every |w q| of this particle
+      float pmag = 0.f;
+      if constexpr (FIX) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+          q[m] *= (R)__int_as_float((127 + fix_bits(GM0 + m)) << 23);
+          pmag = fmaxf(pmag, fabsf((float)q[m]));
+        }
+      }
       const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
       const bool inw = windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
                        ly < W[1] - 1 && lz < W[2] - 1;
@@ -384,6 +478,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
           if (cur >= 0) {
             if (hkey < 0) {  // keep the first cell's sums for the end of the chunk
               hkey = cur;
+              hmag = amag;
 #pragma unroll
               for (int cc = 0; cc < 8; cc++)
 #pragma unroll
@@ -392,17 +487,17 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
                   acc[cc][m] = (R)0;
                 }
             } else {  // a cell in the middle of the lane's particles
+              win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
 #pragma unroll
               for (int cc = 0; cc < 8; cc++)
 #pragma unroll
-                for (int m = 0; m < M; m++) {
-                  win_add(lacc, cur + coff[cc] + m * Wn, acc[cc][m], fix_bits(GM0 + m));
-                  acc[cc][m] = (R)0;
-                }
+                for (int m = 0; m < M; m++) acc[cc][m] = (R)0;
             }
+            amag = 0.f;
           }
         }
         cur = key;
+        amag += pmag;
 #pragma unroll
         for (int cc = 0; cc < 8; cc++)
 #pragma unroll
@@ -415,8 +510,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
                               node_of(uy + ((cc >> 2) & 1), g.ncy),
                               node_of(uz + ((cc >> 1) & 1), g.ncz));
 #pragma unroll
-          for (int m = 0; m < M; m++)
-            acc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m], fix_bits(GM0 + m));
+          for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
         }
       }
       if constexpr (MODE == 3) {
@@ -462,13 +556,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
       }
     }
     // the first cell of a lane that moved on is its own
-    if (KP > 1 && hkey >= 0) {
-#pragma unroll
-      for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-        for (int m = 0; m < M; m++)
-          win_add(lacc, hkey + coff[cc] + m * Wn, hd[cc][m], fix_bits(GM0 + m));
-    }
+    if (KP > 1 && hkey >= 0) win_put<M, WS>(lacc, hkey, sy, sz, hd, hmag);
     // the last cells: runs of lanes with the same cell are summed across the
     // wavefront first (sorted particles), and only each run's last lane adds
     // into the window; lanes that share no cell with a neighbour add directly
@@ -477,16 +565,10 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
     if (hm != ~0ull) {
       const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
       const SegCond sc = seg_cond(lane, 63 - __clzll(hm & le));
-      segscan(acc, sc);
+      segscan(acc, amag, sc);
     }
     const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
-    if (tail && cur >= 0) {
-#pragma unroll
-      for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-        for (int m = 0; m < M; m++)
-          win_add(lacc, cur + coff[cc] + m * Wn, acc[cc][m], fix_bits(GM0 + m));
-    }
+    if (tail && cur >= 0) win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
   }
   if (misses) atomicAdd(dcnt, misses);
   if (MODE == 3) {
@@ -529,7 +611,7 @@ __global__ __launch_bounds__(PASS_WG) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
                           node_of(u0[2] + lz, g.ncz));
 #pragma unroll
       for (int m = 0; m < M; m++) {
-        A v = lacc[qn + m * Wn];
+        A v = lacc[qn + m * WS];
         if (v != (A)0) atomicAdd(gacc + node + (int64_t)m * g.nn, v);
       }
     }
