@@ -162,13 +162,17 @@ def main():
 
     def run(k, step0):
         """k steps; returns the particle-steps done"""
-        if not pickup and not outflow:
+        if not pickup:
+            # an open x keeps the ions that left as dead cells until its next
+            # scheduled sort; reading the count would drop them with an extra
+            # sort, so the count is taken before and after the k steps (the
+            # inflow balances the outflow to a fraction of a percent)
+            n0 = s.n
             s.step(k)
-            return s.n * k
+            return (n0 + s.n) * k // 2 if outflow else n0 * k
         done = 0
         for i in range(k):
-            if pickup:
-                s.ionize_cloud(step0 + i, cloud)  # Ionize_pluto_mp before the step
+            s.ionize_cloud(step0 + i, cloud)  # Ionize_pluto_mp before the step
             done += s.n
             s.step(1)
         return done

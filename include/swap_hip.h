@@ -53,11 +53,17 @@ enum {
                               face nodes 1 and ncx is divided by their share
                               of the particle domain (1.5 and 0.5 cells),
                               ions that leave through either x face during
-                              swap_step are dropped by a sort at the end of
-                              the step (swap_diag.escaped counts them), and
-                              swap_set_inflow feeds the x = 0 face.
-                              Needs ncx >= 4 and sort_every >= 1; no
-                              denoiser. */
+                              swap_step stay in the arrays as dead cells,
+                              which every kernel skips, until the next
+                              scheduled sort drops them (swap_diag.escaped
+                              counts them then); entry points that hand out
+                              particles, their count or particle
+                              diagnostics drop them first (a sort), and the
+                              sources do when the capacity would run out.
+                              The capacity must hold the ions that left
+                              since the last sort.  swap_set_inflow feeds
+                              the x = 0 face.  Needs ncx >= 4 and
+                              sort_every >= 1; no denoiser. */
   SWAP_TRACK_IDS = 4,      /* keep a particle id through sorts (tests) */
   SWAP_UNFUSED_FIELDS = 8  /* field subcycles of swap_step as one kernel per
                               reference routine (nine launches) instead of the

@@ -370,42 +370,60 @@ __global__ __launch_bounds__(256) void k_deltat(Geo g, FC f, const double* b, co
     atomicMin((unsigned int*)dmin, min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3])));
 }
 
-// the reference halves dtsub once per failing node, in its loop order
-// (i outer, j, k inner) and against the dtsub current at that node
-// (gutsf.f:1793-1796).  Rare path; one workgroup walks the nodes in order.
+// check_time_step's halving (gutsf.f:1793-1796) on the device: if the grid
+// minimum of deltat fails the first test, the reference halves dtsub once per
+// failing node, in its loop order (i outer, j, k inner) and against the dtsub
+// current at that node; one workgroup walks the nodes in that order (a rare
+// path).  Writes dtsub (in the context precision, as the host's restatement
+// rounds it) and ntf = ntsub 2^h to ts[0], ts[1], where the subcycle kernels
+// and the host read them.
 template <typename R>
-__global__ void k_halving(Geo g, const R* dtn, R dtsub0, int* out_h) {
+__global__ void k_tstep(Geo g, const R* dtn, const float* dmin, double dtsub0, double thr64,
+                        float thr32, int ntsub, double* ts) {
   __shared__ int flag[1024];
-  R dtsub = dtsub0;
+  const float mn = *dmin;
+  const bool maybe = sizeof(R) == 8 ? ((double)mn <= thr64) : (mn <= thr32);
+  R dtsub = (R)dtsub0;
   int h = 0;
-  int64_t total = g.nn;
-  for (int64_t base = 0; base < total; base += 1024) {
-    int64_t r = base + threadIdx.x;
-    int cand = 0;
-    if (r < total) {
-      int k = (int)(r % g.nz);
-      int j = (int)((r / g.nz) % g.ny);
-      int i = (int)(r / ((int64_t)g.nz * g.ny));
-      cand = dtn[gidx(g, i, j, k)] <= (R)2.0 * dtsub0;
-    }
-    flag[threadIdx.x] = cand;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int t = 0; t < 1024 && base + t < total; t++) {
-        if (!flag[t]) continue;
-        int64_t rr = base + t;
-        int k = (int)(rr % g.nz);
-        int j = (int)((rr / g.nz) % g.ny);
-        int i = (int)(rr / ((int64_t)g.nz * g.ny));
-        if (dtn[gidx(g, i, j, k)] <= (R)2.0 * dtsub) {
-          dtsub = dtsub / (R)2.0;
-          h++;
+  if (maybe) {
+    const int64_t total = g.nn;
+    for (int64_t base = 0; base < total; base += 1024) {
+      int64_t r = base + threadIdx.x;
+      int cand = 0;
+      if (r < total) {
+        int k = (int)(r % g.nz);
+        int j = (int)((r / g.nz) % g.ny);
+        int i = (int)(r / ((int64_t)g.nz * g.ny));
+        cand = dtn[gidx(g, i, j, k)] <= (R)2.0 * (R)dtsub0;
+      }
+      flag[threadIdx.x] = cand;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int t = 0; t < 1024 && base + t < total; t++) {
+          if (!flag[t]) continue;
+          int64_t rr = base + t;
+          int k = (int)(rr % g.nz);
+          int j = (int)((rr / g.nz) % g.ny);
+          int i = (int)(rr / ((int64_t)g.nz * g.ny));
+          if (dtn[gidx(g, i, j, k)] <= (R)2.0 * dtsub) {
+            dtsub = dtsub / (R)2.0;
+            h++;
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) *out_h = h;
+  if (threadIdx.x == 0) {
+    ts[0] = (double)dtsub;
+    ts[1] = (double)ntsub * (double)(1ll << (h < 62 ? h : 62));
+  }
+}
+__global__ void k_fill_f32(float* p, float v) { *p = v; }
+// dtsub given by the caller (swap_field_subcycles)
+__global__ void k_set_dtsub(double* ts, double dtsub, double ntf) {
+  ts[0] = dtsub;
+  ts[1] = ntf;
 }
 
 // ---------------- fused, LDS-tiled field subcycle ----------------
@@ -510,8 +528,10 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
                                                         const double* __restrict__ b12,
                                                         const R* __restrict__ np,
                                                         const R* __restrict__ up,
-                                                        const R* __restrict__ nu, double coef,
+                                                        const R* __restrict__ nu,
+                                                        const double* __restrict__ ts, double cmul,
                                                         int tz, double* __restrict__ out) {
+  const double coef = cmul * ts[0];  // 2 dtsub (predictor) or dtsub (corrector)
   constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
   static_assert(NT >= TX * TY, "one thread per output column at least");
   __shared__ double sb[2][3][PA];  // b1, planes s+1, s+2
@@ -694,10 +714,12 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
                                                         const double* __restrict__ b1,
                                                         const R* __restrict__ np,
                                                         const R* __restrict__ up,
-                                                        const R* __restrict__ nu, double coef,
+                                                        const R* __restrict__ nu,
+                                                        const double* __restrict__ ts, double cmul,
                                                         int tz, double* __restrict__ out,
                                                         R* __restrict__ Eout,
                                                         R* __restrict__ ajout) {
+  const double coef = cmul * ts[0];
   constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
   static_assert(NT >= TX * TY, "one thread per output column at least");
   __shared__ double sb[2][3][PA];  // bh = (b1p2 + b1)/2, planes s+2, s+3
@@ -1019,44 +1041,43 @@ int fl_finalize_up(swap_ctx* c) {
                   (R)c->qn, F<R>(c, SWAP_F_UP)));
   return post(c);
 }
-int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub) {
+// check_time_step (gutsf.f:1776-1822) on the device: per-node deltat and
+// its grid minimum (k_deltat), then dtsub and ntf into c->ts (k_tstep), with
+// maind.f:420-421's reset dtsub = dt/ntsub, ntf = ntsub first
+int fl_time_step_dev(swap_ctx* c) {
   FC f = fc_of(c);
-  double dtsub0 = c->cfg.dt / (double)c->cfg.ntsub;
-  float big = 3.0e38f;
-  HIPCHK(c, hipMemcpyAsync(c->dmin, &big, sizeof(float), hipMemcpyHostToDevice, c->stream));
+  const double dtsub0 = c->cfg.dt / (double)c->cfg.ntsub;
+  const float big = 3.0e38f;
+  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(1), 0, c->stream, c->dmin, big);
   const unsigned nb = (unsigned)std::min<int64_t>(DT_BLOCKS, (c->g.nn + 255) / 256);
   DISPATCH(c, hipLaunchKernelGGL((k_deltat<R>), dim3(nb), dim3(256), 0, c->stream, c->g, f,
                                  (const double*)c->b1, (const R*)F<R>(c, SWAP_F_NP),
                                  (R*)c->scratch, c->dmin));
-  HIPCHK(c, hipGetLastError());
-  float mn = 0;
-  HIPCHK(c, hipMemcpyAsync(&mn, c->dmin, sizeof(float), hipMemcpyDeviceToHost, c->stream));
-  // the step's sort trigger rides on this synchronisation
+  // every node passes unless some deltat <= 2 dtsub0 (compared in the
+  // context precision, as the per-node test is)
+  const double thr64 = 2.0 * dtsub0 * (1 + 1e-6);
+  const float thr32 = 2.0f * (float)dtsub0 * (1 + 1e-6f);
+  DISPATCH(c, hipLaunchKernelGGL((k_tstep<R>), dim3(1), dim3(1024), 0, c->stream, c->g,
+                                 (const R*)c->scratch, (const float*)c->dmin, dtsub0, thr64,
+                                 thr32, c->cfg.ntsub, c->ts));
+  return post(c);
+}
+// the same, handed back to the host (swap_check_time_step)
+int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub) {
+  if (int e = fl_time_step_dev(c)) return e;
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, c->ts, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(&c->box_over, c->dcnt + 4, sizeof(c->box_over),
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  int h = 0;
-  // every node passes unless some deltat <= 2 dtsub0 (compare in the
-  // context precision, as the per-node test does)
-  bool maybe = c->prec == 64 ? ((double)mn <= 2.0 * dtsub0 * (1 + 1e-6))
-                             : (mn <= 2.0f * (float)dtsub0 * (1 + 1e-6f));
-  if (maybe) {
-    int* dh = (int*)c->dred;
-    DISPATCH(c, hipLaunchKernelGGL((k_halving<R>), dim3(1), dim3(1024), 0, c->stream,
-                                   c->g, (const R*)c->scratch, (R)dtsub0, dh));
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(&h, dh, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-  }
-  double ds = dtsub0;
-  int n = c->cfg.ntsub;
-  for (int q = 0; q < h; q++) {
-    if (c->prec == 64) ds = ds / 2.0; else ds = (double)((float)ds / 2.0f);
-    n *= 2;
-  }
-  *ntf = n;
-  *dtsub = c->prec == 64 ? ds : (double)(float)ds;
+  *dtsub = h[0];
+  *ntf = (int)h[1];
   return 0;
+}
+// dtsub of swap_field_subcycles' caller onto the device
+int fl_set_dtsub(swap_ctx* c, double dtsub, int ntf) {
+  hipLaunchKernelGGL(k_set_dtsub, dim3(1), dim3(1), 0, c->stream, c->ts, dtsub, (double)ntf);
+  return post(c);
 }
 int fl_copy_b(swap_ctx* c, const double* src, double* dst) {
   HIPCHK(c, hipMemcpyAsync(dst, src, sizeof(double) * 3 * c->g.nn, hipMemcpyDeviceToDevice,
@@ -1102,7 +1123,7 @@ int z_chunks(swap_ctx* c, Kern kern, int nthreads, int nxy, int* cache) {
   return *cache;
 }
 
-int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last) {
+int fl_subcycle_fused(swap_ctx* c, bool last) {
   FC f = fc_of(c);
   const Geo& g = c->g;
   DISPATCH(c, constexpr int TX = FTile<R>::TX, TY = FTile<R>::TY;
@@ -1115,12 +1136,14 @@ int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last) {
                               dim3(NP), 0, c->stream, g, f,
                                  (const double*)c->b1, (const double*)c->b12,
                                  (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
-                                 (const R*)F<R>(c, SWAP_F_NU), 2.0 * dtsub, tzp, c->b1p2);
+                                 (const R*)F<R>(c, SWAP_F_NU), (const double*)c->ts, 2.0, tzp,
+                                 c->b1p2);
            hipLaunchKernelGGL((k_sub_correct<R, TX, TY, NC>), dim3(gx, gy, (g.ncz + tzc - 1) / tzc),
                               dim3(NC), 0, c->stream, g, f,
                               (const double*)c->b1p2, (const double*)c->b1,
                               (const R*)F<R>(c, SWAP_F_NP), (const R*)F<R>(c, SWAP_F_UP),
-                              (const R*)F<R>(c, SWAP_F_NU), dtsub, tzc, c->b12,
+                              (const R*)F<R>(c, SWAP_F_NU), (const double*)c->ts, 1.0, tzc,
+                              c->b12,
                               last ? F<R>(c, SWAP_F_E) : (R*)nullptr,
                               last ? F<R>(c, SWAP_F_AJ) : (R*)nullptr));
   return post(c);

@@ -238,7 +238,8 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
                                                   int64_t n_sorted, int64_t n_total,
                                                   const int* __restrict__ boxes, int* boxes_next,
                                                   const int* __restrict__ vb, int* vb_next,
-                                                  unsigned long long* dcnt) {
+                                                  unsigned long long* dcnt,
+                                                  const int* __restrict__ tboxes) {
   constexpr int M = NMom<MODE>::value;
   // the LDS gather tile pays off while the item's box is compact (right
   // after a sort); drifted boxes gather from HBM/L2
@@ -260,12 +261,21 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
   bool windowed = false;
   int tile = -1;
   const int w = blockIdx.x;
+  const int* b;
   if (w < nitems) {
     Item it = items[w];
     i0 = it.start;
     i1 = it.end;
     tile = it.tile;
-    const int* b = boxes + 6 * w;
+    b = boxes + 6 * w;
+  } else {  // the unsorted tail (ions appended since the sort), with its own boxes
+    int64_t t = w - nitems;
+    i0 = n_sorted + t * TAIL_ITEM;
+    if (i0 >= n_total) return;
+    i1 = i0 + TAIL_ITEM < n_total ? i0 + TAIL_ITEM : n_total;
+    b = tboxes + 6 * t;
+  }
+  {
     for (int a = 0; a < 3; a++) {
       bl[a] = b[a];
       bh[a] = b[3 + a];
@@ -276,11 +286,6 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
       bh[a] -= sh;
     }
     windowed = bh[0] >= bl[0] && bh[1] >= bl[1] && bh[2] >= bl[2];
-  } else {
-    int64_t t = w - nitems;
-    i0 = n_sorted + t * ITEM_MAX;
-    if (i0 >= n_total) return;
-    i1 = i0 + ITEM_MAX < n_total ? i0 + ITEM_MAX : n_total;
   }
   const bool boxed = windowed;
   int lo_m[3] = {0, 0, 0}, hi_m[3] = {0, 0, 0};
@@ -380,6 +385,7 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
     const int64_t pb = chunk + (int64_t)tid * KP;
     float wv[BUFW ? 9 : 1][BUFW ? 4 : 1];  // v1 xyz, v xyz, fraction xyz per particle
     uint32_t wc[BUFW ? 4 : 1];              // packed cell per particle
+    unsigned gone = 0;                      // bit k: particle k left earlier (kept as is)
     int cur = -1;   // window cell of the running sums acc
     int hkey = -1;  // window cell of hd: the lane's first cell, once the lane left it
     R acc[8][M], hd[8][M];
@@ -397,6 +403,12 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
       if (p >= i1) break;
       if (BUFW && p < i0) continue;  // before the item: another item's particle
       const uint32_t c = P.cell[p];
+      // an ion that left through an open x face in an earlier step waits,
+      // untouched, for the sort that drops it
+      if (OX && c == DEAD_CELL) {
+        gone |= 1u << k;
+        continue;
+      }
       const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
       R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
       const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
@@ -539,7 +551,7 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
       float* dst[9] = {(float*)P.v1x, (float*)P.v1y, (float*)P.v1z, (float*)P.vx,
                        (float*)P.vy,  (float*)P.vz,  (float*)P.fx,  (float*)P.fy,
                        (float*)P.fz};
-      if (pb >= i0 && pb + 4 <= i1) {  // pb is a multiple of 4: 16-byte aligned
+      if (pb >= i0 && pb + 4 <= i1 && !gone) {  // pb is a multiple of 4: 16-byte aligned
 #pragma unroll
         for (int a = 0; a < 9; a++)
           *(float4*)(dst[a] + pb) = make_float4(wv[a][0], wv[a][1], wv[a][2], wv[a][3]);
@@ -548,7 +560,7 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           const int64_t p = pb + k;
-          if (p < i0 || p >= i1) continue;
+          if (p < i0 || p >= i1 || ((gone >> k) & 1u)) continue;
 #pragma unroll
           for (int a = 0; a < 9; a++) dst[a][p] = wv[a][k];
           P.cell[p] = wc[k];
@@ -627,12 +639,73 @@ __global__ void k_pass_reset(int64_t nitems, int64_t nvb, int* boxes_next, int* 
   boxes_next[q] = (q % 6) < 3 ? 0x7fffffff : -0x7fffffff;
 }
 
+// Cell boxes of the unsorted tail's work items (ions appended since the last
+// sort: inflow entrants, cloud ions), TAIL_ITEM consecutive particles each:
+// the entrants of one step fill the inflow face column by column (some 22
+// columns of cells per item at 64 ions per cell and 2.9 cells of drift per
+// step) and the cloud's ions node by node, so a tail item is compact and its
+// deposits fit a window like a sorted item's, also after the thermal spread
+// of the few steps until the next sort.  On a
+// periodic axis the box is taken in whichever of two frames (the cells, or
+// the cells shifted by half the axis) spans less, so an item across the
+// periodic seam keeps a compact box (its low end may then be negative).
+// Ions that left are skipped; an item with no live ion gets the empty box
+// (no window).
+__global__ __launch_bounds__(256) void k_tail_boxes(int64_t n_sorted, int64_t n_total,
+                                                    const uint32_t* __restrict__ cell, Geo g,
+                                                    int* tboxes) {
+  __shared__ int red[12];  // lo, hi of the cells; lo, hi of the shifted cells
+  if (threadIdx.x < 12) red[threadIdx.x] = (threadIdx.x % 6) < 3 ? 0x7fffffff : -0x7fffffff;
+  __syncthreads();
+  const int nc[3] = {g.ncx, g.ncy, g.ncz};
+  const int64_t i0 = n_sorted + (int64_t)blockIdx.x * TAIL_ITEM;
+  const int64_t i1 = i0 + TAIL_ITEM < n_total ? i0 + TAIL_ITEM : n_total;
+  int lo[6], hi[6];
+  for (int a = 0; a < 6; a++) {
+    lo[a] = 0x7fffffff;
+    hi[a] = -0x7fffffff;
+  }
+  for (int64_t p = i0 + threadIdx.x; p < i1; p += 256) {
+    const uint32_t c = cell[p];
+    if (c == DEAD_CELL) continue;
+    const int v[3] = {cell_i(c), cell_j(c), cell_k(c)};
+    for (int a = 0; a < 3; a++) {
+      const int s = v[a] + nc[a] / 2 < nc[a] ? v[a] + nc[a] / 2 : v[a] + nc[a] / 2 - nc[a];
+      lo[a] = min(lo[a], v[a]);
+      hi[a] = max(hi[a], v[a]);
+      lo[3 + a] = min(lo[3 + a], s);
+      hi[3 + a] = max(hi[3 + a], s);
+    }
+  }
+  for (int a = 0; a < 6; a++) {
+    const int m0 = wave_min(lo[a]), m1 = wave_max(hi[a]);
+    if (__lane_id() == 0) {
+      atomicMin(&red[(a / 3) * 6 + a % 3], m0);
+      atomicMax(&red[(a / 3) * 6 + 3 + a % 3], m1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    int l = red[a], h = red[3 + a];
+    const int ls = red[6 + a], hs = red[9 + a];
+    const bool periodic = !(a == 0 && g.ox);
+    if (periodic && l <= h && hs - ls < h - l) {  // the shifted frame is tighter
+      l = ls - nc[a] / 2;
+      h = hs - nc[a] / 2;
+    }
+    tboxes[6 * (int64_t)blockIdx.x + a] = l;
+    tboxes[6 * (int64_t)blockIdx.x + 3 + a] = h;
+  }
+}
+
 int post(swap_ctx* c) {
   HIPCHK(c, hipGetLastError());
   return 0;
 }
 
 }  // namespace
+
 
 // reset the box/velocity-bound outputs of P3
 int pl_box_reset(swap_ctx* c) {
@@ -644,10 +717,17 @@ int pl_box_reset(swap_ctx* c) {
 
 int pl_pass(swap_ctx* c, int mode) {
   if (c->n == 0) return 0;
-  int64_t ns = c->n_sorted < 0 ? 0 : c->n_sorted;
-  int64_t tail = c->n - ns;
-  int64_t nb = (c->n_sorted < 0 ? 0 : c->max_items) + (tail + ITEM_MAX - 1) / ITEM_MAX;
+  const int64_t ns = c->n_sorted < 0 ? 0 : c->n_sorted;
+  const int64_t tail = c->n - ns, nt = (tail + TAIL_ITEM - 1) / TAIL_ITEM;
+  const int64_t nb = (c->n_sorted < 0 ? 0 : c->max_items) + nt;
   if (nb == 0) return 0;
+  // the tail's boxes from the cells as they are now (a read of 4 bytes per
+  // tail ion); with nothing sorted, every particle is in the tail
+  if (nt > 0) {
+    hipLaunchKernelGGL(k_tail_boxes, dim3((unsigned)nt), dim3(256), 0, c->stream, ns, c->n, c->cell,
+                       c->g, c->tboxes);
+    if (int e = post(c)) return e;
+  }
   dim3 grid((unsigned)nb), blk(PASS_WG);
   const int* dn = c->n_sorted < 0 ? c->d_zero : c->d_nitems;
   auto launch = [&](auto R0, auto OX0) {
@@ -662,27 +742,27 @@ int pl_pass(swap_ctx* c, int mode) {
     if (mode == 1 && runs)
       hipLaunchKernelGGL((k_pass<R, 1, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt);
+                         c->dcnt, c->tboxes);
     else if (mode == 1)
       hipLaunchKernelGGL((k_pass<R, 1, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt);
+                         c->dcnt, c->tboxes);
     else if (mode == 2 && runs)
       hipLaunchKernelGGL((k_pass<R, 2, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S,
                          acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
-                         c->vb_next, c->dcnt);
+                         c->vb_next, c->dcnt, c->tboxes);
     else if (mode == 2)
       hipLaunchKernelGGL((k_pass<R, 2, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S,
                          acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
-                         c->vb_next, c->dcnt);
+                         c->vb_next, c->dcnt, c->tboxes);
     else if (runs)
       hipLaunchKernelGGL((k_pass<R, 3, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt);
+                         c->dcnt, c->tboxes);
     else
       hipLaunchKernelGGL((k_pass<R, 3, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt);
+                         c->dcnt, c->tboxes);
   };
   using Open = std::true_type;
   using Periodic = std::false_type;

@@ -484,6 +484,7 @@ int so_sort(swap_ctx* c) {
       HIPCHK(c, hipMemcpyAsync(&live, c->sstart + nk, 4, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
       c->escaped += n - (int64_t)live;
+      c->dead_pending = false;
       // the ions that left (now behind the live ones) leave the input account
       if (int e = sr_input_add(c, live, n - (int64_t)live, -1.0)) return e;
       c->n = n = live;

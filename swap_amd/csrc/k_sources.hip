@@ -490,6 +490,9 @@ int sr_ionize_sw(swap_ctx* c, int step, double ndot, double vsw, double beta_pu,
   }
   int64_t m = (int64_t)llround(dNi);
   if (m < 0) m = 0;
+  if (c->n + m > c->cap && c->dead_pending) {  // room held by ions that left: drop them
+    if (int e = so_sort(c)) return e;
+  }
   if (c->n + m > c->cap) return set_err(c, SWAP_ECAP, "ionize: capacity %lld exceeded",
                                         (long long)c->cap);
   if (m > 0 && !(c->cfg.flags & SWAP_SPECIES) && beta_pu != 1.0)
@@ -641,6 +644,7 @@ __global__ void k_chex(int64_t n, PDev<R> P, const uint32_t* id, double dx, doub
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
   uint32_t c = P.cell[p];
+  if (c == DEAD_CELL) return;  // left through an open x face, dropped by the next sort
   double x = ((double)cell_i(c) + 1.0 + (double)P.fx[p]) * dx;
   double y = ((double)cell_j(c) + 1.0 + (double)P.fy[p]) * dy;
   double z = ((double)cell_k(c) + 1.0 + (double)P.fz[p]) * dz;
@@ -708,6 +712,9 @@ int sr_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cl, int64_t* inject
   uint32_t tot = 0;
   HIPCHK(c, hipMemcpyAsync(&tot, c->coff + nin, 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->n + (int64_t)tot > c->cap && c->dead_pending) {  // room held by ions that left
+    if (int e = so_sort(c)) return e;
+  }
   if (c->n + (int64_t)tot > c->cap)
     return set_err(c, SWAP_ECAP, "ionize_cloud: %lld + %u ions exceed capacity %lld",
                    (long long)c->n, tot, (long long)c->cap);

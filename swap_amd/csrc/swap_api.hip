@@ -178,8 +178,8 @@ int load_input_energy(swap_ctx* c) {
   return sr_input_add(c, 0, c->n, 1.0);
 }
 
-int subcycle_fused(swap_ctx* c, double dtsub, bool last) {
-  TRY(fl_subcycle_fused(c, dtsub, last));
+int subcycle_fused(swap_ctx* c, bool last) {
+  TRY(fl_subcycle_fused(c, last));
   double* corrected = c->b12;
   c->b12 = c->b1;
   c->b1 = corrected;
@@ -188,6 +188,14 @@ int subcycle_fused(swap_ctx* c, double dtsub, bool last) {
 }
 
 }  // namespace
+
+// An open x keeps the ions that left as dead cells until the next sort;
+// what hands out particles, their count or particle diagnostics drops them
+// first (a sort), so callers see the live ions only.
+int compact(swap_ctx* c) {
+  if (!c->g.ox || !c->dead_pending) return 0;
+  return swap_sort(c);
+}
 
 extern "C" {
 
@@ -254,6 +262,12 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   c->scratch_bytes = std::max<size_t>(8 * nn, 1 << 16);
   DMF(c->scratch, c->scratch_bytes);
   DMF(c->dred, 64 * sizeof(double));
+  DMF(c->ts, 2 * sizeof(double));
+  if (hipHostMalloc((void**)&c->hstat, sizeof(*c->hstat), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_stat, hipEventDisableTiming) != hipSuccess) {
+    fprintf(stderr, "swap_create: pinned step status\n");
+    return fail(SWAP_ENOMEM);
+  }
   DMF(c->dmin, 16 * sizeof(float));
   DMF(c->dcnt, 8 * sizeof(unsigned long long));
   DMF(c->d_nitems, 4 * sizeof(int));
@@ -303,6 +317,7 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
   DMF(c->items, sizeof(Item) * (size_t)c->max_items);
   DMF(c->boxes, 24 * (size_t)c->max_items);
   DMF(c->boxes_next, 24 * (size_t)c->max_items);
+  DMF(c->tboxes, 24 * (size_t)(c->cap / TAIL_ITEM + 2));
 #undef DMF
   // the fixed staging buffer of swap_get_particle_array_noalloc on this device
   if ((e = na_setup(c))) {
@@ -327,8 +342,11 @@ int swap_destroy(swap_ctx* c) {
                 c->sruns, c->srbase, c->segkey, c->seginfo, c->segbase, c->nsegc, c->scnt,
                 c->sstart, c->scur, c->snrun, c->srstart, c->sblk, c->items, c->ccnt,
                 c->coff, c->cbp, c->icnt, c->ioff, c->iblk,
-                c->boxes, c->boxes_next, c->vb_mem, c->d_zero};
+                c->boxes, c->boxes_next, c->tboxes, c->vb_mem, c->d_zero};
   for (void* p : ps) dfree(p);
+  dfree(c->ts);
+  if (c->hstat) (void)hipHostFree(c->hstat);
+  if (c->ev_stat) (void)hipEventDestroy(c->ev_stat);
   for (int f = 0; f < SWAP_F_COUNT; f++) dfree(c->fld[f]);
   for (int a = 0; a < 9; a++) {
     dfree(c->p[a]);
@@ -353,7 +371,12 @@ int swap_set_seed(swap_ctx* c, uint64_t seed) {
   return 0;
 }
 
-int64_t swap_num_particles(const swap_ctx* c) { return c ? c->n : -1; }
+int64_t swap_num_particles(const swap_ctx* cc) {
+  if (!cc) return -1;
+  swap_ctx* c = const_cast<swap_ctx*>(cc);  // dropping the ions that left changes no state a caller sees
+  if (compact(c)) return -1;
+  return c->n;
+}
 
 int swap_get_coords(swap_ctx* c, int axis, double* q) {
   if (!c || !q || axis < 0 || axis > 2) return SWAP_EINVAL;
@@ -478,6 +501,7 @@ int swap_load_maxwellian(swap_ctx* c, int64_t n, double vsw, double vth, double 
 
 int swap_get_particle_array(swap_ctx* c, int which, void* host) {
   if (!c || !host) return SWAP_EINVAL;
+  TRY(compact(c));
   size_t n = (size_t)c->n, rn = c->rsz * n;
   switch (which) {
     case SWAP_P_XP: {
@@ -542,6 +566,7 @@ int swap_get_particle_array(swap_ctx* c, int which, void* host) {
 
 int swap_set_particle_array(swap_ctx* c, int which, const void* host) {
   if (!c || !host) return SWAP_EINVAL;
+  TRY(compact(c));
   c->mgen++;
   size_t n = (size_t)c->n, rn = c->rsz * n;
   if (!n) return 0;
@@ -672,11 +697,13 @@ int swap_periodic(swap_ctx* c, int f) {
 
 int swap_get_interp_weights(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
+  TRY(compact(c));
   return pl_weights(c);
 }
 
 int swap_update_np(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
+  TRY(compact(c));
   TRY(zero_moments(c, 0, 1));
   TRY(pl_deposit_simple(c, 0));
   return fl_finalize_np(c);
@@ -684,6 +711,7 @@ int swap_update_np(swap_ctx* c) {
 
 int swap_update_up(swap_ctx* c, int vsrc) {
   if (!c || vsrc < 0 || vsrc > 2) return SWAP_EINVAL;
+  TRY(compact(c));
   TRY(zero_moments(c, 1, 4));
   TRY(pl_deposit_simple(c, vsrc == SWAP_V_CUR ? 1 : (vsrc == SWAP_V_EXTRAP ? 2 : 3)));
   return fl_finalize_up(c);
@@ -693,6 +721,7 @@ int swap_extrapol_up(swap_ctx* c) { return swap_update_up(c, SWAP_V_EXTRAP); }
 
 int swap_get_Ep(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
+  TRY(compact(c));
   TRY(fl_build_src(c, c->fld[SWAP_F_AJ], c->fld[SWAP_F_UP], c->fld[SWAP_F_BTC],
                    c->fld[SWAP_F_AJC], c->S));
   return pl_get_Ep(c);
@@ -700,6 +729,7 @@ int swap_get_Ep(swap_ctx* c) {
 
 int swap_get_vplus_vminus(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
+  TRY(compact(c));
   if (!c->Ep) return set_err(c, SWAP_ESTATE, "get_vplus_vminus before get_Ep");
   return pl_vplus_vminus(c);
 }
@@ -713,6 +743,7 @@ int swap_improve_up(swap_ctx* c) {
 
 int swap_get_vp_final(swap_ctx* c) {
   if (!c) return SWAP_EINVAL;
+  TRY(compact(c));
   if (!c->vpl) return set_err(c, SWAP_ESTATE, "get_vp_final before get_vplus_vminus");
   c->mgen++;
   return pl_vp_final(c);
@@ -820,8 +851,9 @@ int swap_face_to_center(swap_ctx* c, int in, int out) {
 int swap_field_subcycles(swap_ctx* c, int ntf, double dtsub) {
   if (!c || ntf < 0 || !(dtsub > 0)) return SWAP_EINVAL;
   const bool fused = !(c->cfg.flags & SWAP_UNFUSED_FIELDS);
+  if (fused) TRY(fl_set_dtsub(c, dtsub, ntf));
   for (int q = 0; q < ntf; q++)
-    TRY(fused ? subcycle_fused(c, dtsub, q == ntf - 1) : subcycle_impl(c, dtsub));
+    TRY(fused ? subcycle_fused(c, q == ntf - 1) : subcycle_impl(c, dtsub));
   return 0;
 }
 
@@ -849,7 +881,8 @@ int swap_step(swap_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     // sort on schedule, or when particles were loaded (nothing sorted), or
     // when injected ions appended past the sorted range exceed 1/16 of all
-    // (the passes take such a tail without windows until the next sort), or
+    // (the passes take such a tail in small work items of consecutive ions,
+    // whose windows the thermal spread of a few steps outgrows), or
     // when the last P3 found an item whose box outgrew its P1 window (its
     // particles would all deposit straight to HBM)
     bool due = c->cfg.sort_every > 0 &&
@@ -894,6 +927,7 @@ int swap_step(swap_ctx* c, int nsteps) {
     {
       Stage t(c, SWAP_K_P3);
       TRY(pl_pass(c, 3));
+      if (c->g.ox) c->dead_pending = true;  // P3 marks the ions that left
       TRY(sr_inflow_step(c));  // an open x: the ions entering during this step
     }
     {
@@ -910,17 +944,33 @@ int swap_step(swap_ctx* c, int nsteps) {
       c->steps_since_sort++;
     }
     // field subcycles (maind.f:420-476)
-    int ntf;
-    double dtsub;
-    TRY(swap_check_time_step(c, &ntf, &dtsub));
-    {
+    if (c->cfg.flags & SWAP_UNFUSED_FIELDS) {  // per-routine kernels with host dtsub
+      int ntf;
+      double dtsub;
+      TRY(swap_check_time_step(c, &ntf, &dtsub));
       Stage t(c, SWAP_K_FIELD);
       TRY(swap_field_subcycles(c, ntf, dtsub));
-    }
-    // an open x: the sort drops the ions that P3 marked as gone
-    if (c->g.ox) {
-      Stage t(c, SWAP_K_SORT);
-      TRY(swap_sort(c));
+    } else {
+      // check_time_step on the device: the subcycle kernels read dtsub there,
+      // so the usual ntsub subcycles are enqueued at once; ntf (and the sort
+      // trigger of the next step) come back through pinned memory, long
+      // before those subcycles finish, and a step that halved dtsub enqueues
+      // the rest
+      TRY(fl_time_step_dev(c));
+      HIPCHK(c, hipMemcpyAsync(&c->hstat->dtsub, c->ts, 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(&c->hstat->box_over, c->dcnt + 4, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipEventRecord(c->ev_stat, c->stream));
+      Stage t(c, SWAP_K_FIELD);
+      const int n0 = c->cfg.ntsub;
+      for (int q = 0; q < n0; q++) TRY(subcycle_fused(c, q == n0 - 1));
+      HIPCHK(c, hipEventSynchronize(c->ev_stat));
+      const int ntf = (int)c->hstat->ntf;
+      c->box_over = c->hstat->box_over;
+      for (int q = n0; q < ntf; q++) TRY(subcycle_fused(c, q == ntf - 1));
+      c->ntf = ntf;
+      c->dtsub = c->hstat->dtsub;
     }
     c->step_count++;
     c->step_gen = c->mgen;  // macc: the final deposit of this step
@@ -949,11 +999,13 @@ int swap_ionize_cloud(swap_ctx* c, int step, const swap_cloud* cloud, int64_t* i
 
 int swap_diagnostics(swap_ctx* c, swap_diag* d) {
   if (!c || !d) return SWAP_EINVAL;
+  TRY(compact(c));
   return sr_diagnostics(c, d);
 }
 
 int swap_energy_diag(swap_ctx* c, double mion, const double b0[3], swap_energy* d) {
   if (!c || !b0 || !d || !(mion > 0)) return SWAP_EINVAL;
+  TRY(compact(c));
   return sr_energy_diag(c, mion, b0, d);
 }
 
@@ -993,6 +1045,7 @@ int swap_run_state_size(swap_ctx* c, int64_t* bytes) {
 
 int swap_save_run_state(swap_ctx* c, void* host) {
   if (!c || !host) return SWAP_EINVAL;
+  TRY(compact(c));
   RunStateHead h;
   memset(&h, 0, sizeof h);
   memcpy(h.magic, RS_MAGIC, 8);

@@ -10,6 +10,7 @@
 constexpr int TILE_X = 4, TILE_Y = 8, TILE_Z = 8;
 constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
 constexpr int ITEM_MAX = 32768;
+constexpr int TAIL_ITEM = 4096;  // particles per work item of the unsorted tail
 constexpr int PASS_WG = 512;   // threads per particle-pass workgroup
 constexpr int PASS_KP = 4;     // consecutive particles per lane
 constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values (64 KB)
@@ -81,6 +82,7 @@ struct swap_ctx {
   Item* items = nullptr;
   int* boxes = nullptr;       // [max_items][6]: cell box lo[3], hi[3]
   int* boxes_next = nullptr;
+  int* tboxes = nullptr;      // [tail items][6]: boxes of the unsorted tail's work items
   int* d_nitems = nullptr;
   int64_t max_items = 0;
   int64_t n_sorted = -1;      // particles covered by the item list (-1: none)
@@ -110,11 +112,22 @@ struct swap_ctx {
   int64_t islots = 0;
   int fz_pred = 0, fz_corr = 0;  // z chunks of the fused field launches (0: not yet chosen)
   double* dred = nullptr;     // FP64 reduction scratch
+  double* ts = nullptr;       // device dtsub and ntf of the current step (check_time_step)
+  // pinned host copies of ts (dtsub, ntf) and of the box-overflow count, and
+  // the event after which they hold the current step's values
+  struct StepStat {
+    double dtsub, ntf;
+    unsigned long long box_over;
+  }* hstat = nullptr;
+  hipEvent_t ev_stat = nullptr;
   float* dmin = nullptr;      // check_time_step minimum
   void* scratch = nullptr;    // general device scratch
   size_t scratch_bytes = 0;
   int64_t step_count = 0;
   int64_t sorts = 0;          // counting sorts done
+  // an open x: ions may have left since the last sort (cell DEAD_CELL, still
+  // in the arrays until a sort drops them)
+  bool dead_pending = false;
   // learned moment denoiser (k_denoise.hip); null until swap_dn_create
   struct DnState* dn = nullptr;
   // stage timing (swap_profile): event pairs recorded on `stream`
@@ -154,8 +167,11 @@ int fl_update_src(swap_ctx* c, const void* up, const void* ajc, void* S);
 int fl_finalize_np(swap_ctx* c);
 int fl_finalize_up(swap_ctx* c);
 int fl_check_time_step(swap_ctx* c, int* ntf, double* dtsub);
+// check_time_step into c->ts on the device (no host synchronisation)
+int fl_time_step_dev(swap_ctx* c);
+int fl_set_dtsub(swap_ctx* c, double dtsub, int ntf);
 int fl_copy_b(swap_ctx* c, const double* src, double* dst);
-int fl_subcycle_fused(swap_ctx* c, double dtsub, bool last);
+int fl_subcycle_fused(swap_ctx* c, bool last);  // dtsub from c->ts
 int fl_zero(swap_ctx* c, void* p, size_t bytes);
 
 // ---- particle launchers (k_particles.hip) ----
@@ -179,6 +195,8 @@ int pl_box_reset(swap_ctx* c);
 
 // ---- no-alloc staging (swap_noalloc.hip): device address of its buffer ----
 int na_setup(swap_ctx* c);
+// an open x: drop the ions that left since the last sort (swap_api.hip)
+int compact(swap_ctx* c);
 
 // ---- sort (k_sort.hip) ----
 int so_sort(swap_ctx* c);

@@ -42,6 +42,7 @@ extern "C" int swap_get_particle_array_noalloc(swap_ctx* c, int which, void* hos
   if (!c || !host) return SWAP_EINVAL;
   // every other array is a plain device-to-host copy with no staging
   if (which != SWAP_P_XP) return swap_get_particle_array(c, which, host);
+  if (int e = compact(c)) return e;  // an open x: drop the ions that left (a sort, no allocation)
   if (c->n > SWAP_NOALLOC_MAX)
     return set_err(c, SWAP_EINVAL,
                    "get_particle_array_noalloc: %lld particles, the fixed staging buffer holds "
