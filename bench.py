@@ -7,8 +7,12 @@ particle state (split-precision positions), FP64 B advance.  A "step" is one
 full swap_step: the three particle passes, the counting sort when due, and
 all field subcycles.  Particles are generated on the device before timing.
 
-Prints one JSON line with the roofline of the dominant kernel and a CPU
-baseline (the FP64 oracle on a bounded sample).
+Prints one JSON line with the roofline of the dominant kernel, a CPU
+baseline (the oracle with OpenMP on a bounded sample, FP32 and FP64) and the
+metric's second half, "equal_error": the learned denoiser trained on 2000
+pairs of a 64^3 pickup-cloud case, and the denoised 8-ppc FP32 step timed
+against the plain FP64 (and FP32) step at the ppc of equal moment error
+(swap_amd/denoise_study.py equal_error; about a minute).
 
 Other workloads: --workload pickup (a neutral cloud injecting 28-amu ions
 before every step), --precision 64 (the same kernels in FP64), --cells/--ppc
@@ -125,6 +129,10 @@ def main():
                          "(photoionization, charge exchange) before every step; outflow: "
                          "inflow/outflow x with the upstream plasma flowing in at x = 0")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-equal-error", action="store_true",
+                    help="skip the equal-error block (the metric's second half: the "
+                         "denoised low-ppc FP32 step against the plain FP64 step at the ppc "
+                         "of equal moment error, on a 64^3 pickup-cloud case; about a minute)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -234,9 +242,15 @@ def main():
     if pickup:
         out["config"]["cloud"] = cloud
         out["config"]["injected"] = diag["injected"]
+    s.close()
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()
+    if not args.no_equal_error and args.workload == "uniform" and args.precision == 32:
+        from swap_amd import denoise_study
+        print("bench: equal-error block (2000 pairs, training, timed runs)", file=sys.stderr,
+              flush=True)
+        out["equal_error"] = denoise_study.equal_error()
     print(json.dumps(out))
 
 

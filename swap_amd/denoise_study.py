@@ -56,6 +56,67 @@ def step_time(s, steps, cloud, step0):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def gen_pairs(s, p, cl, pairs, segments, spinup, pair_every=1, seed0=20261203):
+    """pairs from the high-ppc FP32 run s: `segments` seeded realisations,
+    each a fresh load with B = B0, spun up for `spinup` steps (the cloud
+    builds its structure), then a pair every `pair_every` steps; returns
+    (particles at the end, generation s, pair-deposit s)"""
+    t0 = time.perf_counter()
+    per_seg = -(-pairs // segments)
+    b0 = np.zeros((3, s.nz, s.ny, s.nx))
+    b0[2] = p["b0z"]
+    tp, j = 0.0, 0
+    for seg in range(segments):
+        s.set_seed(seed0 + 7919 * seg)
+        s.load_maxwellian(p["npart"], p["vsw"], p["vth"])
+        s.set_field("b1", b0)
+        s.set_field("b12", b0)
+        s.prime()
+        k = 0
+        for k in range(spinup):
+            s.ionize_cloud(k, cl)
+            s.step(1)
+        for _ in range(per_seg):
+            if j == pairs:
+                break
+            for _ in range(pair_every):
+                k += 1
+                s.ionize_cloud(k, cl)
+                s.step(1)
+            s.sync()
+            t1 = time.perf_counter()
+            s.dn_add_pair(api.DN_DISJOINT_STEP)
+            tp += time.perf_counter() - t1
+            j += 1
+        _log("segment %d: %d pairs, %.1f s, %d particles, %d field subcycles" %
+             (seg, j, time.perf_counter() - t0, s.n, s.check_time_step()[0]))
+    return s.n, time.perf_counter() - t0, tp
+
+
+def train(s, iters, lr, ntr):
+    """Adam on the first ntr pairs: lr for the first half, 0.3 lr to 80 %,
+    0.1 lr after; returns (last loss, seconds)"""
+    t0 = time.perf_counter()
+    done, loss = 0, float("nan")
+    while done < iters:
+        n = min(250, iters - done)
+        f = 1.0 if done < 0.5 * iters else 0.3 if done < 0.8 * iters else 0.1
+        loss = s.dn_train(n, lr * f, ntr)
+        done += n
+        _log("iter %d lr %.2g loss %.4g" % (done, lr * f, loss))
+    s.sync()
+    return loss, time.perf_counter() - t0
+
+
+def decompose(raw, net, S, ppc):
+    """errors about the truth from the held-out disjoint-pair errors (module
+    docstring): plain high-ppc s2, denoised low-ppc e_net, and the ppc at
+    which a plain run's error equals e_net"""
+    s2 = raw * (S - 1) / S ** 2
+    e_net = net - raw / S
+    return s2, e_net, (ppc * s2 / e_net if e_net > 0 else float("inf"))
+
+
 def timed_run(nc, ppc, cl, precision, steps, warm, params=None, pairs=None, nscale=None,
               vscale=None):
     """ms/step of a fresh run at ppc; with params, the denoiser runs in the
@@ -78,6 +139,74 @@ def timed_run(nc, ppc, cl, precision, steps, warm, params=None, pairs=None, nsca
     ms = step_time(s, steps, cl, warm)
     s.close()
     return ms, gate
+
+
+def equal_error(cells=(64, 64, 64), ppc=128, stride=16, pairs=2000, segments=20, spinup=20,
+                iters=4000, lr=5e-4, n0_rate=1.0, time_steps=5, ppc_cap=1024):
+    """The equal-error comparison of the headline metric on a structured
+    case, small enough for a few minutes: a pickup-ion cloud (N0 rate
+    n0_rate, r0 = 3 cells, H = 8 cells) in a drifting plasma at `cells`.
+    Pairs come from FP32 runs at `ppc`, `segments` seeded realisations of
+    pairs / segments pairs each; the last realisation's pairs are held out
+    (their errors set the gate and every figure below), so training and
+    evaluation never share a realisation.  The denoised FP32 run at
+    ppc / stride is timed against the plain FP64 run (and the plain FP32
+    run) at ppc_eq, the ppc whose moment error equals the denoised one
+    (capped at ppc_cap: the ratio is then a lower bound).  For scale, the
+    error of a predictor that knew each snapshot's mean but no structure is
+    the truth's variance about its mean (signal_var), reported with its own
+    equivalent ppc.  Short realisations (spinup + pairs / segments steps)
+    keep the periodic domain from filling with pickup ions.  Returns a
+    dict."""
+    nc = tuple(cells)
+    S = stride
+    p = physics.workload(*nc, ppc)
+    cl = physics.cloud(p, N0_rate=n0_rate, r0_cells=3.0, H_cells=8.0)
+    per_seg = -(-pairs // segments)
+    s = physics.make_swap(p, capacity=_capacity(p, spinup + per_seg),
+                          flags=api.TRACK_IDS | api.SPECIES)
+    s.dn_create(p["n0"], p["vsw"], max_pairs=pairs, stride=S, seed=5)
+    n_high, gen_s, _ = gen_pairs(s, p, cl, pairs, segments, spinup)
+    ntr = pairs - per_seg
+    loss, train_s = train(s, iters, lr, ntr)
+    raw, net = s.dn_eval(ntr, pairs)
+    held = [s.dn_get_pair(i) for i in range(ntr, pairs)]
+    tvar, tmax = 0.0, 0.0
+    for _, t in held[:20]:
+        tvar += float(np.mean(t.var(axis=(1, 2, 3)))) / min(len(held), 20)
+        tmax = max(tmax, float(np.abs(t).max()))
+    params = s.dn_params()
+    s.close()
+    s2, e_net, ppc_eq = decompose(raw, net, S, ppc)
+    out = {
+        "case": "pickup-ion cloud (N0 rate %g, r0 3 cells, H 8 cells) in a drifting plasma, "
+                "%dx%dx%d cells" % ((n0_rate,) + nc),
+        "cells": list(nc), "ppc_high": ppc, "ppc_low": ppc // S,
+        "pairs_train": ntr, "pairs_test": pairs - ntr, "segments": segments,
+        "spinup_steps": spinup, "iters": iters, "pair_gen_s": round(gen_s, 2),
+        "train_s": round(train_s, 2), "last_loss": loss,
+        "units": "mean squared error per value of (np/n0 - 1, up/vsw), about the truth",
+        "mse_plain_ppc_high": s2, "mse_plain_ppc_low": S * s2, "mse_denoised_ppc_low": e_net,
+        # variance of the truth about its mean, next to the plain high-ppc noise
+        "signal_var": tvar - S / (S - 1) * s2, "target_absmax": tmax,
+        "ppc_equivalent": ppc_eq,
+    }
+    sv = out["signal_var"]
+    out["ppc_equivalent_mean_predictor"] = ppc * s2 / sv if sv > 0 else float("inf")
+    nscale, vscale = p["n0"], p["vsw"]
+    ms_dn, gate = timed_run(nc, ppc // S, cl, api.FP32, time_steps, 2, params, held, nscale,
+                            vscale)
+    q = int(math.ceil(min(ppc_eq, ppc_cap))) if math.isfinite(ppc_eq) else ppc_cap
+    ms64, _ = timed_run(nc, q, cl, api.FP64, time_steps, 2)
+    ms32, _ = timed_run(nc, q, cl, api.FP32, time_steps, 2)
+    out.update({
+        "gate_open": bool(gate), "ppc_timed": q, "lower_bound": bool(not ppc_eq <= ppc_cap),
+        "ms_step_fp32_denoised_ppc_low": ms_dn, "ms_step_fp64_plain_ppc_eq": ms64,
+        "ms_step_fp32_plain_ppc_eq": ms32,
+        "speedup_vs_fp64_equal_error": ms64 / ms_dn,
+        "speedup_vs_fp32_equal_error": ms32 / ms_dn,
+    })
+    return out
 
 
 def main(argv=None):
@@ -107,57 +236,20 @@ def main(argv=None):
     p = physics.workload(*nc, args.ppc)
     cl = physics.cloud(p, N0_rate=args.n0_rate, r0_cells=3.0, H_cells=12.0)
 
-    # 1. pairs from the high-ppc FP32 run: --segments seeded realisations,
-    # each a fresh load with B = B0, spun up, then a pair every --pair-every
-    # steps (short segments keep the cloud from driving the plasma into the
-    # low-density, high-B state whose subcycling blows up the step time)
-    t0 = time.perf_counter()
+    # 1. pairs from the high-ppc FP32 run (short segments keep the cloud from
+    # driving the plasma into the low-density, high-B state whose subcycling
+    # blows up the step time)
     per_seg = -(-args.pairs // args.segments)
     s = physics.make_swap(p, capacity=_capacity(p, args.spinup + per_seg * args.pair_every),
                           flags=api.TRACK_IDS | api.SPECIES)
     s.dn_create(p["n0"], p["vsw"], max_pairs=args.pairs, stride=S, seed=5)
-    b0 = np.zeros((3, s.nz, s.ny, s.nx))
-    b0[2] = p["b0z"]
-    tp, j = 0.0, 0
-    for seg in range(args.segments):
-        s.set_seed(20261203 + 7919 * seg)
-        s.load_maxwellian(p["npart"], p["vsw"], p["vth"])
-        s.set_field("b1", b0)
-        s.set_field("b12", b0)
-        s.prime()
-        k = 0
-        for k in range(args.spinup):
-            s.ionize_cloud(k, cl)
-            s.step(1)
-        for _ in range(per_seg):
-            if j == args.pairs:
-                break
-            for _ in range(args.pair_every):
-                k += 1
-                s.ionize_cloud(k, cl)
-                s.step(1)
-            s.sync()
-            t1 = time.perf_counter()
-            s.dn_add_pair(api.DN_DISJOINT_STEP)
-            tp += time.perf_counter() - t1
-            j += 1
-        _log("segment %d: %d pairs, %.1f s" % (seg, j, time.perf_counter() - t0))
-    n_high = s.n
-    gen_s = time.perf_counter() - t0
+    n_high, gen_s, tp = gen_pairs(s, p, cl, args.pairs, args.segments, args.spinup,
+                                  args.pair_every)
 
     # 2. training and held-out evaluation
     ntr = args.pairs - args.test_pairs
     raw0, net0 = s.dn_eval(ntr, args.pairs)
-    t0 = time.perf_counter()
-    done, loss = 0, float("nan")
-    while done < args.iters:
-        n = min(250, args.iters - done)
-        lr = args.lr * (1.0 if done < 0.5 * args.iters else 0.3 if done < 0.8 * args.iters else 0.1)
-        loss = s.dn_train(n, lr, ntr)
-        done += n
-        _log("iter %d lr %.2g loss %.4g" % (done, lr, loss))
-    s.sync()
-    train_s = time.perf_counter() - t0
+    loss, train_s = train(s, args.iters, args.lr, ntr)
     raw, net = s.dn_eval(ntr, args.pairs)
     held = [s.dn_get_pair(i) for i in range(ntr, args.pairs)]
     # where the structure is: the same decomposition restricted to the
@@ -177,10 +269,8 @@ def main(argv=None):
         acc["tvar_in"] += float(np.mean(t[:, inside].var(axis=1))) / nh
     params = s.dn_params()
     s.close()
-    s2 = raw * (S - 1) / S ** 2
-    e_net = net - raw / S
+    s2, e_net, ppc_eq = decompose(raw, net, S, args.ppc)
     e_raw_low = S * s2
-    ppc_eq = args.ppc * s2 / e_net if e_net > 0 else float("inf")
     out = {
         "study": "learned moment denoiser",
         "cells": list(nc), "ppc_high": args.ppc, "ppc_low": args.ppc // S,
