@@ -16,8 +16,10 @@
 // tap-flipped, transposed weights (the adjoint of a periodic convolution),
 // scaled by the leaky ReLU's slope of the layer below (1 or 1/16, by the sign
 // of its stored activation); weight gradients are a second GEMM
-// (M = K entries, N = output channels, reduction over voxels) accumulated
-// into FP32 with atomics.  FP32 master weights, Adam.
+// (M = K entries, N = output channels, reduction over voxels, LDS-staged
+// voxel blocks) accumulated into FP32 with atomics.  FP32 master weights,
+// Adam.
+#include <algorithm>
 #include <utility>
 
 #include "swap_internal.h"
@@ -52,13 +54,6 @@ struct G3 {
   int nx, ny, nz;
   int64_t V;
 };
-__device__ __forceinline__ int64_t nbr(const G3& g, int x, int y, int z, int t) {
-  int a = x + t % 3 - 1, b = y + (t / 3) % 3 - 1, c = z + t / 9 - 1;
-  a = a < 0 ? a + g.nx : (a >= g.nx ? a - g.nx : a);
-  b = b < 0 ? b + g.ny : (b >= g.ny ? b - g.ny : b);
-  c = c < 0 ? c + g.nz : (c >= g.nz ? c - g.nz : c);
-  return (int64_t)a + (int64_t)g.nx * ((int64_t)b + (int64_t)g.ny * c);
-}
 
 // Y = epilogue(conv(X, W)); Wt is [r16(COUT)][k16(CIN)], zero padded.
 // A workgroup computes a CTX x CTY x CTZ block of output voxels.  Its input
@@ -154,42 +149,102 @@ __global__ __launch_bounds__(256) void k_conv_t(G3 g, const bf16* __restrict__ X
   }
 }
 
-// dW[k][co] += sum over a chunk of voxels of X[nbr(v, tap(k))][ci(k)] dY[v][co]
+// dW[k][co] += sum over voxels v of X[v + offset(tap(k))][ci(k)] dY[v][co], with
+// k = tap CIN + ci: an implicit GEMM on MFMA with M = the 27 CIN weight rows
+// (tiles of 16 rows: one tap's 16 channels, or 4 taps x 4 channels), N =
+// output channels, reduced over voxels in 16-voxel x-runs.  A workgroup
+// takes CTX x CTY x CTZ voxel blocks in turn (grid-stride), stages each
+// block's input halo (as k_conv_t) and its output gradients in LDS
+// (channels-last, 32 channels padded to 36 so the 16 voxels of a lane group
+// fall in distinct banks), and keeps its partial dW in registers across the
+// blocks: wavefront w owns row tiles w, w + 4, ...; lane l supplies weight
+// row l % 16 (A) and output channel l % 16 (B) at the run's voxels
+// 4(l/16) .. 4(l/16)+3, and holds rows 4(l/16)+e of channel l % 16.  Each
+// workgroup adds its partial to dW once, with FP32 atomics.
+constexpr int WG_BLOCKS = 512;  // workgroups of a weight-gradient launch
 template <int CIN, int COUT>
-__global__ __launch_bounds__(64) void k_wgrad(G3 g, const bf16* __restrict__ X,
-                                             const bf16* __restrict__ dY, float* dW,
-                                             int64_t chunk) {
-  constexpr int K = 27 * CIN;
-  const int lane = threadIdx.x;
-  const int m = lane & 15, kq = (lane >> 4) * 4;
-  const int k = blockIdx.y * 16 + m, co = blockIdx.z * 16 + m;
-  const bool kok = k < K, cok = co < COUT;
-  const int t = kok ? k / CIN : 0, ci = kok ? k % CIN : 0;
-  const int64_t vb0 = (int64_t)blockIdx.x * chunk;
-  const int64_t vb1 = vb0 + chunk < g.V ? vb0 + chunk : g.V;
-  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
-  for (int64_t vb = vb0; vb < vb1; vb += 16) {
-    s4 a, b;
+__global__ __launch_bounds__(256) void k_wgrad(G3 g, const bf16* __restrict__ X,
+                                              const bf16* __restrict__ dY, float* dW) {
+  static_assert(CIN == 4 || CIN == 32, "input channels");
+  static_assert(COUT == 4 || COUT == 32, "output channels");
+  constexpr int VS = CIN == 32 ? 36 : CIN, YS = COUT == 32 ? 36 : COUT;
+  constexpr int K = 27 * CIN, MT = (K + 15) / 16, NT = r16(COUT) / 16, CPV = CIN / 4;
+  constexpr int MTW = (MT + 3) / 4, BV = CTX * CTY * CTZ, YPV = COUT / 4;
+  __shared__ __align__(16) bf16 sx[CHV * VS];
+  __shared__ __align__(16) bf16 sy[BV * YS];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m = lane & 15, gq = lane >> 4;
+  const int nbx = (g.nx + CTX - 1) / CTX, nby = (g.ny + CTY - 1) / CTY,
+            nbz = (g.nz + CTZ - 1) / CTZ;
+  const int nb = nbx * nby * nbz;
+  f4 acc[MTW][NT];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int64_t v = vb + kq + e;
-      const bool ok = v < vb1;
-      short av = 0, bv = 0;
-      if (ok && kok) {
-        const int x = (int)(v % g.nx), y = (int)((v / g.nx) % g.ny),
-                  z = (int)(v / ((int64_t)g.nx * g.ny));
-        av = (short)X[nbr(g, x, y, z, t) * CIN + ci];
-      }
-      if (ok && cok) bv = (short)dY[v * COUT + co];
-      a[e] = av;
-      b[e] = bv;
+  for (int j = 0; j < MTW; j++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[j][nt] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    const int x0 = (blk % nbx) * CTX, y0 = ((blk / nbx) % nby) * CTY, z0 = blk / (nbx * nby) * CTZ;
+    __syncthreads();  // the previous block's LDS reads are done
+    for (int q = tid; q < CHV * CPV; q += 256) {
+      const int hv = q / CPV, ch = q % CPV;
+      const int hx = hv % CHX, hy = (hv / CHX) % CHY, hz = hv / (CHX * CHY);
+      const int64_t gv = wrapi(x0 - 1 + hx, g.nx) +
+                         (int64_t)g.nx * (wrapi(y0 - 1 + hy, g.ny) +
+                                          (int64_t)g.ny * wrapi(z0 - 1 + hz, g.nz));
+      *(s4*)(sx + hv * VS + 4 * ch) = *(const s4*)(X + gv * CIN + 4 * ch);
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    // output gradients of the block's voxels; voxels past the grid's end add nothing
+    for (int q = tid; q < BV * YPV; q += 256) {
+      const int lv = q / YPV, ch = q % YPV;
+      const int x = x0 + lv % CTX, y = y0 + (lv / CTX) % CTY, z = z0 + lv / (CTX * CTY);
+      s4 v = s4{0, 0, 0, 0};
+      if (x < g.nx && y < g.ny && z < g.nz)
+        v = *(const s4*)(dY + (x + (int64_t)g.nx * (y + (int64_t)g.ny * z)) * COUT + 4 * ch);
+      *(s4*)(sy + lv * YS + 4 * ch) = v;
+    }
+    __syncthreads();
+    for (int r = 0; r < CTY * CTZ; r++) {  // the block's 16-voxel x-runs
+      const int ly = r % CTY, lz = r / CTY;
+      s4 b[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        const int co = nt * 16 + m;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          b[nt][e] = co < COUT ? (short)sy[(4 * gq + e + CTX * (ly + CTY * lz)) * YS + co] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < MTW; j++) {
+        const int mt = w + 4 * j;
+        if (mt >= MT) break;
+        const int k = mt * 16 + m;
+        s4 a = s4{0, 0, 0, 0};
+        if (k < K) {
+          const int t = k / CIN, ci = k % CIN;
+          const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
+          const int hb = dx + CHX * ((ly + dy) + CHY * (lz + dz));
+#pragma unroll
+          for (int e = 0; e < 4; e++) a[e] = (short)sx[(hb + 4 * gq + e) * VS + ci];
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++)
+          acc[j][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[j][nt], 0, 0, 0);
+      }
+    }
   }
 #pragma unroll
-  for (int e = 0; e < 4; e++) {
-    const int kk = blockIdx.y * 16 + kq + e;
-    if (kk < K && cok) atomicAdd(dW + (int64_t)kk * COUT + co, acc[e]);
+  for (int j = 0; j < MTW; j++) {
+    const int mt = w + 4 * j;
+    if (mt >= MT) break;
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const int co = nt * 16 + m;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int k = mt * 16 + 4 * gq + e;
+        if (k < K && co < COUT) atomicAdd(dW + (int64_t)k * COUT + co, acc[j][nt][e]);
+      }
+    }
   }
 }
 
@@ -408,8 +463,8 @@ int dn_backward(swap_ctx* c, const float* tgt) {
   HIPCHK(c, hipMemsetAsync(d->dloss, 0, sizeof(double), c->stream));
   hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, 4 * V, d->out, tgt, d->gr[NL],
                      d->dloss);
-  const int64_t chunk = 2048;
-  const unsigned nch = (unsigned)((V + chunk - 1) / chunk);
+  const int nvb = (int)(ceil_div(g.nx, CTX) * ceil_div(g.ny, CTY) * ceil_div(g.nz, CTZ));
+  const dim3 wgrid((unsigned)std::min(nvb, WG_BLOCKS)), wblk(256);
   float* gw[NL];
   float* gb[NL];
   for (int l = 0; l < NL; l++) {
@@ -417,22 +472,20 @@ int dn_backward(swap_ctx* c, const float* tgt) {
     gb[l] = gw[l] + wcount(l);
   }
   // layer 4 (32 -> 4, no activation)
-  hipLaunchKernelGGL((k_wgrad<32, 4>), dim3(nch, k16(32) / 16, 1), dim3(64), 0, c->stream, g,
-                     d->act[3], d->gr[4], gw[3], chunk);
+  hipLaunchKernelGGL((k_wgrad<32, 4>), wgrid, wblk, 0, c->stream, g, d->act[3], d->gr[4], gw[3]);
   hipLaunchKernelGGL(k_bgrad<4>, dim3(256), dim3(256), 0, c->stream, V, d->gr[4], gb[3]);
   hipLaunchKernelGGL((k_conv_t<4, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[4], d->Wb[3],
                      nullptr, d->act[3], nullptr, d->gr[3], nullptr);
   // layers 3 and 2 (32 -> 32)
   for (int l = 2; l >= 1; l--) {
-    hipLaunchKernelGGL((k_wgrad<32, 32>), dim3(nch, k16(32) / 16, 2), dim3(64), 0, c->stream, g,
-                       d->act[l], d->gr[l + 1], gw[l], chunk);
+    hipLaunchKernelGGL((k_wgrad<32, 32>), wgrid, wblk, 0, c->stream, g, d->act[l], d->gr[l + 1],
+                       gw[l]);
     hipLaunchKernelGGL(k_bgrad<32>, dim3(256), dim3(256), 0, c->stream, V, d->gr[l + 1], gb[l]);
     hipLaunchKernelGGL((k_conv_t<32, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[l + 1],
                        d->Wb[l], nullptr, d->act[l], nullptr, d->gr[l], nullptr);
   }
   // layer 1 (4 -> 32); no gradient to the input
-  hipLaunchKernelGGL((k_wgrad<4, 32>), dim3(nch, k16(4) / 16, 2), dim3(64), 0, c->stream, g,
-                     d->act[0], d->gr[1], gw[0], chunk);
+  hipLaunchKernelGGL((k_wgrad<4, 32>), wgrid, wblk, 0, c->stream, g, d->act[0], d->gr[1], gw[0]);
   hipLaunchKernelGGL(k_bgrad<32>, dim3(256), dim3(256), 0, c->stream, V, d->gr[1], gb[0]);
   return post(c);
 }
