@@ -6,11 +6,13 @@
 // first three layers, residual output (out = in +
 // conv4(...)).  Activations are channels-last BF16 [voxel][channel] over the
 // interior nodes (i fastest).  Every convolution is an implicit GEMM on MFMA
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulation): M = voxels, N = output
-// channels, K = 27 taps x input channels.  An M-tile is 16 voxels along x;
-// lane l supplies voxel l%16 and the four K entries 4(l/16)..4(l/16)+3 of the
-// A operand, output channel l%16 of the B operand, and receives outputs
-// (voxel 4(l/16)+e, channel l%16).
+// (FP32 accumulation): M = voxels, N = output channels, K = 27 taps x input
+// channels.  An M-tile is 16 voxels along x; lane l supplies voxel l%16 of
+// the A operand and output channel l%16 of the B operand, and receives
+// outputs (voxel 4(l/16)+e, channel l%16).  With 32 input channels one
+// v_mfma_f32_16x16x32_bf16 takes a whole tap (lane l: channels
+// 8(l/16)..8(l/16)+7); with 4, v_mfma_f32_16x16x16_bf16 takes four taps'
+// channels in K entries 4(l/16)..4(l/16)+3.
 //
 // Backward: the loss gradient is propagated with the same kernel through the
 // tap-flipped, transposed weights (the adjoint of a periodic convolution),
@@ -27,6 +29,8 @@
 namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 using bf16 = uint16_t;
 
@@ -78,9 +82,10 @@ __global__ __launch_bounds__(256) void k_conv_t(G3 g, const bf16* __restrict__ X
                                                 const float* __restrict__ resid,
                                                 bf16* __restrict__ Yb, float* __restrict__ Yf) {
   static_assert(CIN == 4 || CIN == 32, "input channels");
-  // LDS stride per voxel in BF16: 32 channels padded to 36 (72 B), so the 16
-  // voxels one lane group reads fall in distinct banks
-  constexpr int VS = CIN == 32 ? 36 : CIN;
+  // LDS stride per voxel in BF16: 32 channels padded to 40 (80 B: 16-byte
+  // aligned rows, and the 16 voxels of one lane group's 16-byte reads fall
+  // in distinct banks)
+  constexpr int VS = CIN == 32 ? 40 : CIN;
   constexpr int K = 27 * CIN, KP = k16(CIN), NT = r16(COUT) / 16, CPV = CIN / 4;
   __shared__ __align__(16) bf16 sx[CHV * VS];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -100,6 +105,29 @@ __global__ __launch_bounds__(256) void k_conv_t(G3 g, const bf16* __restrict__ X
   for (int mt = 0; mt < CTZ; mt++)
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
+  if constexpr (CIN == 32) {
+    // one tap's 32 channels per MFMA (16x16x32): lane group gq supplies
+    // channels 8 gq .. 8 gq + 7 of A (voxel m + tap offset) and of B (weight
+    // row m), the same K slots on both sides
+    const int ci = 8 * gq;
+    for (int t = 0; t < 27; t++) {
+      const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
+      bf8 a[CTZ];
+#pragma unroll
+      for (int mt = 0; mt < CTZ; mt++) {
+        const int hv = (m + dx) + CHX * ((w + dy) + CHY * (mt + dz));
+        a[mt] = __builtin_bit_cast(bf8, *(const s8*)(sx + hv * VS + ci));
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        const bf8 b =
+            __builtin_bit_cast(bf8, *(const s8*)(Wt + (int64_t)(nt * 16 + m) * KP + 32 * t + ci));
+#pragma unroll
+        for (int mt = 0; mt < CTZ; mt++)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b, acc[mt][nt], 0, 0, 0);
+      }
+    }
+  } else
   for (int kk = 0; kk < KP; kk += 16) {
     const int k0 = kk + 4 * gq;
     const bool kok = k0 < K;  // the zero-padded tail of K (taps past 26)
@@ -159,12 +187,14 @@ __global__ __launch_bounds__(256) void k_conv_t(G3 g, const bf16* __restrict__ X
 // fall in distinct banks), and keeps its partial dW in registers across the
 // blocks: wavefront w owns row tiles w, w + 4, ...; lane l supplies weight
 // row l % 16 (A) and output channel l % 16 (B) at the run's voxels
-// 4(l/16) .. 4(l/16)+3, and holds rows 4(l/16)+e of channel l % 16.  Each
-// workgroup adds its partial to dW once, with FP32 atomics.
+// 4(l/16) .. 4(l/16)+3, and holds rows 4(l/16)+e of channel l % 16.  The
+// bias gradient db[co] = sum_v dY[v][co] is summed from the same LDS tiles.
+// Each workgroup adds its partials to dW and db once, with FP32 atomics.
 constexpr int WG_BLOCKS = 512;  // workgroups of a weight-gradient launch
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_wgrad(G3 g, const bf16* __restrict__ X,
-                                              const bf16* __restrict__ dY, float* dW) {
+                                              const bf16* __restrict__ dY, float* dW,
+                                              float* db) {
   static_assert(CIN == 4 || CIN == 32, "input channels");
   static_assert(COUT == 4 || COUT == 32, "output channels");
   constexpr int VS = CIN == 32 ? 36 : CIN, YS = COUT == 32 ? 36 : COUT;
@@ -172,7 +202,9 @@ __global__ __launch_bounds__(256) void k_wgrad(G3 g, const bf16* __restrict__ X,
   constexpr int MTW = (MT + 3) / 4, BV = CTX * CTY * CTZ, YPV = COUT / 4;
   __shared__ __align__(16) bf16 sx[CHV * VS];
   __shared__ __align__(16) bf16 sy[BV * YS];
+  __shared__ float sb[256];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  float bsum = 0.f;  // db partial of channel tid % COUT over voxels tid / COUT + k 256 / COUT
   const int m = lane & 15, gq = lane >> 4;
   const int nbx = (g.nx + CTX - 1) / CTX, nby = (g.ny + CTY - 1) / CTY,
             nbz = (g.nz + CTZ - 1) / CTZ;
@@ -203,6 +235,7 @@ __global__ __launch_bounds__(256) void k_wgrad(G3 g, const bf16* __restrict__ X,
       *(s4*)(sy + lv * YS + 4 * ch) = v;
     }
     __syncthreads();
+    for (int lv = tid / COUT; lv < BV; lv += 256 / COUT) bsum += bf2f(sy[lv * YS + tid % COUT]);
     for (int r = 0; r < CTY * CTZ; r++) {  // the block's 16-voxel x-runs
       const int ly = r % CTY, lz = r / CTY;
       s4 b[NT];
@@ -246,23 +279,12 @@ __global__ __launch_bounds__(256) void k_wgrad(G3 g, const bf16* __restrict__ X,
       }
     }
   }
-}
-
-// db[co] += sum_v dY[v][co]
-template <int COUT>
-__global__ __launch_bounds__(256) void k_bgrad(int64_t V, const bf16* __restrict__ dY, float* db) {
-  __shared__ float s[256];
-  constexpr int PER = 256 / COUT;
-  const int co = threadIdx.x % COUT, r = threadIdx.x / COUT;
-  float sum = 0.f;
-  for (int64_t v = (int64_t)blockIdx.x * PER + r; v < V; v += (int64_t)gridDim.x * PER)
-    sum += bf2f(dY[v * COUT + co]);
-  s[threadIdx.x] = sum;
+  sb[tid] = bsum;
   __syncthreads();
-  if (threadIdx.x < COUT) {
+  if (tid < COUT) {
     float t = 0.f;
-    for (int q = 0; q < PER; q++) t += s[q * COUT + threadIdx.x];
-    atomicAdd(db + threadIdx.x, t);
+    for (int q = tid; q < 256; q += COUT) t += sb[q];
+    atomicAdd(db + tid, t);
   }
 }
 
@@ -472,21 +494,20 @@ int dn_backward(swap_ctx* c, const float* tgt) {
     gb[l] = gw[l] + wcount(l);
   }
   // layer 4 (32 -> 4, no activation)
-  hipLaunchKernelGGL((k_wgrad<32, 4>), wgrid, wblk, 0, c->stream, g, d->act[3], d->gr[4], gw[3]);
-  hipLaunchKernelGGL(k_bgrad<4>, dim3(256), dim3(256), 0, c->stream, V, d->gr[4], gb[3]);
+  hipLaunchKernelGGL((k_wgrad<32, 4>), wgrid, wblk, 0, c->stream, g, d->act[3], d->gr[4], gw[3],
+                     gb[3]);
   hipLaunchKernelGGL((k_conv_t<4, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[4], d->Wb[3],
                      nullptr, d->act[3], nullptr, d->gr[3], nullptr);
   // layers 3 and 2 (32 -> 32)
   for (int l = 2; l >= 1; l--) {
     hipLaunchKernelGGL((k_wgrad<32, 32>), wgrid, wblk, 0, c->stream, g, d->act[l], d->gr[l + 1],
-                       gw[l]);
-    hipLaunchKernelGGL(k_bgrad<32>, dim3(256), dim3(256), 0, c->stream, V, d->gr[l + 1], gb[l]);
+                       gw[l], gb[l]);
     hipLaunchKernelGGL((k_conv_t<32, 32, EPI_MASK>), grid, blk, 0, c->stream, g, d->gr[l + 1],
                        d->Wb[l], nullptr, d->act[l], nullptr, d->gr[l], nullptr);
   }
   // layer 1 (4 -> 32); no gradient to the input
-  hipLaunchKernelGGL((k_wgrad<4, 32>), wgrid, wblk, 0, c->stream, g, d->act[0], d->gr[1], gw[0]);
-  hipLaunchKernelGGL(k_bgrad<32>, dim3(256), dim3(256), 0, c->stream, V, d->gr[1], gb[0]);
+  hipLaunchKernelGGL((k_wgrad<4, 32>), wgrid, wblk, 0, c->stream, g, d->act[0], d->gr[1], gw[0],
+                     gb[0]);
   return post(c);
 }
 
