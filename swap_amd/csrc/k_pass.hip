@@ -217,16 +217,15 @@ __device__ __forceinline__ int xnode(const Geo& g, int u) {
 // A 512-thread workgroup is 2 waves on each of a CU's 4 SIMDs, so occupancy
 // moves in whole workgroups: two of them (4 waves per SIMD) share a CU when
 // a wave needs at most 128 VGPRs and the LDS holds both 64 KB windows.  The
-// drifted P2 is held to that (127 VGPRs, no spills; left to itself it took
-// 132 and lost the second workgroup: 5.3 -> 6.7 ms per pass); the drifted P1
-// fits it unasked (104), and holding it there as well cost 0.2 ms per pass
-// (the scheduler's choices, same occupancy).  The drifted P3 would spill some
-// 200 VGPRs at 128 and, like the passes right after a sort (the P2/P3 gather
-// tile takes the LDS of a second workgroup), runs one workgroup per CU with
-// the registers it needs; 2 is the compiler's own default for 512 threads.
+// drifted FP32 P2 and P3 are held to that (93 and 128 VGPRs, no spills; P1
+// fits unasked with 90); P3 took 165 and one workgroup per CU before its
+// early exit for dead cells bounded the live ranges (9.8 -> 8.5 ms).  The
+// passes right after a sort (the P2/P3 gather tile takes the LDS of a second
+// workgroup) and the FP64 build run one workgroup per CU with the registers
+// they need; 2 is the compiler's own default for 512 threads.
 template <typename R, int MODE, int KP>
 struct PassWaves {
-  static constexpr int value = (sizeof(R) == 4 && KP == 1 && MODE == 2) ? 4 : 2;
+  static constexpr int value = (sizeof(R) == 4 && KP == 1 && MODE >= 2) ? 4 : 2;
 };
 
 template <typename R, int MODE, int KP, bool OX>
@@ -404,8 +403,11 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
       if (BUFW && p < i0) continue;  // before the item: another item's particle
       const uint32_t c = P.cell[p];
       // an ion that left through an open x face in an earlier step waits,
-      // untouched, for the sort that drops it
-      if (OX && c == DEAD_CELL) {
+      // untouched, for the sort that drops it.  A periodic run has no such
+      // cells, but the test is kept there too: the early exit bounds the
+      // live ranges, and the drifted periodic P3 fits 128 VGPRs with it (two
+      // workgroups per CU) where it spilled 212 without
+      if (c == DEAD_CELL) {
         gone |= 1u << k;
         continue;
       }

@@ -114,10 +114,7 @@ __device__ __forceinline__ int half_move(double& f, double v, double s) {
   f = t - sh;
   return (int)sh;
 }
-__device__ __forceinline__ int wrapc(int c, int nc) {
-  c %= nc;
-  return c < 0 ? c + nc : c;
-}
+__device__ __forceinline__ int wrapc(int c, int nc) { return wrap_range(c, nc); }
 
 // order-preserving int image of a float (for integer atomic min/max)
 __device__ __forceinline__ int fkey(float f) {

@@ -28,11 +28,15 @@ SWAP_HD int canon(int s, int n) { return s == 0 ? n - 2 : (s == n - 1 ? 1 : s); 
 // storage index of unwrapped node coordinate u (node u = lower corner of
 // cell u); folds ghost contributions into their periodic images
 // (update_np's fold, gutsp.f:1496-1498)
-SWAP_HD int node_of(int u, int nc) {
-  int r = (u - 1) % nc;
-  if (r < 0) r += nc;
-  return r + 1;
+// r mod nc in [0, nc) for any r, without an integer division: callers pass
+// coordinates within a period or two of the grid, which take one add or
+// subtract at most (a division costs dozens of instructions and registers)
+SWAP_HD int wrap_range(int r, int nc) {
+  while (r < 0) r += nc;
+  while (r >= nc) r -= nc;
+  return r;
 }
+SWAP_HD int node_of(int u, int nc) { return wrap_range(u - 1, nc) + 1; }
 
 // The x axis of an inflow/outflow run (Geo::ox) has no periodic image: its
 // ghost layers are zero-gradient copies of the nearest interior node, and
