@@ -26,11 +26,15 @@
 // (reduced by the previous P3).  Particles outside the window deposit
 // straight to HBM (counted as misses).
 //
-// Right after a sort, P2 and P3 gather the 6-component node source
+// P3 always takes one particle per lane: right after a sort its 4-per-lane
+// form (gather tile in LDS, one workgroup per CU) ran at 10.2 ms against
+// 7.6 ms for the one-per-lane form at two workgroups per CU, whose segmented
+// scan merges the sorted runs across lanes instead.
+//
+// Right after a sort, P2 gathers the 6-component node source
 // S = (ajc - upc, btc) from an LDS copy of the item's node box (one 32-byte
 // record per node), filled with coalesced loads when the item starts; on
-// later steps the boxes have drifted apart and the gather reads S through
-// the caches.
+// later steps, and in P3, the gather reads S through the caches.
 #include <type_traits>
 #include "particle_math.h"
 
@@ -190,17 +194,6 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// a[k] = v for a loop index k that is the same in every lane (the selects
-// keep a[] in registers; an indexed store would put it in scratch memory)
-template <typename T>
-__device__ __forceinline__ void lane_put(T (&a)[4], int k, T v) {
-#pragma unroll
-  for (int j = 0; j < 4; j++) a[j] = k == j ? v : a[j];
-}
-template <typename T>
-__device__ __forceinline__ void lane_put(T (&a)[1], int, T v) {
-  a[0] = v;
-}
 
 // node of unwrapped x node coordinate u: periodic, or on an open x (OX) the
 // clamp of node_of_x.  A template flag, so that the periodic kernels carry
@@ -210,22 +203,23 @@ __device__ __forceinline__ int xnode(const Geo& g, int u) {
   return OX ? (u < 1 ? 1 : (u > g.ncx ? g.ncx : u)) : node_of(u, g.ncx);
 }
 
-// KP particles per lane: PASS_KP right after a sort (cells in runs), 1
-// otherwise (the cells of neighbouring particles differ; the scan across the
-// wavefront then merges neighbours that still share a cell).  OX: open x.
+// KP particles per lane: PASS_KP in P1 and P2 right after a sort (cells in
+// runs), 1 otherwise and in P3 (the scan across the wavefront merges
+// neighbours that share a cell).  OX: open x.
 // Waves per SIMD each pass variant is compiled for (its register budget).
 // A 512-thread workgroup is 2 waves on each of a CU's 4 SIMDs, so occupancy
 // moves in whole workgroups: two of them (4 waves per SIMD) share a CU when
 // a wave needs at most 128 VGPRs and the LDS holds both 64 KB windows.  The
 // drifted FP32 P2 and P3 are held to that (93 and 128 VGPRs, no spills; P1
 // fits unasked with 90); P3 took 165 and one workgroup per CU before its
-// early exit for dead cells bounded the live ranges (9.8 -> 8.5 ms).  The
-// passes right after a sort (the P2/P3 gather tile takes the LDS of a second
-// workgroup) and the FP64 build run one workgroup per CU with the registers
-// they need; 2 is the compiler's own default for 512 threads.
+// early exit for dead cells bounded the live ranges (9.8 -> 8.5 ms).  P1
+// right after a sort is held there too (24 VGPRs spill; 6.37 -> 6.28 ms
+// mean P1).  P2 right after a sort (its gather tile takes the LDS of a
+// second workgroup) and the FP64 build run one workgroup per CU with the
+// registers they need; 2 is the compiler's own default for 512 threads.
 template <typename R, int MODE, int KP>
 struct PassWaves {
-  static constexpr int value = (sizeof(R) == 4 && KP == 1 && MODE >= 2) ? 4 : 2;
+  static constexpr int value = (sizeof(R) == 4 && (KP == 1 ? MODE >= 2 : MODE == 1)) ? 4 : 2;
 };
 
 template <typename R, int MODE, int KP, bool OX>
@@ -372,19 +366,12 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
   float vlo[3] = {3e38f, 3e38f, 3e38f}, vhi[3] = {-3e38f, -3e38f, -3e38f};
   unsigned long long misses = 0;
 
-  // P3 with 4 consecutive particles per lane (FP32) keeps the lane's new
-  // state in registers and writes each array with one 16-byte store: a store
-  // instruction then covers whole cache lines (per-particle stores left each
-  // line a quarter written per instruction, and lines reached HBM in pieces).
-  // Lane groups start at multiples of 4 particles, so those stores are
-  // aligned; a group that straddles the item's ends is written per particle.
-  constexpr bool BUFW = MODE == 3 && KP == 4 && sizeof(R) == 4;
-  const int64_t ib = BUFW ? (i0 & ~(int64_t)3) : i0;
-  for (int64_t chunk = ib; chunk < i1; chunk += (int64_t)PASS_WG * KP) {
+  // P3 runs with one particle per lane (KP = 1) also right after a sort:
+  // consecutive lanes then write consecutive particles, so every store
+  // instruction covers whole cache lines
+  static_assert(MODE != 3 || KP == 1, "P3 takes one particle per lane");
+  for (int64_t chunk = i0; chunk < i1; chunk += (int64_t)PASS_WG * KP) {
     const int64_t pb = chunk + (int64_t)tid * KP;
-    float wv[BUFW ? 9 : 1][BUFW ? 4 : 1];  // v1 xyz, v xyz, fraction xyz per particle
-    uint32_t wc[BUFW ? 4 : 1];              // packed cell per particle
-    unsigned gone = 0;                      // bit k: particle k left earlier (kept as is)
     int cur = -1;   // window cell of the running sums acc
     int hkey = -1;  // window cell of hd: the lane's first cell, once the lane left it
     R acc[8][M], hd[8][M];
@@ -400,17 +387,13 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
     for (int k = 0; k < KP; k++) {
       const int64_t p = pb + k;
       if (p >= i1) break;
-      if (BUFW && p < i0) continue;  // before the item: another item's particle
       const uint32_t c = P.cell[p];
       // an ion that left through an open x face in an earlier step waits,
       // untouched, for the sort that drops it.  A periodic run has no such
       // cells, but the test is kept there too: the early exit bounds the
       // live ranges, and the drifted periodic P3 fits 128 VGPRs with it (two
       // workgroups per CU) where it spilled 212 without
-      if (c == DEAD_CELL) {
-        gone |= 1u << k;
-        continue;
-      }
+      if (c == DEAD_CELL) continue;
       const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
       R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
       const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
@@ -456,14 +439,8 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
             vlo[m] = fminf(vlo[m], (float)vn[m]);
             vhi[m] = fmaxf(vhi[m], (float)vn[m]);
           }
-          if constexpr (BUFW) {
-            lane_put(wv[0], k, (float)v[0]); lane_put(wv[1], k, (float)v[1]);
-            lane_put(wv[2], k, (float)v[2]); lane_put(wv[3], k, (float)vn[0]);
-            lane_put(wv[4], k, (float)vn[1]); lane_put(wv[5], k, (float)vn[2]);
-          } else {
-            P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
-            P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
-          }
+          P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
+          P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
           ux += half_move(f0, vn[0], K.s[0]);
           uy += half_move(f1, vn[1], K.s[1]);
           uz += half_move(f2, vn[2], K.s[2]);
@@ -540,33 +517,8 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
           bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
           bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
         }
-        if constexpr (BUFW) {
-          lane_put(wv[6], k, (float)f0); lane_put(wv[7], k, (float)f1);
-          lane_put(wv[8], k, (float)f2); lane_put(wc, k, cnew);
-        } else {
-          P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
-          P.cell[p] = cnew;
-        }
-      }
-    }
-    if constexpr (BUFW) {
-      float* dst[9] = {(float*)P.v1x, (float*)P.v1y, (float*)P.v1z, (float*)P.vx,
-                       (float*)P.vy,  (float*)P.vz,  (float*)P.fx,  (float*)P.fy,
-                       (float*)P.fz};
-      if (pb >= i0 && pb + 4 <= i1 && !gone) {  // pb is a multiple of 4: 16-byte aligned
-#pragma unroll
-        for (int a = 0; a < 9; a++)
-          *(float4*)(dst[a] + pb) = make_float4(wv[a][0], wv[a][1], wv[a][2], wv[a][3]);
-        *(uint4*)(P.cell + pb) = make_uint4(wc[0], wc[1], wc[2], wc[3]);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int64_t p = pb + k;
-          if (p < i0 || p >= i1 || ((gone >> k) & 1u)) continue;
-#pragma unroll
-          for (int a = 0; a < 9; a++) dst[a][p] = wv[a][k];
-          P.cell[p] = wc[k];
-        }
+        P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
+        P.cell[p] = cnew;
       }
     }
     // the first cell of a lane that moved on is its own
@@ -757,10 +709,6 @@ int pl_pass(swap_ctx* c, int mode) {
       hipLaunchKernelGGL((k_pass<R, 2, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S,
                          acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
                          c->vb_next, c->dcnt, c->tboxes);
-    else if (runs)
-      hipLaunchKernelGGL((k_pass<R, 3, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
-                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt, c->tboxes);
     else
       hipLaunchKernelGGL((k_pass<R, 3, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
                          c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
