@@ -213,13 +213,14 @@ __device__ __forceinline__ int xnode(const Geo& g, int u) {
 // drifted FP32 P2 and P3 are held to that (93 and 128 VGPRs, no spills; P1
 // fits unasked with 90); P3 took 165 and one workgroup per CU before its
 // early exit for dead cells bounded the live ranges (9.8 -> 8.5 ms).  P1
-// right after a sort is held there too (24 VGPRs spill; 6.37 -> 6.28 ms
-// mean P1).  P2 right after a sort (its gather tile takes the LDS of a
-// second workgroup) and the FP64 build run one workgroup per CU with the
-// registers they need; 2 is the compiler's own default for 512 threads.
+// right after a sort would spill 24 VGPRs at 128 (its HBM traffic doubled
+// for 1 % of its time), and P2 right after a sort gives the LDS of a second
+// workgroup to its gather tile (without the tile: 4.9 -> 5.7 ms); they and
+// the FP64 build run one workgroup per CU with the registers they need; 2
+// is the compiler's own default for 512 threads.
 template <typename R, int MODE, int KP>
 struct PassWaves {
-  static constexpr int value = (sizeof(R) == 4 && (KP == 1 ? MODE >= 2 : MODE == 1)) ? 4 : 2;
+  static constexpr int value = (sizeof(R) == 4 && KP == 1 && MODE >= 2) ? 4 : 2;
 };
 
 template <typename R, int MODE, int KP, bool OX>
