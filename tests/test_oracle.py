@@ -759,6 +759,24 @@ def test_step_fp32_tracks_fp64_and_conserves_charge():
     assert total == pytest.approx(n, rel=1e-12)
 
 
+def test_step_threads_repeat_and_agree():
+    """the OpenMP oracle (the bench's CPU baseline): each thread deposits
+    into its own grid copy and the copies are summed in thread order, so a
+    thread count repeats bit for bit; 1 and 4 threads differ only by the
+    order of those sums"""
+    out = []
+    for threads in (4, 4, 1):
+        o = make(width=64, npart=4 * int(np.prod(NC)))
+        o.set_threads(threads)
+        st = seeded_state(o, 64)
+        o.step(st)
+        out.append(st)
+    for name in ("xp", "vp", "np", "up", "b1", "E"):
+        assert np.array_equal(out[0][name], out[1][name]), name
+        x, y = out[0][name], out[2][name]
+        assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(y)), name
+
+
 def test_chex_draw_follows_the_ion_id():
     """keyed by id, a permutation of the ions permutes the decisions (the
     sort cannot change who is hit); keyed by index it does not"""
