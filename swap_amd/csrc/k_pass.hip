@@ -384,17 +384,16 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
         acc[cc][m] = (R)0;
         if (KP > 1) hd[cc][m] = (R)0;
       }
-#pragma unroll 1
-    for (int k = 0; k < KP; k++) {
-      const int64_t p = pb + k;
-      if (p >= i1) break;
-      const uint32_t c = P.cell[p];
-      // an ion that left through an open x face in an earlier step waits,
-      // untouched, for the sort that drops it.  A periodic run has no such
-      // cells, but the test is kept there too: the early exit bounds the
-      // live ranges, and the drifted periodic P3 fits 128 VGPRs with it (two
-      // workgroups per CU) where it spilled 212 without
-      if (c == DEAD_CELL) continue;
+    if constexpr (KP == 1) {
+      // One particle per lane, computed by every lane: a lane past the
+      // item's end or holding an ion that left reads a loadable particle and
+      // contributes zero moments, so the 8 x M sums are plain products with
+      // no zeroing on divergent paths (each such path cost 32 moves per
+      // chunk); only the stores and the out-of-window deposit are guarded.
+      const int64_t p = pb < i1 ? pb : i0;
+      const uint32_t craw = P.cell[p];
+      const bool valid = pb < i1 && craw != DEAD_CELL;
+      const uint32_t c = valid ? craw : 0u;
       const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
       R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
       const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
@@ -402,7 +401,7 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
       weights8(f0, f1, f2, w8);
       int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
           uz = unwrap(ck, cen[2], g.ncz);
-      bool alive = true;  // false once the particle left through an open x face
+      bool alive = valid;  // false once the particle left through an open x face
       const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
       if constexpr (MODE == 1) {
         q[0] = r;
@@ -411,21 +410,7 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
         q[3] = r * ((R)1.5 * v[2] - (R)0.5 * P.v1z[p]);
       } else {
         R Ag[3], Bg[3], E[3], vm[3], vpl[3];
-        const int tx = ux - bl[0], ty = uy - bl[1], tz = uz - bl[2];
-        if (tiled && (unsigned)tx < (unsigned)(T[0] - 1) && (unsigned)ty < (unsigned)(T[1] - 1) &&
-            (unsigned)tz < (unsigned)(T[2] - 1)) {
-          const int tb = tx + T[0] * (ty + T[1] * tz);
-#pragma unroll
-          for (int m = 0; m < 3; m++) Ag[m] = Bg[m] = (R)0;
-#pragma unroll
-          for (int cc = 0; cc < 8; cc++) {
-            const float4 a = gtile[2 * (tb + toff[cc])], b = gtile[2 * (tb + toff[cc]) + 1];
-            Ag[0] += w8[cc] * a.x; Ag[1] += w8[cc] * a.y; Ag[2] += w8[cc] * a.z;
-            Bg[0] += w8[cc] * a.w; Bg[1] += w8[cc] * b.x; Bg[2] += w8[cc] * b.y;
-          }
-        } else {
-          gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
-        }
+        gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
         const R mr = P.mr ? (R)P.mr[p] : (R)1;
         ep_of(Ag, Bg, mr, E);
         rotate(K.dt, v, E, Bg, mr, vm, vpl);
@@ -437,21 +422,23 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
 #pragma unroll
           for (int m = 0; m < 3; m++) {
             vn[m] = vpl[m] + hdt * E[m];
-            vlo[m] = fminf(vlo[m], (float)vn[m]);
-            vhi[m] = fmaxf(vhi[m], (float)vn[m]);
+            if (valid) {
+              vlo[m] = fminf(vlo[m], (float)vn[m]);
+              vhi[m] = fmaxf(vhi[m], (float)vn[m]);
+            }
           }
-          P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
-          P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
+          if (valid) {
+            P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
+            P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
+          }
           ux += half_move(f0, vn[0], K.s[0]);
           uy += half_move(f1, vn[1], K.s[1]);
           uz += half_move(f2, vn[2], K.s[2]);
-          if (OX) alive = ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}: no deposit
+          if (OX) alive = alive && ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}
           weights8(f0, f1, f2, w8);
           q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
         }
       }
-      // fixed-point units (a power-of-two scale: exact), and the bound of
-      // every |w q| of this particle
       float pmag = 0.f;
       if constexpr (FIX) {
 #pragma unroll
@@ -461,40 +448,9 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
         }
       }
       const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
-      const bool inw = windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
+      const bool inw = alive && windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
                        ly < W[1] - 1 && lz < W[2] - 1;
-      if (!alive) {
-      } else if (inw) {
-        const int key = lx + W[0] * (ly + W[1] * lz);
-        if (KP > 1 && key != cur) {
-          if (cur >= 0) {
-            if (hkey < 0) {  // keep the first cell's sums for the end of the chunk
-              hkey = cur;
-              hmag = amag;
-#pragma unroll
-              for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                  hd[cc][m] = acc[cc][m];
-                  acc[cc][m] = (R)0;
-                }
-            } else {  // a cell in the middle of the lane's particles
-              win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
-#pragma unroll
-              for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-                for (int m = 0; m < M; m++) acc[cc][m] = (R)0;
-            }
-            amag = 0.f;
-          }
-        }
-        cur = key;
-        amag += pmag;
-#pragma unroll
-        for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-          for (int m = 0; m < M; m++) acc[cc][m] += w8[cc] * q[m];
-      } else {
+      if (alive && !inw) {
         misses++;
 #pragma unroll
         for (int cc = 0; cc < 8; cc++) {
@@ -505,21 +461,170 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
           for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
         }
       }
+#pragma unroll
+      for (int m = 0; m < M; m++) q[m] = inw ? q[m] : (R)0;
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
+      cur = inw ? lx + W[0] * (ly + W[1] * lz) : -1;
+      amag = inw ? pmag : 0.f;
       if constexpr (MODE == 3) {
         ux += half_move(f0, vn[0], K.s[0]);
         uy += half_move(f1, vn[1], K.s[1]);
         uz += half_move(f2, vn[2], K.s[2]);
-        uint32_t cnew;
-        if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
-          cnew = DEAD_CELL;  // the sort at the end of the step drops it
-        } else {
-          cnew = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
-          bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
-          bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
-          bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+        if (valid) {
+          uint32_t cnew;
+          if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
+            cnew = DEAD_CELL;  // the next sort drops it
+          } else {
+            cnew = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
+            bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
+            bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
+            bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+          }
+          P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
+          P.cell[p] = cnew;
         }
-        P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
-        P.cell[p] = cnew;
+      }
+    } else {
+  #pragma unroll 1
+      for (int k = 0; k < KP; k++) {
+        const int64_t p = pb + k;
+        if (p >= i1) break;
+        const uint32_t c = P.cell[p];
+        // an ion that left through an open x face in an earlier step waits,
+        // untouched, for the sort that drops it.  A periodic run has no such
+        // cells, but the test is kept there too: the early exit bounds the
+        // live ranges, and the drifted periodic P3 fits 128 VGPRs with it (two
+        // workgroups per CU) where it spilled 212 without
+        if (c == DEAD_CELL) continue;
+        const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
+        R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
+        const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
+        R w8[8], q[M], vn[3] = {0, 0, 0};
+        weights8(f0, f1, f2, w8);
+        int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
+            uz = unwrap(ck, cen[2], g.ncz);
+        bool alive = true;  // false once the particle left through an open x face
+        const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
+        if constexpr (MODE == 1) {
+          q[0] = r;
+          q[1] = r * ((R)1.5 * v[0] - (R)0.5 * P.v1x[p]);
+          q[2] = r * ((R)1.5 * v[1] - (R)0.5 * P.v1y[p]);
+          q[3] = r * ((R)1.5 * v[2] - (R)0.5 * P.v1z[p]);
+        } else {
+          R Ag[3], Bg[3], E[3], vm[3], vpl[3];
+          const int tx = ux - bl[0], ty = uy - bl[1], tz = uz - bl[2];
+          if (tiled && (unsigned)tx < (unsigned)(T[0] - 1) && (unsigned)ty < (unsigned)(T[1] - 1) &&
+              (unsigned)tz < (unsigned)(T[2] - 1)) {
+            const int tb = tx + T[0] * (ty + T[1] * tz);
+  #pragma unroll
+            for (int m = 0; m < 3; m++) Ag[m] = Bg[m] = (R)0;
+  #pragma unroll
+            for (int cc = 0; cc < 8; cc++) {
+              const float4 a = gtile[2 * (tb + toff[cc])], b = gtile[2 * (tb + toff[cc]) + 1];
+              Ag[0] += w8[cc] * a.x; Ag[1] += w8[cc] * a.y; Ag[2] += w8[cc] * a.z;
+              Bg[0] += w8[cc] * a.w; Bg[1] += w8[cc] * b.x; Bg[2] += w8[cc] * b.y;
+            }
+          } else {
+            gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
+          }
+          const R mr = P.mr ? (R)P.mr[p] : (R)1;
+          ep_of(Ag, Bg, mr, E);
+          rotate(K.dt, v, E, Bg, mr, vm, vpl);
+          if constexpr (MODE == 2) {
+  #pragma unroll
+            for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
+          } else {
+            const R hdt = (R)0.5 * K.dt;
+  #pragma unroll
+            for (int m = 0; m < 3; m++) {
+              vn[m] = vpl[m] + hdt * E[m];
+              vlo[m] = fminf(vlo[m], (float)vn[m]);
+              vhi[m] = fmaxf(vhi[m], (float)vn[m]);
+            }
+            P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
+            P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
+            ux += half_move(f0, vn[0], K.s[0]);
+            uy += half_move(f1, vn[1], K.s[1]);
+            uz += half_move(f2, vn[2], K.s[2]);
+            if (OX) alive = ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}: no deposit
+            weights8(f0, f1, f2, w8);
+            q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
+          }
+        }
+        // fixed-point units (a power-of-two scale: exact), and the bound of
+        // every |w q| of this particle
+        float pmag = 0.f;
+        if constexpr (FIX) {
+  #pragma unroll
+          for (int m = 0; m < M; m++) {
+            q[m] *= (R)__int_as_float((127 + fix_bits(GM0 + m)) << 23);
+            pmag = fmaxf(pmag, fabsf((float)q[m]));
+          }
+        }
+        const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
+        const bool inw = windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
+                         ly < W[1] - 1 && lz < W[2] - 1;
+        if (!alive) {
+        } else if (inw) {
+          const int key = lx + W[0] * (ly + W[1] * lz);
+          if (KP > 1 && key != cur) {
+            if (cur >= 0) {
+              if (hkey < 0) {  // keep the first cell's sums for the end of the chunk
+                hkey = cur;
+                hmag = amag;
+  #pragma unroll
+                for (int cc = 0; cc < 8; cc++)
+  #pragma unroll
+                  for (int m = 0; m < M; m++) {
+                    hd[cc][m] = acc[cc][m];
+                    acc[cc][m] = (R)0;
+                  }
+              } else {  // a cell in the middle of the lane's particles
+                win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
+  #pragma unroll
+                for (int cc = 0; cc < 8; cc++)
+  #pragma unroll
+                  for (int m = 0; m < M; m++) acc[cc][m] = (R)0;
+              }
+              amag = 0.f;
+            }
+          }
+          cur = key;
+          amag += pmag;
+  #pragma unroll
+          for (int cc = 0; cc < 8; cc++)
+  #pragma unroll
+            for (int m = 0; m < M; m++) acc[cc][m] += w8[cc] * q[m];
+        } else {
+          misses++;
+  #pragma unroll
+          for (int cc = 0; cc < 8; cc++) {
+            int64_t node = gidx(g, xnode<OX>(g, ux + (cc & 1)),
+                                node_of(uy + ((cc >> 2) & 1), g.ncy),
+                                node_of(uz + ((cc >> 1) & 1), g.ncz));
+  #pragma unroll
+            for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
+          }
+        }
+        if constexpr (MODE == 3) {
+          ux += half_move(f0, vn[0], K.s[0]);
+          uy += half_move(f1, vn[1], K.s[1]);
+          uz += half_move(f2, vn[2], K.s[2]);
+          uint32_t cnew;
+          if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
+            cnew = DEAD_CELL;  // the sort at the end of the step drops it
+          } else {
+            cnew = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
+            bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
+            bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
+            bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+          }
+          P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
+          P.cell[p] = cnew;
+        }
       }
     }
     // the first cell of a lane that moved on is its own
