@@ -28,12 +28,18 @@ SWAP_HD int canon(int s, int n) { return s == 0 ? n - 2 : (s == n - 1 ? 1 : s); 
 // storage index of unwrapped node coordinate u (node u = lower corner of
 // cell u); folds ghost contributions into their periodic images
 // (update_np's fold, gutsp.f:1496-1498)
-// r mod nc in [0, nc) for any r, without an integer division: callers pass
-// coordinates within a period or two of the grid, which take one add or
-// subtract at most (a division costs dozens of instructions and registers)
+// r mod nc in [0, nc) for any r.  Callers pass coordinates within a period
+// of the grid, which take one conditional add and one conditional subtract;
+// anything farther (never in the particle passes) takes the division in a
+// branch of its own, so the common path carries no division (a division
+// costs some 20 instructions per call)
 SWAP_HD int wrap_range(int r, int nc) {
-  while (r < 0) r += nc;
-  while (r >= nc) r -= nc;
+  r += r < 0 ? nc : 0;
+  r -= r >= nc ? nc : 0;
+  if (__builtin_expect(r < 0 || r >= nc, 0)) {
+    r %= nc;
+    if (r < 0) r += nc;
+  }
   return r;
 }
 SWAP_HD int node_of(int u, int nc) { return wrap_range(u - 1, nc) + 1; }
