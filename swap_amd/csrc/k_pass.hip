@@ -210,14 +210,14 @@ __device__ __forceinline__ int xnode(const Geo& g, int u) {
 // A 512-thread workgroup is 2 waves on each of a CU's 4 SIMDs, so occupancy
 // moves in whole workgroups: two of them (4 waves per SIMD) share a CU when
 // a wave needs at most 128 VGPRs and the LDS holds both 64 KB windows.  The
-// drifted FP32 P2 and P3 are held to that (93 and 128 VGPRs, no spills; P1
-// fits unasked with 90); P3 took 165 and one workgroup per CU before its
-// early exit for dead cells bounded the live ranges (9.8 -> 8.5 ms).  P1
-// right after a sort would spill 24 VGPRs at 128 (its HBM traffic doubled
-// for 1 % of its time), and P2 right after a sort gives the LDS of a second
-// workgroup to its gather tile (without the tile: 4.9 -> 5.7 ms); they and
-// the FP64 build run one workgroup per CU with the registers they need; 2
-// is the compiler's own default for 512 threads.
+// one-particle-per-lane FP32 P2 and P3 are held to that (93 and 126 VGPRs,
+// no spills; P1 fits unasked with 97): P3 at one workgroup per CU (165
+// VGPRs when its divergent exits kept the 32 sums live on every path) took
+// 9.8 ms against 7.7.  P1 right after a sort would spill 24 VGPRs at 128
+// (its HBM traffic doubled for 1 % of its time), and P2 right after a sort
+// gives the LDS of a second workgroup to its gather tile (without the tile:
+// 4.9 -> 5.7 ms); they and the FP64 build run one workgroup per CU with the
+// registers they need; 2 is the compiler's own default for 512 threads.
 template <typename R, int MODE, int KP>
 struct PassWaves {
   static constexpr int value = (sizeof(R) == 4 && KP == 1 && MODE >= 2) ? 4 : 2;
@@ -488,25 +488,21 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
         }
       }
     } else {
-  #pragma unroll 1
+      // KP particles per lane (P1 and P2 right after a sort: cells in runs)
+      static_assert(MODE != 3, "P3 takes one particle per lane");
+#pragma unroll 1
       for (int k = 0; k < KP; k++) {
         const int64_t p = pb + k;
         if (p >= i1) break;
         const uint32_t c = P.cell[p];
-        // an ion that left through an open x face in an earlier step waits,
-        // untouched, for the sort that drops it.  A periodic run has no such
-        // cells, but the test is kept there too: the early exit bounds the
-        // live ranges, and the drifted periodic P3 fits 128 VGPRs with it (two
-        // workgroups per CU) where it spilled 212 without
-        if (c == DEAD_CELL) continue;
+        if (c == DEAD_CELL) continue;  // left through an open x face; the next sort drops it
         const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
-        R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
+        const R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
         const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
-        R w8[8], q[M], vn[3] = {0, 0, 0};
+        R w8[8], q[M];
         weights8(f0, f1, f2, w8);
-        int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
-            uz = unwrap(ck, cen[2], g.ncz);
-        bool alive = true;  // false once the particle left through an open x face
+        const int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
+                  uz = unwrap(ck, cen[2], g.ncz);
         const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
         if constexpr (MODE == 1) {
           q[0] = r;
@@ -519,9 +515,9 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
           if (tiled && (unsigned)tx < (unsigned)(T[0] - 1) && (unsigned)ty < (unsigned)(T[1] - 1) &&
               (unsigned)tz < (unsigned)(T[2] - 1)) {
             const int tb = tx + T[0] * (ty + T[1] * tz);
-  #pragma unroll
+#pragma unroll
             for (int m = 0; m < 3; m++) Ag[m] = Bg[m] = (R)0;
-  #pragma unroll
+#pragma unroll
             for (int cc = 0; cc < 8; cc++) {
               const float4 a = gtile[2 * (tb + toff[cc])], b = gtile[2 * (tb + toff[cc]) + 1];
               Ag[0] += w8[cc] * a.x; Ag[1] += w8[cc] * a.y; Ag[2] += w8[cc] * a.z;
@@ -533,32 +529,14 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
           const R mr = P.mr ? (R)P.mr[p] : (R)1;
           ep_of(Ag, Bg, mr, E);
           rotate(K.dt, v, E, Bg, mr, vm, vpl);
-          if constexpr (MODE == 2) {
-  #pragma unroll
-            for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
-          } else {
-            const R hdt = (R)0.5 * K.dt;
-  #pragma unroll
-            for (int m = 0; m < 3; m++) {
-              vn[m] = vpl[m] + hdt * E[m];
-              vlo[m] = fminf(vlo[m], (float)vn[m]);
-              vhi[m] = fmaxf(vhi[m], (float)vn[m]);
-            }
-            P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
-            P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
-            ux += half_move(f0, vn[0], K.s[0]);
-            uy += half_move(f1, vn[1], K.s[1]);
-            uz += half_move(f2, vn[2], K.s[2]);
-            if (OX) alive = ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}: no deposit
-            weights8(f0, f1, f2, w8);
-            q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
-          }
+#pragma unroll
+          for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
         }
         // fixed-point units (a power-of-two scale: exact), and the bound of
         // every |w q| of this particle
         float pmag = 0.f;
         if constexpr (FIX) {
-  #pragma unroll
+#pragma unroll
           for (int m = 0; m < M; m++) {
             q[m] *= (R)__int_as_float((127 + fix_bits(GM0 + m)) << 23);
             pmag = fmaxf(pmag, fabsf((float)q[m]));
@@ -567,63 +545,44 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
         const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
         const bool inw = windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
                          ly < W[1] - 1 && lz < W[2] - 1;
-        if (!alive) {
-        } else if (inw) {
+        if (inw) {
           const int key = lx + W[0] * (ly + W[1] * lz);
-          if (KP > 1 && key != cur) {
-            if (cur >= 0) {
-              if (hkey < 0) {  // keep the first cell's sums for the end of the chunk
-                hkey = cur;
-                hmag = amag;
-  #pragma unroll
-                for (int cc = 0; cc < 8; cc++)
-  #pragma unroll
-                  for (int m = 0; m < M; m++) {
-                    hd[cc][m] = acc[cc][m];
-                    acc[cc][m] = (R)0;
-                  }
-              } else {  // a cell in the middle of the lane's particles
-                win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
-  #pragma unroll
-                for (int cc = 0; cc < 8; cc++)
-  #pragma unroll
-                  for (int m = 0; m < M; m++) acc[cc][m] = (R)0;
-              }
-              amag = 0.f;
+          if (key != cur && cur >= 0) {
+            if (hkey < 0) {  // keep the first cell's sums for the end of the chunk
+              hkey = cur;
+              hmag = amag;
+#pragma unroll
+              for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                  hd[cc][m] = acc[cc][m];
+                  acc[cc][m] = (R)0;
+                }
+            } else {  // a cell in the middle of the lane's particles
+              win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
+#pragma unroll
+              for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+                for (int m = 0; m < M; m++) acc[cc][m] = (R)0;
             }
+            amag = 0.f;
           }
           cur = key;
           amag += pmag;
-  #pragma unroll
+#pragma unroll
           for (int cc = 0; cc < 8; cc++)
-  #pragma unroll
+#pragma unroll
             for (int m = 0; m < M; m++) acc[cc][m] += w8[cc] * q[m];
         } else {
           misses++;
-  #pragma unroll
+#pragma unroll
           for (int cc = 0; cc < 8; cc++) {
             int64_t node = gidx(g, xnode<OX>(g, ux + (cc & 1)),
                                 node_of(uy + ((cc >> 2) & 1), g.ncy),
                                 node_of(uz + ((cc >> 1) & 1), g.ncz));
-  #pragma unroll
+#pragma unroll
             for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
           }
-        }
-        if constexpr (MODE == 3) {
-          ux += half_move(f0, vn[0], K.s[0]);
-          uy += half_move(f1, vn[1], K.s[1]);
-          uz += half_move(f2, vn[2], K.s[2]);
-          uint32_t cnew;
-          if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
-            cnew = DEAD_CELL;  // the sort at the end of the step drops it
-          } else {
-            cnew = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
-            bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
-            bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
-            bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
-          }
-          P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
-          P.cell[p] = cnew;
         }
       }
     }
