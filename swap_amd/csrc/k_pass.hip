@@ -422,10 +422,9 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
 #pragma unroll
           for (int m = 0; m < 3; m++) {
             vn[m] = vpl[m] + hdt * E[m];
-            if (valid) {
-              vlo[m] = fminf(vlo[m], (float)vn[m]);
-              vhi[m] = fmaxf(vhi[m], (float)vn[m]);
-            }
+            const float fv = (float)vn[m];  // plain compares: no NaN quieting
+            vlo[m] = valid && fv < vlo[m] ? fv : vlo[m];
+            vhi[m] = valid && fv > vhi[m] ? fv : vhi[m];
           }
           if (valid) {
             P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];

@@ -46,17 +46,20 @@ __device__ __forceinline__ void src_node<float>(const float* __restrict__ S, int
 }
 
 // trilinear gather of S (gutsp.f:217-251); A = gathered (aj - up), B =
-// gathered btc; the corners are summed in the reference's corner order
+// gathered btc; the corners are summed in the reference's corner order.
+// Addresses: one record pointer per y-z corner pair (the x neighbour is the
+// next record, an immediate offset), so the 8 corners take 4 pointers.
 template <typename R>
 __device__ __forceinline__ void gather6(const Geo& g, const R* __restrict__ S, int64_t base,
                                         const R w[8], R A[3], R B[3]) {
-  int64_t off[8];
-  corner_off(g, off);
+  const int64_t sy = (int64_t)g.nx * SREC, sz = (int64_t)g.nx * g.ny * SREC;
+  const R* p0 = S + base * SREC;
+  const R* pc[4] = {p0, p0 + sz, p0 + sy, p0 + sy + sz};  // corners 0-1, 2-3, 4-5, 6-7
   R v[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     R c[6];
-    src_node(S, base + off[k], c);
+    src_node(pc[k >> 1], (int64_t)(k & 1), c);
 #pragma unroll
     for (int m = 0; m < 6; m++) v[m] += c[m] * w[k];
   }
