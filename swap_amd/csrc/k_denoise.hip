@@ -110,6 +110,7 @@ __global__ __launch_bounds__(256) void k_conv_t(G3 g, const bf16* __restrict__ X
     // channels 8 gq .. 8 gq + 7 of A (voxel m + tap offset) and of B (weight
     // row m), the same K slots on both sides
     const int ci = 8 * gq;
+#pragma unroll 3
     for (int t = 0; t < 27; t++) {
       const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
       bf8 a[CTZ];
