@@ -158,31 +158,89 @@ __device__ __forceinline__ double dppf(double v) {
   int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, true);
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
-// one step of the scan on every value of a lane's sums
-template <int CTRL, int ROWS, typename T, int M>
-__device__ __forceinline__ void seg_step(T (&a)[8][M], bool cond) {
+// One step of the scan on every value of a lane's sums (and on mag).  FP32:
+// a <- t * f + a with t the neighbour's value read through DPP and f 1
+// where the step's condition holds in this lane, else 0: one v_fmac_f32_dpp
+// per value (t * 1 + a rounds once, exactly as a + t; t * 0 + a is a).
+// The compiler keeps the DPP read a separate move, so the step is written
+// out.  The hazard recognizer does not look into the block: its first
+// instruction may read a register that the instruction before it wrote, or
+// follow a VALU write of EXEC, so it opens with 5 wait states; inside the
+// block no value is read by DPP within 2 instructions of its write.  Rows
+// outside the row mask are not written (the neighbour lies outside them).
+#define SWAP_FMAC_DPP(i, CTL) "v_fmac_f32_dpp %" #i ", %" #i ", %[f] " CTL "\n"
+#define SWAP_FMAC_DPP8(r, i0, i1, i2, i3, i4, i5, i6, i7, CTL)                        \
+  SWAP_FMAC_DPP(i0, CTL) SWAP_FMAC_DPP(i1, CTL) SWAP_FMAC_DPP(i2, CTL)                 \
+  SWAP_FMAC_DPP(i3, CTL) SWAP_FMAC_DPP(i4, CTL) SWAP_FMAC_DPP(i5, CTL)                 \
+  SWAP_FMAC_DPP(i6, CTL) SWAP_FMAC_DPP(i7, CTL)
+#define SWAP_SEG_STEP_ASM(NAME, CTL)                                                          \
+  __device__ __forceinline__ void NAME(float (&a)[8][4], float& mag, float f) {              \
+    asm("s_nop 4\n" SWAP_FMAC_DPP8(0, 0, 1, 2, 3, 4, 5, 6, 7, CTL)                          \
+            SWAP_FMAC_DPP8(1, 8, 9, 10, 11, 12, 13, 14, 15, CTL)                             \
+                SWAP_FMAC_DPP8(2, 16, 17, 18, 19, 20, 21, 22, 23, CTL)                       \
+                    SWAP_FMAC_DPP8(3, 24, 25, 26, 27, 28, 29, 30, 31, CTL)                   \
+                        SWAP_FMAC_DPP(32, CTL)                                               \
+        : "+&v"(a[0][0]), "+&v"(a[0][1]), "+&v"(a[0][2]), "+&v"(a[0][3]), "+&v"(a[1][0]),    \
+          "+&v"(a[1][1]), "+&v"(a[1][2]), "+&v"(a[1][3]), "+&v"(a[2][0]), "+&v"(a[2][1]),    \
+          "+&v"(a[2][2]), "+&v"(a[2][3]), "+&v"(a[3][0]), "+&v"(a[3][1]), "+&v"(a[3][2]),    \
+          "+&v"(a[3][3]), "+&v"(a[4][0]), "+&v"(a[4][1]), "+&v"(a[4][2]), "+&v"(a[4][3]),    \
+          "+&v"(a[5][0]), "+&v"(a[5][1]), "+&v"(a[5][2]), "+&v"(a[5][3]), "+&v"(a[6][0]),    \
+          "+&v"(a[6][1]), "+&v"(a[6][2]), "+&v"(a[6][3]), "+&v"(a[7][0]), "+&v"(a[7][1]),    \
+          "+&v"(a[7][2]), "+&v"(a[7][3]), "+&v"(mag)                                         \
+        : [f] "v"(f));                                                                       \
+  }                                                                                          \
+  __device__ __forceinline__ void NAME(float (&a)[8][3], float& mag, float f) {              \
+    asm("s_nop 4\n" SWAP_FMAC_DPP8(0, 0, 1, 2, 3, 4, 5, 6, 7, CTL)                          \
+            SWAP_FMAC_DPP8(1, 8, 9, 10, 11, 12, 13, 14, 15, CTL)                             \
+                SWAP_FMAC_DPP8(2, 16, 17, 18, 19, 20, 21, 22, 23, CTL)                       \
+                    SWAP_FMAC_DPP(24, CTL)                                                   \
+        : "+&v"(a[0][0]), "+&v"(a[0][1]), "+&v"(a[0][2]), "+&v"(a[1][0]), "+&v"(a[1][1]),    \
+          "+&v"(a[1][2]), "+&v"(a[2][0]), "+&v"(a[2][1]), "+&v"(a[2][2]), "+&v"(a[3][0]),    \
+          "+&v"(a[3][1]), "+&v"(a[3][2]), "+&v"(a[4][0]), "+&v"(a[4][1]), "+&v"(a[4][2]),    \
+          "+&v"(a[5][0]), "+&v"(a[5][1]), "+&v"(a[5][2]), "+&v"(a[6][0]), "+&v"(a[6][1]),    \
+          "+&v"(a[6][2]), "+&v"(a[7][0]), "+&v"(a[7][1]), "+&v"(a[7][2]), "+&v"(mag)         \
+        : [f] "v"(f));                                                                       \
+  }
+SWAP_SEG_STEP_ASM(seg_shr1, "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+SWAP_SEG_STEP_ASM(seg_shr2, "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+SWAP_SEG_STEP_ASM(seg_shr4, "row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+SWAP_SEG_STEP_ASM(seg_shr8, "row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+SWAP_SEG_STEP_ASM(seg_bc15, "row_bcast:15 row_mask:0xa bank_mask:0xf bound_ctrl:1")
+SWAP_SEG_STEP_ASM(seg_bc31, "row_bcast:31 row_mask:0xc bank_mask:0xf bound_ctrl:1")
+#undef SWAP_SEG_STEP_ASM
+#undef SWAP_FMAC_DPP8
+#undef SWAP_FMAC_DPP
+// FP64: the neighbour's value where the condition holds
+template <int CTRL, int ROWS, int M>
+__device__ __forceinline__ void seg_step(double (&a)[8][M], float& mag, bool cond) {
 #pragma unroll
   for (int cc = 0; cc < 8; cc++)
 #pragma unroll
     for (int m = 0; m < M; m++) {
-      T t = dppf<CTRL, ROWS>(a[cc][m]);
+      double t = dppf<CTRL, ROWS>(a[cc][m]);
       a[cc][m] = cond ? a[cc][m] + t : a[cc][m];
     }
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step1(float& a, bool cond) {
-  const float t = dppf<CTRL, ROWS>(a);
-  a = cond ? a + t : a;
+  const float t = dppf<CTRL, ROWS>(mag);
+  mag = cond ? mag + t : mag;
 }
 // the run sums a and their magnitude bound mag
-template <typename T, int M>
-__device__ __forceinline__ void segscan(T (&a)[8][M], float& mag, const SegCond& c) {
-  if (c.lv & 1u) { seg_step<0x111, 0xf>(a, c.c1); seg_step1<0x111, 0xf>(mag, c.c1); }
-  if (c.lv & 2u) { seg_step<0x112, 0xf>(a, c.c2); seg_step1<0x112, 0xf>(mag, c.c2); }
-  if (c.lv & 4u) { seg_step<0x114, 0xf>(a, c.c4); seg_step1<0x114, 0xf>(mag, c.c4); }
-  if (c.lv & 8u) { seg_step<0x118, 0xf>(a, c.c8); seg_step1<0x118, 0xf>(mag, c.c8); }
-  if (c.lv & 16u) { seg_step<0x142, 0xa>(a, c.c15); seg_step1<0x142, 0xa>(mag, c.c15); }
-  if (c.lv & 32u) { seg_step<0x143, 0xc>(a, c.c31); seg_step1<0x143, 0xc>(mag, c.c31); }
+template <int M>
+__device__ __forceinline__ void segscan(float (&a)[8][M], float& mag, const SegCond& c) {
+  if (c.lv & 1u) seg_shr1(a, mag, c.c1 ? 1.f : 0.f);
+  if (c.lv & 2u) seg_shr2(a, mag, c.c2 ? 1.f : 0.f);
+  if (c.lv & 4u) seg_shr4(a, mag, c.c4 ? 1.f : 0.f);
+  if (c.lv & 8u) seg_shr8(a, mag, c.c8 ? 1.f : 0.f);
+  if (c.lv & 16u) seg_bc15(a, mag, c.c15 ? 1.f : 0.f);
+  if (c.lv & 32u) seg_bc31(a, mag, c.c31 ? 1.f : 0.f);
+}
+template <int M>
+__device__ __forceinline__ void segscan(double (&a)[8][M], float& mag, const SegCond& c) {
+  if (c.lv & 1u) seg_step<0x111, 0xf>(a, mag, c.c1);
+  if (c.lv & 2u) seg_step<0x112, 0xf>(a, mag, c.c2);
+  if (c.lv & 4u) seg_step<0x114, 0xf>(a, mag, c.c4);
+  if (c.lv & 8u) seg_step<0x118, 0xf>(a, mag, c.c8);
+  if (c.lv & 16u) seg_step<0x142, 0xa>(a, mag, c.c15);
+  if (c.lv & 32u) seg_step<0x143, 0xc>(a, mag, c.c31);
 }
 
 __device__ __forceinline__ int wave_min(int v) {
