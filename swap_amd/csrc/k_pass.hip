@@ -397,6 +397,21 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
       W[a]--;
     }
   }
+  if (windowed) {
+    // a box itself over the LDS budget (the thermal spread of several steps
+    // since the sort) is clipped: its widest axis gives up one cell at a
+    // time, at alternate ends, so that the dense core keeps the window and
+    // the particles of the outer cells deposit straight to HBM (misses)
+    int flip[3] = {0, 0, 0};
+    for (int it = 0; it < 96 && (int64_t)W[0] * W[1] * W[2] * M > WIN_VALUES; it++) {
+      int a = W[1] > W[0] ? 1 : 0;
+      if (W[2] > W[a]) a = 2;
+      if (W[a] <= 2) break;
+      if (flip[a]) u0[a]++;
+      W[a]--;
+      flip[a] ^= 1;
+    }
+  }
   int64_t Wn64 = (int64_t)W[0] * W[1] * W[2];
   if (windowed && (W[0] < 2 || W[1] < 2 || W[2] < 2 || Wn64 * M > WIN_VALUES)) windowed = false;
   const int Wn = windowed ? (int)Wn64 : 0;
@@ -678,8 +693,8 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
     if (w < nitems) {
       if (tid == 0 && sred[0] <= sred[3] && sred[1] <= sred[4] && sred[2] <= sred[5] &&
           (int64_t)(sred[3] - sred[0] + 2) * (sred[4] - sred[1] + 2) * (sred[5] - sred[2] + 2) *
-                  4 > WIN_VALUES)
-        atomicAdd(dcnt + 4, 1ull);  // next step's P1 window would not fit
+                  4 > BOX_SORT_VALUES)
+        atomicAdd(dcnt + 4, 1ull);  // next step's P1 window would lose too much
       int* tn = vb_next + 6 * (tile + 1);
       if (tid < 3) {
         atomicMin(boxes_next + 6 * w + tid, sred[tid]);

@@ -1,30 +1,34 @@
-// k_sort.hip -- stable counting sort of particles by (tile, cell in tile).
+// k_sort.hip -- stable sort of particles by tile, cells in order within
+// each source chunk.
 //
 // Build-only component (the reference keeps particles in load order).  The
-// key of a particle is tile * TILE_CELLS + the cell's linear index inside its
-// tile, so a work item (a run of one tile's particles) touches a compact
-// block of nodes.
+// tiles are blocks of TILE_X x TILE_Y x TILE_Z cells; a work item (a run of
+// one tile's particles) touches a compact block of nodes.
 //
 // The sort is two-level.  The particles are cut into chunks of SCH
-// consecutive indices; a chunk's keys are first sorted stably inside LDS
-// (a 4-bit LSD radix sort), which turns the chunk into segments, one per
-// distinct key.  Globally only segments are counted:
-//   A: per chunk, LDS sort; record each segment (key, start, length), count
-//      particles and segments per key (two atomics per segment) and store
+// consecutive indices; a chunk's fine keys (tile * TILE_CELLS + the cell's
+// linear index inside its tile) are first sorted stably inside LDS (a 4-bit
+// LSD radix sort), which turns the chunk into segments, one per distinct
+// tile, each in cell order.  Globally only segments are counted:
+//   A: per chunk, LDS sort; record each segment (tile, start, length), count
+//      particles and segments per tile (two atomics per segment) and store
 //      each sorted position's source index and segment;
 //   scans of both counts;
-//   B: write each segment (chunk position, length) into its key's list;
-//   C: per key, order its segments by chunk and turn lengths into
+//   B: write each segment (chunk position, length) into its tile's list;
+//   C: per tile, order its segments by chunk and turn lengths into
 //      destination offsets;
 //   D: per chunk, walk the sorted positions: every particle is written
 //      once, with all its arrays, to its segment's offset + its rank, so
 //      the writes of a segment are contiguous.
-// A chunk holds at most one segment per key, so ordering a key's segments
-// by chunk and keeping the LDS sort stable gives exactly the stable order,
-// independent of the order in which atomics were served.  Particles drift
-// several cells per step, which breaks runs of equal keys within a
-// wavefront into pieces of one or two; grouping by chunk keeps the number
-// of global atomics and of scattered writes at the number of segments.
+// The result is the stable order of the key (tile, source chunk, cell):
+// a chunk holds at most one segment per tile, so ordering a tile's segments
+// by chunk gives it independent of the order in which atomics were served.
+// A tile's particles are thus a few runs in cell order (one per chunk that
+// held some), which keeps neighbours in a cell together for the deposit,
+// while a segment is a chunk's whole share of a tile: the particles drift
+// with the flow and several cells between sorts, so segments by cell would
+// hold 10-20 particles (partial cache lines at every segment's ends, written
+// by different workgroups) where segments by tile hold hundreds.
 #include "swap_internal.h"
 
 namespace {
@@ -62,29 +66,18 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum,
   return r;
 }
 
-__device__ __forceinline__ uint32_t tile_key(uint32_t c, int ntx, int nty) {
-  int i = cell_i(c), j = cell_j(c), k = cell_k(c);
-  uint32_t tile = (uint32_t)(i / TILE_X) + (uint32_t)ntx * ((uint32_t)(j / TILE_Y) +
-                                                            (uint32_t)nty * (uint32_t)(k / TILE_Z));
-  return tile * TILE_CELLS + (uint32_t)(i % TILE_X) +
-         TILE_X * ((uint32_t)(j % TILE_Y) + TILE_Y * (uint32_t)(k % TILE_Z));
-}
-
-// the key of a particle that left through an open x face is one past the
-// last tile key: it sorts behind every live particle
-__device__ __forceinline__ uint32_t sort_key(uint32_t c, int ntx, int nty, uint32_t dead_key) {
-  return c == DEAD_CELL ? dead_key : tile_key(c, ntx, nty);
-}
-
-// A: one workgroup per chunk.  The radix sort runs on key - (the chunk's
-// smallest key) over only as many bits as the chunk's key range needs: a
-// chunk is one z layer of a tile, whose particles stay within a tile or two
-// between sorts, so about 9 of the up to 32 key bits (3 passes instead of 6;
-// a chunk at the periodic x seam spans the grid and takes every pass).
-// Padding positions get the largest key of that width and, the sort being
-// stable, end up last.
+// A: one workgroup per chunk.  The radix sort runs on a chunk-relative key
+// that keeps the order of (tile, cell in tile): the tile's x, y and z
+// offsets from the chunk's smallest, each in as many bits as the chunk's
+// range of that coordinate needs, above the cell's 8-bit index in its tile.
+// A chunk holds particles of a tile or two in each direction, so the key
+// has about 14 bits (4 passes) where the absolute key would need up to 22 (a
+// chunk at a periodic seam spans the grid and takes more).  Ions that left
+// through an open x sort behind all live ones (the relative tile one past
+// the largest), padding positions last.
+__device__ __forceinline__ int nbits_of(uint32_t v) { return v ? 32 - __clz((int)v) : 0; }
 __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restrict__ cell,
-                                              int ntx, int nty, uint32_t dead_key, int nbits,
+                                              int ntx, int nty, uint32_t dead_tile,
                                               uint32_t* cnt,
                                               uint32_t* nrun, uint32_t* segkey,
                                               uint32_t* seginfo, uint32_t* nsegc,
@@ -93,56 +86,78 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
   __shared__ uint16_t sidx[SCH];
   __shared__ uint32_t scan[SCNT + SCNT / 16];
   __shared__ uint32_t wsum[ST / 64 + 1];
+  __shared__ int red[6][ST / 64];
   const int t = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * SCH;
   const int m = (int)min((int64_t)SCH, n - c0);
-  const uint32_t padkey = nbits >= 32 ? 0xFFFFFFFFu : ((1u << nbits) - 1u);
-  __shared__ uint32_t red[2][ST / 64];
-  uint32_t key[SE], idx[SE];
+  uint32_t cs[SE], key[SE], idx[SE];
   if (t * SE + SE <= m) {
     const uint4* src = (const uint4*)(cell + c0) + 2 * t;  // c0 is a multiple of 4096
     uint4 a = src[0], b = src[1];
-    uint32_t cs[SE] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-    for (int e = 0; e < SE; e++) key[e] = sort_key(cs[e], ntx, nty, dead_key);
+    cs[0] = a.x; cs[1] = a.y; cs[2] = a.z; cs[3] = a.w;
+    cs[4] = b.x; cs[5] = b.y; cs[6] = b.z; cs[7] = b.w;
   } else {
 #pragma unroll
     for (int e = 0; e < SE; e++) {
       int l = t * SE + e;
-      key[e] = l < m ? sort_key(cell[c0 + l], ntx, nty, dead_key) : padkey;
+      cs[e] = l < m ? cell[c0 + l] : DEAD_CELL;
     }
   }
 #pragma unroll
   for (int e = 0; e < SE; e++) idx[e] = (uint32_t)(t * SE + e);
 
-  // the chunk's key range (valid positions only) and the bits it needs
-  uint32_t lo = 0xFFFFFFFFu, hi = 0;
+  // the chunk's range of tile coordinates (live particles only)
+  int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, mx[3] = {-1, -1, -1};
 #pragma unroll
   for (int e = 0; e < SE; e++)
-    if (t * SE + e < m) {
-      lo = min(lo, key[e]);
-      hi = max(hi, key[e]);
-    }
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
-    hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
-  }
-  if ((t & 63) == 0) {
-    red[0][t >> 6] = lo;
-    red[1][t >> 6] = hi;
-  }
-  __syncthreads();
-  lo = red[0][0];
-  hi = red[1][0];
-  for (int q = 1; q < ST / 64; q++) {
-    lo = min(lo, red[0][q]);
-    hi = max(hi, red[1][q]);
-  }
-  int nbl = 0;
-  while (nbl < nbits && ((hi - lo) >> nbl) != 0u) nbl++;
-  const uint32_t padrel = nbl >= 32 ? 0xFFFFFFFFu : ((1u << nbl) - 1u);
+    if (cs[e] != DEAD_CELL) {
+      const int tc[3] = {cell_i(cs[e]) / TILE_X, cell_j(cs[e]) / TILE_Y, cell_k(cs[e]) / TILE_Z};
 #pragma unroll
-  for (int e = 0; e < SE; e++) key[e] = t * SE + e < m ? key[e] - lo : padrel;
+      for (int a = 0; a < 3; a++) {
+        mn[a] = min(mn[a], tc[a]);
+        mx[a] = max(mx[a], tc[a]);
+      }
+    }
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    for (int o = 32; o > 0; o >>= 1) {
+      mn[a] = min(mn[a], __shfl_xor(mn[a], o));
+      mx[a] = max(mx[a], __shfl_xor(mx[a], o));
+    }
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      red[a][t >> 6] = mn[a];
+      red[3 + a][t >> 6] = mx[a];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    mn[a] = red[a][0];
+    mx[a] = red[3 + a][0];
+    for (int q = 1; q < ST / 64; q++) {
+      mn[a] = min(mn[a], red[a][q]);
+      mx[a] = max(mx[a], red[3 + a][q]);
+    }
+    if (mx[a] < mn[a]) mn[a] = mx[a] = 0;  // no live particle
+  }
+  const int bx = nbits_of((uint32_t)(mx[0] - mn[0])), by = nbits_of((uint32_t)(mx[1] - mn[1]));
+  // the largest relative tile any live particle can have, the dead ions'
+  // key one tile past it, padding after them
+  const uint32_t tmax = ((((uint32_t)(mx[2] - mn[2]) << by) | (uint32_t)(mx[1] - mn[1])) << bx) |
+                        (uint32_t)(mx[0] - mn[0]);
+  const uint32_t deadrel = (tmax + 1u) << 8, padrel = deadrel | 255u;
+  const int nbl = nbits_of(padrel);
+#pragma unroll
+  for (int e = 0; e < SE; e++) {
+    const uint32_t c = cs[e];
+    const int i = cell_i(c), j = cell_j(c), k = cell_k(c);
+    const uint32_t tr = ((((uint32_t)(k / TILE_Z - mn[2]) << by) | (uint32_t)(j / TILE_Y - mn[1]))
+                         << bx) | (uint32_t)(i / TILE_X - mn[0]);
+    const uint32_t local =
+        (uint32_t)(i % TILE_X + TILE_X * (j % TILE_Y + TILE_Y * (k % TILE_Z)));
+    key[e] = t * SE + e >= m ? padrel : c == DEAD_CELL ? deadrel : (tr << 8) | local;
+  }
 
   // stable LSD radix sort, 4 bits per pass; thread t holds positions
   // t*SE .. t*SE+SE-1 before and after every pass
@@ -201,13 +216,14 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
   }
   __syncthreads();
 
-  // segments: a head is a valid position whose key differs from the previous
+  // segments: a head is a valid position whose tile differs from the
+  // previous one's (the tile of a fine key k is k / TILE_CELLS)
   bool hd[SE];
   uint32_t heads = 0;
 #pragma unroll
   for (int e = 0; e < SE; e++) {
     int p = t * SE + e;
-    hd[e] = p < m && (p == 0 || skey[p - 1] != key[e]);
+    hd[e] = p < m && (p == 0 || (skey[p - 1] >> 8) != (key[e] >> 8));
     heads += hd[e] ? 1u : 0u;
   }
   uint32_t nseg;
@@ -222,7 +238,15 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
   __syncthreads();
   for (uint32_t q = t; q < nseg; q += ST) {
     uint32_t st = scan[q], len = scan[q + 1] - st;
-    uint32_t k = skey[st] + lo;  // back to the absolute key
+    // the segment's tile
+    const uint32_t tr = skey[st] >> 8;
+    uint32_t k = dead_tile;
+    if (tr <= tmax) {
+      const uint32_t tx = (uint32_t)mn[0] + (tr & ((1u << bx) - 1u)),
+                     ty = (uint32_t)mn[1] + ((tr >> bx) & ((1u << by) - 1u)),
+                     tz = (uint32_t)mn[2] + (tr >> (bx + by));
+      k = tx + (uint32_t)ntx * (ty + (uint32_t)nty * tz);
+    }
     segkey[c0 + q] = k;
     seginfo[c0 + q] = st | (len << 12);
     atomicAdd(cnt + k, len);
@@ -238,7 +262,7 @@ __global__ __launch_bounds__(ST) void k_sortA(int64_t n, const uint32_t* __restr
   if (t == 0) nsegc[blockIdx.x] = nseg;
 }
 
-// B: each segment into its key's list
+// B: each segment into its tile's list
 __global__ void k_sortB(const uint32_t* nsegc, const uint32_t* segkey, const uint32_t* seginfo,
                         const uint32_t* rstart, uint32_t* rcur, uint2* runs) {
   const int64_t c0 = (int64_t)blockIdx.x * SCH;
@@ -250,8 +274,8 @@ __global__ void k_sortB(const uint32_t* nsegc, const uint32_t* segkey, const uin
   }
 }
 
-// C: one thread per segment record: its destination is the key's start plus
-// the lengths of the key's segments from earlier chunks
+// C: one thread per segment record: its destination is the tile's start
+// plus the lengths of the tile's segments from earlier chunks
 __global__ void k_sortC(const uint32_t* segkey, const uint32_t* kstart, const uint32_t* rstart,
                         int64_t nkeys, const uint2* runs, uint32_t* segbase) {
   const uint32_t nruns = rstart[nkeys];
@@ -383,21 +407,20 @@ __global__ void k_scan_add(int64_t n, uint32_t* out, const uint32_t* bsum) {
 }
 
 // work items: each tile's run is cut into pieces of at most ITEM_MAX; the
-// box of an item is the tile's x-y extent and the z layers of its first and
-// last particle (sorted cells are in tile-local linear order)
+// box of an item is its tile's extent
 __global__ void k_items_count(int64_t ntiles, const uint32_t* start, uint32_t* nit) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  uint32_t a = start[t * TILE_CELLS], b = start[(t + 1) * TILE_CELLS];
+  uint32_t a = start[t], b = start[t + 1];
   nit[t] = (b - a + ITEM_MAX - 1) / ITEM_MAX;
 }
 __global__ void k_items_fill(int64_t ntiles, const uint32_t* start, const uint32_t* nit,
-                             const uint32_t* ioff, const uint32_t* cell_sorted, int ntx, int nty,
-                             int ncx, int ncy, Item* items, int* boxes) {
+                             const uint32_t* ioff, int ntx, int nty, int ncx, int ncy, int ncz,
+                             Item* items, int* boxes) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  uint32_t a = start[t * TILE_CELLS], b = start[(t + 1) * TILE_CELLS];
-  int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty);
+  uint32_t a = start[t], b = start[t + 1];
+  int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), tz = (int)(t / ((int64_t)ntx * nty));
   for (uint32_t q = 0; q < nit[t]; q++) {
     uint32_t s = a + q * ITEM_MAX, e = min(b, s + ITEM_MAX);
     uint32_t w = ioff[t] + q;
@@ -408,10 +431,10 @@ __global__ void k_items_fill(int64_t ntiles, const uint32_t* start, const uint32
     int* bx = boxes + 6 * w;
     bx[0] = tx * TILE_X;
     bx[1] = ty * TILE_Y;
-    bx[2] = cell_k(cell_sorted[s]);
+    bx[2] = tz * TILE_Z;
     bx[3] = min((tx + 1) * TILE_X, ncx) - 1;
     bx[4] = min((ty + 1) * TILE_Y, ncy) - 1;
-    bx[5] = cell_k(cell_sorted[e - 1]);
+    bx[5] = min((tz + 1) * TILE_Z, ncz) - 1;
   }
 }
 
@@ -440,20 +463,18 @@ int so_scan_u32(swap_ctx* c, int64_t n, const uint32_t* in, uint32_t* out, uint3
 
 int so_sort(swap_ctx* c) {
   int64_t n = c->n;
-  const int64_t nk = c->nkeys;
-  // an open x run has one more key, nk, for the particles that left; they
-  // end up behind the live ones, which are the first sstart[nk]
-  const int64_t nkd = nk + (c->g.ox ? 1 : 0);
+  const int64_t nk = c->nkeys, nt = c->ntiles;
+  // an open x run has one more tile, nt, for the particles that left; they
+  // end up behind the live ones, which are the first sstart[nt]
+  const int64_t nkd = nt + (c->g.ox ? 1 : 0);
   const int ntx = c->ntile[0], nty = c->ntile[1];
   if (n > 0) {
     const int64_t nch = (n + SCH - 1) / SCH;
-    int nbits = 0;
-    while (nbits < 32 && (1ull << nbits) < (unsigned long long)nkd) nbits++;
     HIPCHK(c, hipMemsetAsync(c->scnt, 0, sizeof(uint32_t) * nkd, c->stream));
     HIPCHK(c, hipMemsetAsync(c->snrun, 0, sizeof(uint32_t) * nkd, c->stream));
     hipLaunchKernelGGL(k_sortA, dim3((unsigned)nch), dim3(ST), 0, c->stream, n, c->cell, ntx,
-                       nty, (uint32_t)nk, nbits, c->scnt, c->snrun, c->segkey, c->seginfo,
-                       c->nsegc, c->srbase);
+                       nty, (uint32_t)nt, c->scnt, c->snrun, c->segkey, c->seginfo, c->nsegc,
+                       c->srbase);
     if (int e = post(c)) return e;
     if (int e = scan(c, nkd, c->scnt, c->sstart)) return e;
     if (int e = scan(c, nkd, c->snrun, c->srstart)) return e;
@@ -481,7 +502,7 @@ int so_sort(swap_ctx* c) {
     std::swap(c->id, c->id2);
     if (c->g.ox) {
       uint32_t live = 0;
-      HIPCHK(c, hipMemcpyAsync(&live, c->sstart + nk, 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(&live, c->sstart + nt, 4, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
       c->escaped += n - (int64_t)live;
       c->dead_pending = false;
@@ -490,7 +511,7 @@ int so_sort(swap_ctx* c) {
       c->n = n = live;
     }
   } else {
-    HIPCHK(c, hipMemsetAsync(c->sstart, 0, sizeof(uint32_t) * (nk + 1), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sstart, 0, sizeof(uint32_t) * (nt + 1), c->stream));
   }
   // work items and their cell boxes; the item count stays on the device
   uint32_t* nit = c->scur;  // reuse: ntiles + 1 entries each
@@ -499,8 +520,8 @@ int so_sort(swap_ctx* c) {
                      c->ntiles, c->sstart, nit);
   if (int e = scan(c, c->ntiles, nit, ioff)) return e;
   hipLaunchKernelGGL(k_items_fill, dim3(nblk(c->ntiles, 256)), dim3(256), 0, c->stream, c->ntiles,
-                     c->sstart, nit, ioff, c->cell, ntx, nty, c->cfg.ncx, c->cfg.ncy, c->items,
-                     c->boxes);
+                     c->sstart, nit, ioff, ntx, nty, c->cfg.ncx, c->cfg.ncy, c->cfg.ncz,
+                     c->items, c->boxes);
   HIPCHK(c, hipMemcpyAsync(c->d_nitems, ioff + c->ntiles, sizeof(int), hipMemcpyDeviceToDevice,
                            c->stream));
   if (int e = post(c)) return e;

@@ -217,6 +217,20 @@ int swap_create(const swap_config* cfg, swap_ctx** out) {
     delete c;
     return SWAP_EINVAL;
   }
+  {
+    // the sort's chunk-relative key holds the tile offsets of a chunk that
+    // spans the grid in 24 bits above the cell index (with one more tile for
+    // the ions that left an open x): at most 23 bits of tile coordinates
+    auto bits = [](int v) { return v > 0 ? 32 - __builtin_clz((unsigned)v) : 0; };
+    if (cfg->ncx >= 2 && cfg->ncy >= 2 && cfg->ncz >= 2 &&
+        bits((cfg->ncx + TILE_X - 1) / TILE_X - 1) + bits((cfg->ncy + TILE_Y - 1) / TILE_Y - 1) +
+                bits((cfg->ncz + TILE_Z - 1) / TILE_Z - 1) > 23) {
+      fprintf(stderr, "swap_create: grid too large for the sort keys (at most 23 bits of "
+                      "tile coordinates, e.g. 2046 x 1022 x 1024 cells)\n");
+      delete c;
+      return SWAP_EINVAL;
+    }
+  }
   if ((cfg->flags & SWAP_OUTFLOW_X) && (cfg->ncx < 4 || cfg->sort_every < 1)) {
     fprintf(stderr, "swap_create: an open x needs ncx >= 4 and sort_every >= 1 "
                     "(the sort removes escaped ions)\n");

@@ -4,9 +4,10 @@ import numpy as np
 import oracle as O
 from swap_amd import physics
 
-# tile key of a packed cell, the sort order of swap_amd/csrc/k_sort.hip
-# (TILE_X, TILE_Y, TILE_Z of swap_amd/csrc/swap_internal.h)
+# tiles of the sort in swap_amd/csrc/k_sort.hip (TILE_X, TILE_Y, TILE_Z of
+# swap_amd/csrc/swap_internal.h) and its chunk of consecutive particles
 TILE = (4, 8, 8)
+SORT_CHUNK = 4096
 
 
 def unpack(cell):
@@ -15,15 +16,30 @@ def unpack(cell):
         (cell >> 21).astype(np.int64)
 
 
-def tile_keys(cell, nc):
+def tiles(cell, nc):
+    """tile index and cell index inside the tile of packed cells, and the
+    number of tiles"""
     i, j, k = unpack(cell)
     ntx = (nc[0] + TILE[0] - 1) // TILE[0]
     nty = (nc[1] + TILE[1] - 1) // TILE[1]
     ntz = (nc[2] + TILE[2] - 1) // TILE[2]
     tile = i // TILE[0] + ntx * (j // TILE[1] + nty * (k // TILE[2]))
     local = i % TILE[0] + TILE[0] * (j % TILE[1] + TILE[1] * (k % TILE[2]))
-    return (tile * TILE[0] * TILE[1] * TILE[2] + local).astype(np.uint32), \
-        ntx * nty * ntz * TILE[0] * TILE[1] * TILE[2]
+    return tile, local, ntx * nty * ntz
+
+
+def sort_perm(cell, nc):
+    """the permutation of the particle sort: stable by (tile, chunk of
+    SORT_CHUNK consecutive source positions, cell in tile), through the
+    oracle's stable counting sort of that composite key"""
+    tile, local, nt = tiles(cell, nc)
+    nch = (len(cell) + SORT_CHUNK - 1) // SORT_CHUNK
+    cells = TILE[0] * TILE[1] * TILE[2]
+    chunk = np.arange(len(cell), dtype=np.int64) // SORT_CHUNK
+    key = (tile * nch + chunk) * cells + local
+    nkeys = nt * nch * cells
+    assert nkeys < 2 ** 32
+    return O.stable_sort(key.astype(np.uint32), nkeys)
 
 
 def rel(a, b):

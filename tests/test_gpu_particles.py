@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import Pair, rel, tile_keys, unpack
+from helpers import Pair, rel, sort_perm, unpack
 from swap_amd import api
 
 pytestmark = pytest.mark.gpu
@@ -159,17 +159,30 @@ def test_split_move_is_bit_exact():
 
 
 def test_sort_permutation_is_stable_and_bit_exact():
+    """the sort's permutation is the stable order of (tile, source chunk,
+    cell in tile), in the random order of a load and again after a move
+    (6 chunks of 4096 here)"""
     pr = Pair(ncx=32, ncy=16, ncz=8, ppc=6)
     s, p = pr.s, pr.p
+    nc = (p["ncx"], p["ncy"], p["ncz"])
     cell = s.particle_array(api.P_CELL)
-    keys, nkeys = tile_keys(cell, (p["ncx"], p["ncy"], p["ncz"]))
-    perm = O.stable_sort(keys, nkeys)
+    perm = sort_perm(cell, nc)
     s.sort()
     assert np.array_equal(s.particle_array(api.P_ID), perm)
     assert np.array_equal(s.particle_array(api.P_CELL), cell[perm])
-    # sorting again is the identity
+    # sorting the sorted arrays again reorders only within tiles (a tile's
+    # runs from several source chunks that now share a chunk merge)
+    perm2 = sort_perm(cell[perm], nc)
     s.sort()
-    assert np.array_equal(s.particle_array(api.P_ID), perm)
+    assert np.array_equal(s.particle_array(api.P_ID), perm[perm2])
+    # after a half move
+    s.move_ion_half()
+    ids = s.particle_array(api.P_ID)
+    cell = s.particle_array(api.P_CELL)
+    perm3 = sort_perm(cell, nc)
+    s.sort()
+    assert np.array_equal(s.particle_array(api.P_ID), ids[perm3])
+    assert np.array_equal(s.particle_array(api.P_CELL), cell[perm3])
 
 
 def test_fixed_point_deposit_accuracy():
