@@ -8,33 +8,20 @@
 // (gutsp.f:1704-1756) is done once per node after the sums.
 //
 // One workgroup runs one work item: a run of one tile's particles after the
-// counting sort, processed in chunks of PASS_WG * PASS_KP particles.  Lane t
-// of a chunk takes PASS_KP consecutive particles and walks them in order,
-// summing each particle's 8 corners x M moments in registers while the
-// deposit cell stays the same.  A lane keeps the sums of its first cell
-// aside when it moves on, adds the sums of any middle cell into the LDS
-// window at once, and at the end of the chunk the sums of its last cell go
-// through one segmented scan across the wavefront (runs of lanes in the same
-// cell, which is what sorted particles give) so that only a run's last lane
-// adds them.  Window and HBM additions are exact fixed-point integer atomics
-// (FP64 in the FP64 build).
-// After a move (no sort since) the cells change from particle to particle
-// and the additions degrade to one per particle.  The window covers the
-// item's deposit footprint and is added to HBM once per item.  The window of P1/P2
-// is the item's cell box at x^n; P3's is that box widened by the half-step
-// displacement range from signed per-axis velocity bounds of the item's tile
-// (reduced by the previous P3).  Particles outside the window deposit
-// straight to HBM (counted as misses).
+// counting sort, processed in chunks of PASS_WG particles, one per lane.
+// Each lane forms its particle's 8 corners x M moments; runs of lanes whose
+// particles share a cell (the sort leaves each tile's particles in a few
+// runs in cell order) are summed across the wavefront by a segmented scan,
+// so that only a run's last lane adds into the LDS window.  Window and HBM
+// additions are exact fixed-point integer atomics (FP64 in the FP64 build).
+// The window covers the item's deposit footprint and is added to HBM once
+// per item.  The window of P1/P2 is the item's cell box at x^n; P3's is that
+// box widened by the half-step displacement range from signed per-axis
+// velocity bounds of the item's tile (reduced by the previous P3).
+// Particles outside the window deposit straight to HBM (counted as misses).
 //
-// P3 always takes one particle per lane: right after a sort its 4-per-lane
-// form (gather tile in LDS, one workgroup per CU) ran at 10.2 ms against
-// 7.6 ms for the one-per-lane form at two workgroups per CU, whose segmented
-// scan merges the sorted runs across lanes instead.
-//
-// Right after a sort, P2 gathers the 6-component node source
-// S = (ajc - upc, btc) from an LDS copy of the item's node box (one 32-byte
-// record per node), filled with coalesced loads when the item starts; on
-// later steps, and in P3, the gather reads S through the caches.
+// The gather of the 6-component node source S = (ajc - upc, btc) in P2 and
+// P3 reads one 32-byte record per corner through the caches.
 #include <type_traits>
 #include "particle_math.h"
 
@@ -243,6 +230,25 @@ __device__ __forceinline__ void segscan(double (&a)[8][M], float& mag, const Seg
   if (c.lv & 32u) seg_step<0x143, 0xc>(a, mag, c.c31);
 }
 
+// The sums a of each lane's cell `cur` (-1: nothing to add) into the window:
+// runs of consecutive lanes with the same cell are summed across the
+// wavefront first (sorted particles), and only each run's last lane adds
+// into the window; lanes that share no cell with a neighbour add directly.
+// Every lane of the wavefront calls it.
+template <int M, int WS, typename T, typename A>
+__device__ __forceinline__ void put_runs(T (&a)[8][M], float& mag, int cur, A* lacc, int sy,
+                                         int sz, int lane) {
+  const int prevkey = __shfl_up(cur, 1);
+  const unsigned long long hm = __ballot(lane == 0 || prevkey != cur);
+  if (hm != ~0ull) {
+    const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+    const SegCond sc = seg_cond(lane, 63 - __clzll(hm & le));
+    segscan(a, mag, sc);
+  }
+  const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
+  if (tail && cur >= 0) win_put<M, WS>(lacc, cur, sy, sz, a, mag);
+}
+
 __device__ __forceinline__ int wave_min(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
   return v;
@@ -261,47 +267,38 @@ __device__ __forceinline__ int xnode(const Geo& g, int u) {
   return OX ? (u < 1 ? 1 : (u > g.ncx ? g.ncx : u)) : node_of(u, g.ncx);
 }
 
-// KP particles per lane: PASS_KP in P1 and P2 right after a sort (cells in
-// runs), 1 otherwise and in P3 (the scan across the wavefront merges
-// neighbours that share a cell).  OX: open x.
-// Waves per SIMD each pass variant is compiled for (its register budget).
-// A 512-thread workgroup is 2 waves on each of a CU's 4 SIMDs, so occupancy
-// moves in whole workgroups: two of them (4 waves per SIMD) share a CU when
-// a wave needs at most 128 VGPRs and the LDS holds both 64 KB windows.  The
-// one-particle-per-lane FP32 P2 and P3 are held to that (93 and 126 VGPRs,
-// no spills; P1 fits unasked with 97): P3 at one workgroup per CU (165
-// VGPRs when its divergent exits kept the 32 sums live on every path) took
-// 9.8 ms against 7.7.  P1 right after a sort would spill 24 VGPRs at 128
-// (its HBM traffic doubled for 1 % of its time), and P2 right after a sort
-// gives the LDS of a second workgroup to its gather tile (without the tile:
-// 4.9 -> 5.7 ms); they and the FP64 build run one workgroup per CU with the
-// registers they need; 2 is the compiler's own default for 512 threads.
-template <typename R, int MODE, int KP>
+// One particle per lane in every pass.  Occupancy moves in whole 512-thread
+// workgroups (2 waves on each of a CU's 4 SIMDs): two of them (4 waves per
+// SIMD) share a CU when a wave needs at most 128 VGPRs and the LDS holds
+// both windows (2 x 81.6 KB of the CU's 160 KB).  The FP32 passes are held
+// to that (P3 needs 122 VGPRs with no spills); P3 at one workgroup per CU
+// (165 VGPRs when its divergent exits kept the 32 sums live on every path)
+// took 9.8 ms against 7.7.  The FP64 build runs one workgroup per CU with
+// the registers it needs (2 is the compiler's own default for 512 threads).
+// Measured losers: 4 consecutive particles per lane summed in registers
+// (P1/P2 on the step right after a sort; one workgroup per CU) once the
+// sort leaves cells in short runs, 5.1 / 5.2 ms against 4.8 / 4.3 ms for one
+// particle per lane; P3 depositing the next step's P1 moments at x^{n+1}
+// into a second window (1024-thread workgroups) instead of P1 reading the
+// particles again: P3 7.45 -> 12.5 ms against P1's 5.2 ms.
+template <typename R>
 struct PassWaves {
-  static constexpr int value = (sizeof(R) == 4 && KP == 1 && MODE >= 2) ? 4 : 2;
+  static constexpr int value = sizeof(R) == 4 ? 4 : 2;
 };
 
-template <typename R, int MODE, int KP, bool OX>
-__global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pass(Geo g, PDev<R> P, PCoef<R> K,
-                                                  const R* __restrict__ S,
-                                                  typename AccOf<R>::T* __restrict__ gacc,
-                                                  const Item* __restrict__ items,
-                                                  const int* __restrict__ d_nitems,
-                                                  int64_t n_sorted, int64_t n_total,
-                                                  const int* __restrict__ boxes, int* boxes_next,
-                                                  const int* __restrict__ vb, int* vb_next,
-                                                  unsigned long long* dcnt,
-                                                  const int* __restrict__ tboxes) {
+template <typename R, int MODE, bool OX>
+__global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
+    Geo g, PDev<R> P, PCoef<R> K, const R* __restrict__ S,
+    typename AccOf<R>::T* __restrict__ gacc, const Item* __restrict__ items,
+    const int* __restrict__ d_nitems, int64_t n_sorted, int64_t n_total,
+    const int* __restrict__ boxes, int* boxes_next, const int* __restrict__ vb, int* vb_next,
+    unsigned long long* dcnt, const int* __restrict__ tboxes) {
   constexpr int M = NMom<MODE>::value;
-  // the LDS gather tile pays off while the item's box is compact (right
-  // after a sort); drifted boxes gather from HBM/L2
-  constexpr bool LTILE = MODE != 1 && sizeof(R) == 4 && KP > 1;
   constexpr int GM0 = MODE == 2 ? 1 : 0;  // moment index of window moment 0
   constexpr int WS = WIN_VALUES / M;       // window values per moment (LDS stride)
   constexpr bool FIX = sizeof(R) == 4;     // fixed-point moment sums (FP32 build)
   using A = typename AccOf<R>::T;
   __shared__ A lacc[WIN_VALUES];
-  __shared__ float4 gtile[LTILE ? 2 * GTILE_NODES : 1];
   __shared__ int sred[12];
 
   const int tid = threadIdx.x;
@@ -339,7 +336,6 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
     }
     windowed = bh[0] >= bl[0] && bh[1] >= bl[1] && bh[2] >= bl[2];
   }
-  const bool boxed = windowed;
   int lo_m[3] = {0, 0, 0}, hi_m[3] = {0, 0, 0};
   float mid[3] = {0.f, 0.f, 0.f};  // centre of each axis's displacement range
   if (MODE == 3 && windowed) {
@@ -369,12 +365,11 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
       }
     }
   }
-  int u0[3], W[3], cen[3], T[3];
+  int u0[3], W[3], cen[3];
   for (int a = 0; a < 3; a++) {
     u0[a] = bl[a] + lo_m[a];
     W[a] = bh[a] + hi_m[a] - u0[a] + 2;
     cen[a] = (bl[a] + bh[a]) >> 1;
-    T[a] = bh[a] - bl[a] + 2;  // gather tile: nodes bl .. bh+1
   }
   if (MODE == 3 && windowed) {
     // a P3 window over the LDS budget (boxes grown by a few steps of drift)
@@ -415,263 +410,142 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
   int64_t Wn64 = (int64_t)W[0] * W[1] * W[2];
   if (windowed && (W[0] < 2 || W[1] < 2 || W[2] < 2 || Wn64 * M > WIN_VALUES)) windowed = false;
   const int Wn = windowed ? (int)Wn64 : 0;
-  const int Tn = T[0] * T[1] * T[2];
-  const bool tiled = LTILE && boxed && Tn <= GTILE_NODES;
-  if (windowed)
-    for (int q = tid; q < Wn * M; q += PASS_WG) lacc[q % Wn + (q / Wn) * WS] = (A)0;
-  if constexpr (LTILE) {
-    if (tiled)
-      for (int q = tid; q < Tn; q += PASS_WG) {
-        int a = q % T[0], b = (q / T[0]) % T[1], c = q / (T[0] * T[1]);
-        int64_t node = gidx(g, xnode<OX>(g, bl[0] + a), node_of(bl[1] + b, g.ncy),
-                            node_of(bl[2] + c, g.ncz));
-        gtile[2 * q] = *(const float4*)(S + node * SREC);
-        gtile[2 * q + 1] = *(const float4*)(S + node * SREC + 4);
-      }
-  }
+  for (int q = tid; q < Wn * M; q += PASS_WG) lacc[q % Wn + (q / Wn) * WS] = (A)0;
   if (MODE == 3 && tid < 12) sred[tid] = (tid % 6) < 3 ? 0x7fffffff : -0x7fffffff - 1;
   __syncthreads();
 
   const int sy = W[0], sz = W[0] * W[1];  // window strides of the y and z corners
-  const int toff[8] = {0, 1, T[0] * T[1], 1 + T[0] * T[1], T[0], 1 + T[0],
-                       T[0] + T[0] * T[1], 1 + T[0] + T[0] * T[1]};
   int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
   int bmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
   float vlo[3] = {3e38f, 3e38f, 3e38f}, vhi[3] = {-3e38f, -3e38f, -3e38f};
   unsigned long long misses = 0;
 
-  // P3 runs with one particle per lane (KP = 1) also right after a sort:
-  // consecutive lanes then write consecutive particles, so every store
-  // instruction covers whole cache lines
-  static_assert(MODE != 3 || KP == 1, "P3 takes one particle per lane");
-  for (int64_t chunk = i0; chunk < i1; chunk += (int64_t)PASS_WG * KP) {
-    const int64_t pb = chunk + (int64_t)tid * KP;
-    int cur = -1;   // window cell of the running sums acc
-    int hkey = -1;  // window cell of hd: the lane's first cell, once the lane left it
-    R acc[8][M], hd[8][M];
-    float amag = 0.f, hmag = 0.f;  // bounds of |acc|, |hd| in fixed-point units
+  // Consecutive lanes take consecutive particles, so every load and store
+  // instruction covers whole cache lines.  A chunk's particle reads are
+  // issued before the previous chunk's work, so that their latency overlaps
+  // it.  A lane past the item's end reads a loadable particle and, like a
+  // lane holding an ion that left, contributes zero moments: the 8 x M sums
+  // are plain products with no zeroing on divergent paths (each such path
+  // cost 32 moves per chunk); only the stores and the out-of-window deposit
+  // are guarded.
+  struct Pre {
+    uint32_t cell;
+    R f[3], v[3], v1[3];
+    float bp;
+  } nxt{};
+  auto fetch = [&](int64_t pb) {
+    const int64_t p = pb < i1 ? pb : i0;
+    nxt.cell = P.cell[p];
+    nxt.f[0] = P.fx[p]; nxt.f[1] = P.fy[p]; nxt.f[2] = P.fz[p];
+    nxt.v[0] = P.vx[p]; nxt.v[1] = P.vy[p]; nxt.v[2] = P.vz[p];
+    if (MODE == 1) {
+      nxt.v1[0] = P.v1x[p]; nxt.v1[1] = P.v1y[p]; nxt.v1[2] = P.v1z[p];
+    }
+    nxt.bp = P.bp ? P.bp[p] : 1.f;
+  };
+  if (i0 < i1) fetch(i0 + tid);
+  for (int64_t chunk = i0; chunk < i1; chunk += PASS_WG) {
+    const int64_t pb = chunk + tid;
+    const int64_t p = pb < i1 ? pb : i0;
+    const Pre cu = nxt;
+    if (chunk + PASS_WG < i1) fetch(pb + PASS_WG);
+    const bool valid = pb < i1 && cu.cell != DEAD_CELL;
+    const uint32_t c = valid ? cu.cell : 0u;
+    const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
+    R f0 = cu.f[0], f1 = cu.f[1], f2 = cu.f[2];
+    const R r = P.bp ? (R)1 / (R)cu.bp : (R)1;
+    R w8[8], q[M], vn[3] = {0, 0, 0};
+    weights8(f0, f1, f2, w8);
+    int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
+        uz = unwrap(ck, cen[2], g.ncz);
+    bool alive = valid;  // false once the particle left through an open x face
+    const R v[3] = {cu.v[0], cu.v[1], cu.v[2]};
+    if constexpr (MODE == 1) {
+      q[0] = r;
+      q[1] = r * ((R)1.5 * v[0] - (R)0.5 * cu.v1[0]);
+      q[2] = r * ((R)1.5 * v[1] - (R)0.5 * cu.v1[1]);
+      q[3] = r * ((R)1.5 * v[2] - (R)0.5 * cu.v1[2]);
+    } else {
+      R Ag[3], Bg[3], E[3], vm[3], vpl[3];
+      gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
+      const R mr = P.mr ? (R)P.mr[p] : (R)1;
+      ep_of(Ag, Bg, mr, E);
+      rotate(K.dt, v, E, Bg, mr, vm, vpl);
+      if constexpr (MODE == 2) {
 #pragma unroll
-    for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-      for (int m = 0; m < M; m++) {
-        acc[cc][m] = (R)0;
-        if (KP > 1) hd[cc][m] = (R)0;
-      }
-    if constexpr (KP == 1) {
-      // One particle per lane, computed by every lane: a lane past the
-      // item's end or holding an ion that left reads a loadable particle and
-      // contributes zero moments, so the 8 x M sums are plain products with
-      // no zeroing on divergent paths (each such path cost 32 moves per
-      // chunk); only the stores and the out-of-window deposit are guarded.
-      const int64_t p = pb < i1 ? pb : i0;
-      const uint32_t craw = P.cell[p];
-      const bool valid = pb < i1 && craw != DEAD_CELL;
-      const uint32_t c = valid ? craw : 0u;
-      const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
-      R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
-      const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
-      R w8[8], q[M], vn[3] = {0, 0, 0};
-      weights8(f0, f1, f2, w8);
-      int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
-          uz = unwrap(ck, cen[2], g.ncz);
-      bool alive = valid;  // false once the particle left through an open x face
-      const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
-      if constexpr (MODE == 1) {
-        q[0] = r;
-        q[1] = r * ((R)1.5 * v[0] - (R)0.5 * P.v1x[p]);
-        q[2] = r * ((R)1.5 * v[1] - (R)0.5 * P.v1y[p]);
-        q[3] = r * ((R)1.5 * v[2] - (R)0.5 * P.v1z[p]);
+        for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
       } else {
-        R Ag[3], Bg[3], E[3], vm[3], vpl[3];
-        gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
-        const R mr = P.mr ? (R)P.mr[p] : (R)1;
-        ep_of(Ag, Bg, mr, E);
-        rotate(K.dt, v, E, Bg, mr, vm, vpl);
-        if constexpr (MODE == 2) {
+        const R hdt = (R)0.5 * K.dt;
 #pragma unroll
-          for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
-        } else {
-          const R hdt = (R)0.5 * K.dt;
-#pragma unroll
-          for (int m = 0; m < 3; m++) {
-            vn[m] = vpl[m] + hdt * E[m];
-            const float fv = (float)vn[m];  // plain compares: no NaN quieting
-            vlo[m] = valid && fv < vlo[m] ? fv : vlo[m];
-            vhi[m] = valid && fv > vhi[m] ? fv : vhi[m];
-          }
-          if (valid) {
-            P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
-            P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
-          }
-          ux += half_move(f0, vn[0], K.s[0]);
-          uy += half_move(f1, vn[1], K.s[1]);
-          uz += half_move(f2, vn[2], K.s[2]);
-          if (OX) alive = alive && ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}
-          weights8(f0, f1, f2, w8);
-          q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
+        for (int m = 0; m < 3; m++) {
+          vn[m] = vpl[m] + hdt * E[m];
+          const float fv = (float)vn[m];  // plain compares: no NaN quieting
+          vlo[m] = valid && fv < vlo[m] ? fv : vlo[m];
+          vhi[m] = valid && fv > vhi[m] ? fv : vhi[m];
         }
-      }
-      float pmag = 0.f;
-      if constexpr (FIX) {
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-          q[m] *= (R)__int_as_float((127 + fix_bits(GM0 + m)) << 23);
-          pmag = fmaxf(pmag, fabsf((float)q[m]));
+        if (valid) {
+          P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
+          P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
         }
-      }
-      const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
-      const bool inw = alive && windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
-                       ly < W[1] - 1 && lz < W[2] - 1;
-      if (alive && !inw) {
-        misses++;
-#pragma unroll
-        for (int cc = 0; cc < 8; cc++) {
-          int64_t node = gidx(g, xnode<OX>(g, ux + (cc & 1)),
-                              node_of(uy + ((cc >> 2) & 1), g.ncy),
-                              node_of(uz + ((cc >> 1) & 1), g.ncz));
-#pragma unroll
-          for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
-        }
-      }
-#pragma unroll
-      for (int m = 0; m < M; m++) q[m] = inw ? q[m] : (R)0;
-#pragma unroll
-      for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-        for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
-      cur = inw ? lx + W[0] * (ly + W[1] * lz) : -1;
-      amag = inw ? pmag : 0.f;
-      if constexpr (MODE == 3) {
         ux += half_move(f0, vn[0], K.s[0]);
         uy += half_move(f1, vn[1], K.s[1]);
         uz += half_move(f2, vn[2], K.s[2]);
-        if (valid) {
-          uint32_t cnew;
-          if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
-            cnew = DEAD_CELL;  // the next sort drops it
-          } else {
-            cnew = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
-            bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
-            bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
-            bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
-          }
-          P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
-          P.cell[p] = cnew;
-        }
-      }
-    } else {
-      // KP particles per lane (P1 and P2 right after a sort: cells in runs)
-      static_assert(MODE != 3, "P3 takes one particle per lane");
-#pragma unroll 1
-      for (int k = 0; k < KP; k++) {
-        const int64_t p = pb + k;
-        if (p >= i1) break;
-        const uint32_t c = P.cell[p];
-        if (c == DEAD_CELL) continue;  // left through an open x face; the next sort drops it
-        const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
-        const R f0 = P.fx[p], f1 = P.fy[p], f2 = P.fz[p];
-        const R r = P.bp ? (R)1 / (R)P.bp[p] : (R)1;
-        R w8[8], q[M];
+        if (OX) alive = alive && ux >= 0 && ux < g.ncx;  // gone before x^{n+1/2}
         weights8(f0, f1, f2, w8);
-        const int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
-                  uz = unwrap(ck, cen[2], g.ncz);
-        const R v[3] = {P.vx[p], P.vy[p], P.vz[p]};
-        if constexpr (MODE == 1) {
-          q[0] = r;
-          q[1] = r * ((R)1.5 * v[0] - (R)0.5 * P.v1x[p]);
-          q[2] = r * ((R)1.5 * v[1] - (R)0.5 * P.v1y[p]);
-          q[3] = r * ((R)1.5 * v[2] - (R)0.5 * P.v1z[p]);
-        } else {
-          R Ag[3], Bg[3], E[3], vm[3], vpl[3];
-          const int tx = ux - bl[0], ty = uy - bl[1], tz = uz - bl[2];
-          if (tiled && (unsigned)tx < (unsigned)(T[0] - 1) && (unsigned)ty < (unsigned)(T[1] - 1) &&
-              (unsigned)tz < (unsigned)(T[2] - 1)) {
-            const int tb = tx + T[0] * (ty + T[1] * tz);
-#pragma unroll
-            for (int m = 0; m < 3; m++) Ag[m] = Bg[m] = (R)0;
-#pragma unroll
-            for (int cc = 0; cc < 8; cc++) {
-              const float4 a = gtile[2 * (tb + toff[cc])], b = gtile[2 * (tb + toff[cc]) + 1];
-              Ag[0] += w8[cc] * a.x; Ag[1] += w8[cc] * a.y; Ag[2] += w8[cc] * a.z;
-              Bg[0] += w8[cc] * a.w; Bg[1] += w8[cc] * b.x; Bg[2] += w8[cc] * b.y;
-            }
-          } else {
-            gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
-          }
-          const R mr = P.mr ? (R)P.mr[p] : (R)1;
-          ep_of(Ag, Bg, mr, E);
-          rotate(K.dt, v, E, Bg, mr, vm, vpl);
-#pragma unroll
-          for (int m = 0; m < 3; m++) q[m] = r * ((R)0.5 * (vpl[m] + vm[m]));
-        }
-        // fixed-point units (a power-of-two scale: exact), and the bound of
-        // every |w q| of this particle
-        float pmag = 0.f;
-        if constexpr (FIX) {
-#pragma unroll
-          for (int m = 0; m < M; m++) {
-            q[m] *= (R)__int_as_float((127 + fix_bits(GM0 + m)) << 23);
-            pmag = fmaxf(pmag, fabsf((float)q[m]));
-          }
-        }
-        const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
-        const bool inw = windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
-                         ly < W[1] - 1 && lz < W[2] - 1;
-        if (inw) {
-          const int key = lx + W[0] * (ly + W[1] * lz);
-          if (key != cur && cur >= 0) {
-            if (hkey < 0) {  // keep the first cell's sums for the end of the chunk
-              hkey = cur;
-              hmag = amag;
-#pragma unroll
-              for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                  hd[cc][m] = acc[cc][m];
-                  acc[cc][m] = (R)0;
-                }
-            } else {  // a cell in the middle of the lane's particles
-              win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
-#pragma unroll
-              for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-                for (int m = 0; m < M; m++) acc[cc][m] = (R)0;
-            }
-            amag = 0.f;
-          }
-          cur = key;
-          amag += pmag;
-#pragma unroll
-          for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-            for (int m = 0; m < M; m++) acc[cc][m] += w8[cc] * q[m];
-        } else {
-          misses++;
-#pragma unroll
-          for (int cc = 0; cc < 8; cc++) {
-            int64_t node = gidx(g, xnode<OX>(g, ux + (cc & 1)),
-                                node_of(uy + ((cc >> 2) & 1), g.ncy),
-                                node_of(uz + ((cc >> 1) & 1), g.ncz));
-#pragma unroll
-            for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
-          }
-        }
+        q[0] = r; q[1] = r * vn[0]; q[2] = r * vn[1]; q[3] = r * vn[2];
       }
     }
-    // the first cell of a lane that moved on is its own
-    if (KP > 1 && hkey >= 0) win_put<M, WS>(lacc, hkey, sy, sz, hd, hmag);
-    // the last cells: runs of lanes with the same cell are summed across the
-    // wavefront first (sorted particles), and only each run's last lane adds
-    // into the window; lanes that share no cell with a neighbour add directly
-    const int prevkey = __shfl_up(cur, 1);
-    const unsigned long long hm = __ballot(lane == 0 || prevkey != cur);
-    if (hm != ~0ull) {
-      const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
-      const SegCond sc = seg_cond(lane, 63 - __clzll(hm & le));
-      segscan(acc, amag, sc);
+    // fixed-point units (a power-of-two scale: exact), and the bound of
+    // every |w q| of this particle
+    float pmag = 0.f;
+    if constexpr (FIX) {
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        q[m] *= (R)__int_as_float((127 + fix_bits(GM0 + m)) << 23);
+        pmag = fmaxf(pmag, fabsf((float)q[m]));
+      }
     }
-    const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
-    if (tail && cur >= 0) win_put<M, WS>(lacc, cur, sy, sz, acc, amag);
+    const int lx = ux - u0[0], ly = uy - u0[1], lz = uz - u0[2];
+    const bool inw = alive && windowed && lx >= 0 && ly >= 0 && lz >= 0 && lx < W[0] - 1 &&
+                     ly < W[1] - 1 && lz < W[2] - 1;
+    if (alive && !inw) {
+      misses++;
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++) {
+        int64_t node = gidx(g, xnode<OX>(g, ux + (cc & 1)), node_of(uy + ((cc >> 2) & 1), g.ncy),
+                            node_of(uz + ((cc >> 1) & 1), g.ncz));
+#pragma unroll
+        for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
+      }
+    }
+    R acc[8][M];
+#pragma unroll
+    for (int m = 0; m < M; m++) q[m] = inw ? q[m] : (R)0;
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+      for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
+    const int cur = inw ? lx + W[0] * (ly + W[1] * lz) : -1;
+    float amag = inw ? pmag : 0.f;
+    if constexpr (MODE == 3) {
+      ux += half_move(f0, vn[0], K.s[0]);
+      uy += half_move(f1, vn[1], K.s[1]);
+      uz += half_move(f2, vn[2], K.s[2]);
+      if (valid) {
+        uint32_t cnew;
+        if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
+          cnew = DEAD_CELL;  // the next sort drops it
+        } else {
+          cnew = pack_cell(wrapc(ux, g.ncx), wrapc(uy, g.ncy), wrapc(uz, g.ncz));
+          bmin[0] = min(bmin[0], ux); bmax[0] = max(bmax[0], ux);
+          bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
+          bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
+        }
+        P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
+        P.cell[p] = cnew;
+      }
+    }
+    put_runs<M, WS>(acc, amag, cur, lacc, sy, sz, lane);
   }
   if (misses) atomicAdd(dcnt, misses);
   if (MODE == 3) {
@@ -707,16 +581,14 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R, MODE, KP>::value)) void k_pa
     if (tid < 3) atomicMin(vb_next + tid, sred[6 + tid]);
     else atomicMax(vb_next + tid, sred[6 + tid]);
   }
-  if (windowed) {
-    for (int qn = tid; qn < Wn; qn += PASS_WG) {
-      int lx = qn % W[0], ly = (qn / W[0]) % W[1], lz = qn / (W[0] * W[1]);
-      int64_t node = gidx(g, xnode<OX>(g, u0[0] + lx), node_of(u0[1] + ly, g.ncy),
-                          node_of(u0[2] + lz, g.ncz));
+  for (int qn = tid; qn < Wn; qn += PASS_WG) {
+    int lx = qn % W[0], ly = (qn / W[0]) % W[1], lz = qn / (W[0] * W[1]);
+    int64_t node = gidx(g, xnode<OX>(g, u0[0] + lx), node_of(u0[1] + ly, g.ncy),
+                        node_of(u0[2] + lz, g.ncz));
 #pragma unroll
-      for (int m = 0; m < M; m++) {
-        A v = lacc[qn + m * WS];
-        if (v != (A)0) atomicAdd(gacc + node + (int64_t)m * g.nn, v);
-      }
+    for (int m = 0; m < M; m++) {
+      A v = lacc[qn + m * WS];
+      if (v != (A)0) atomicAdd(gacc + node + (int64_t)m * g.nn, v);
     }
   }
 }
@@ -819,7 +691,7 @@ int pl_pass(swap_ctx* c, int mode) {
                        c->g, c->tboxes);
     if (int e = post(c)) return e;
   }
-  dim3 grid((unsigned)nb), blk(PASS_WG);
+  const dim3 grid((unsigned)nb), blk(PASS_WG);
   const int* dn = c->n_sorted < 0 ? c->d_zero : c->d_nitems;
   auto launch = [&](auto R0, auto OX0) {
     using R = decltype(R0);
@@ -829,27 +701,14 @@ int pl_pass(swap_ctx* c, int mode) {
     const R* S = (const R*)c->S;
     using A = typename AccOf<R>::T;
     A* acc = (A*)c->macc;
-    const bool runs = c->n_sorted >= 0 && c->steps_since_sort == 0;
-    if (mode == 1 && runs)
-      hipLaunchKernelGGL((k_pass<R, 1, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
-                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt, c->tboxes);
-    else if (mode == 1)
-      hipLaunchKernelGGL((k_pass<R, 1, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
-                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt, c->tboxes);
-    else if (mode == 2 && runs)
-      hipLaunchKernelGGL((k_pass<R, 2, PASS_KP, OX>), grid, blk, 0, c->stream, c->g, P, K, S,
-                         acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
-                         c->vb_next, c->dcnt, c->tboxes);
-    else if (mode == 2)
-      hipLaunchKernelGGL((k_pass<R, 2, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S,
-                         acc + c->g.nn, c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb,
-                         c->vb_next, c->dcnt, c->tboxes);
-    else
-      hipLaunchKernelGGL((k_pass<R, 3, 1, OX>), grid, blk, 0, c->stream, c->g, P, K, S, acc,
-                         c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,
-                         c->dcnt, c->tboxes);
+#define SWAP_PASS(MODE, ACC)                                                                   \
+  hipLaunchKernelGGL((k_pass<R, MODE, OX>), grid, blk, 0, c->stream, c->g, P, K, S, ACC,      \
+                     c->items, dn, ns, c->n, c->boxes, c->boxes_next, c->vb, c->vb_next,     \
+                     c->dcnt, c->tboxes)
+    if (mode == 1) SWAP_PASS(1, acc);
+    else if (mode == 2) SWAP_PASS(2, acc + c->g.nn);  // up only: moments 1..3
+    else SWAP_PASS(3, acc);
+#undef SWAP_PASS
   };
   using Open = std::true_type;
   using Periodic = std::false_type;

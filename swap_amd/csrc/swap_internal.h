@@ -12,11 +12,11 @@ constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
 constexpr int ITEM_MAX = 32768;
 constexpr int TAIL_ITEM = 4096;  // particles per work item of the unsorted tail
 constexpr int PASS_WG = 512;   // threads per particle-pass workgroup
-constexpr int PASS_KP = 4;     // consecutive particles per lane
-constexpr int WIN_VALUES = 8192;  // LDS deposit window, accumulator values (64 KB)
+// LDS deposit window, accumulator values: 81.6 KB, so that two workgroups
+// share a CU's 160 KB; divisible by 3 and 4 (the moments of P2 and P1/P3)
+constexpr int WIN_VALUES = 10200;
 constexpr int BOX_SORT_VALUES = WIN_VALUES;  // box (nodes x 4) that calls for a sort
 constexpr float WIN_SHIFT = 3.f;  // largest half-move cell shift a P3 window covers
-constexpr int GTILE_NODES = 512;  // LDS gather tile, nodes of 32 B (16 KB)
 constexpr int SREC = 8;  // values per node of the gather source S (6 used, 2 zero)
 
 struct Item {
