@@ -99,12 +99,16 @@ SWAP_HD double u24(uint32_t r) { return (double)(r >> 8) * (1.0 / 16777216.0); }
 // atomics, in LDS and in HBM.  Integer addition is exact and associative, so
 // the node sums carry no accumulation error and do not depend on the order
 // in which atomics are served; the only rounding is that of each
-// contribution to the fixed grid: 2^-26 for the density moment (weights
-// times 1/beta_p), 2^-18 km/s for the velocity moments -- below 1e-9 of the
-// node sums of the workloads here.  A contribution below 2^31 units (|w r|
-// < 32, |w r v| < 8192 km/s) converts in five operations; larger ones take
-// an exact 64-bit path.  The FP64 build accumulates doubles.
-constexpr int FIX_BITS_N = 26, FIX_BITS_V = 18;
+// contribution to the fixed grid: 2^-24 for the density moment (weights
+// times 1/beta_p), 2^-15 km/s for the velocity moments -- below 1e-9 of the
+// node sums of the workloads here, and below the FP32 rounding of a partial
+// sum over a few particles.  A contribution below 2^31 units (|w r| < 128,
+// |w r v| < 65536 km/s) converts in a few operations; larger ones take an
+// exact 64-bit path.  The units are as coarse as that allows so that the
+// partial sums over the runs of a sorted wavefront (up to 64 particles at
+// several hundred km/s) stay on the short path.  The FP64 build accumulates
+// doubles.
+constexpr int FIX_BITS_N = 24, FIX_BITS_V = 15;
 SWAP_HD constexpr int fix_bits(int moment) { return moment == 0 ? FIX_BITS_N : FIX_BITS_V; }
 template <typename R>
 struct AccOf {

@@ -187,7 +187,7 @@ def test_sort_permutation_is_stable_and_bit_exact():
 
 def test_fixed_point_deposit_accuracy():
     """200k particles in one cell: the fixed-point deposit (each FP32
-    contribution rounded to a 64-bit count of 2^-26, the counts added as
+    contribution rounded to a 64-bit count of 2^-24, the counts added as
     integers) matches the exact sum of the FP32 weights to a few ulps (plain
     FP32 accumulation would be off by ~1e-5)."""
     from swap_amd import physics
