@@ -249,6 +249,53 @@ __device__ __forceinline__ void put_runs(T (&a)[8][M], float& mag, int cur, A* l
   if (tail && cur >= 0) win_put<M, WS>(lacc, cur, sy, sz, a, mag);
 }
 
+// The value of lane (lane ^ J) for every lane of the wavefront, in VALU:
+// DPP quad permutes (J = 1, 2), row shifts into alternate 4- or 8-lane banks
+// (J = 4, 8), and the row-pair and half swaps of gfx950 (J = 16, 32).
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t x, int lane) {
+  if constexpr (J == 1) {
+    return __builtin_amdgcn_update_dpp(0u, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (J == 2) {
+    return __builtin_amdgcn_update_dpp(0u, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (J == 4) {
+    const uint32_t o = __builtin_amdgcn_update_dpp(0u, x, 0x104, 0xF, 0x5, false);  // row_shl:4
+    return __builtin_amdgcn_update_dpp(o, x, 0x114, 0xF, 0xA, false);               // row_shr:4
+  } else if constexpr (J == 8) {
+    const uint32_t o = __builtin_amdgcn_update_dpp(0u, x, 0x108, 0xF, 0x3, false);  // row_shl:8
+    return __builtin_amdgcn_update_dpp(o, x, 0x118, 0xF, 0xC, false);               // row_shr:8
+  } else if constexpr (J == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return ((lane >> 4) & 1) ? r[0] : r[1];
+  } else {
+    static_assert(J == 32, "xor distance");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return lane >= 32 ? r[0] : r[1];
+  }
+}
+// one compare-exchange of a bitonic network: blocks of K lanes ascending
+// where (lane & K) == 0, the lower lane of each pair J apart keeping the min
+template <int J, int K>
+__device__ __forceinline__ uint32_t bitonic_ce(uint32_t v, int lane) {
+  const uint32_t o = xor_lane<J>(v, lane);
+  const bool keep_min = ((lane & K) == 0) == ((lane & J) == 0);
+  return keep_min ? min(v, o) : max(v, o);
+}
+// ascending sort of 64 distinct values, one per lane (21 compare-exchanges)
+__device__ __forceinline__ uint32_t wave_sort(uint32_t v, int lane) {
+  v = bitonic_ce<1, 2>(v, lane);
+  v = bitonic_ce<2, 4>(v, lane);  v = bitonic_ce<1, 4>(v, lane);
+  v = bitonic_ce<4, 8>(v, lane);  v = bitonic_ce<2, 8>(v, lane);  v = bitonic_ce<1, 8>(v, lane);
+  v = bitonic_ce<8, 16>(v, lane); v = bitonic_ce<4, 16>(v, lane);
+  v = bitonic_ce<2, 16>(v, lane); v = bitonic_ce<1, 16>(v, lane);
+  v = bitonic_ce<16, 32>(v, lane); v = bitonic_ce<8, 32>(v, lane);
+  v = bitonic_ce<4, 32>(v, lane);  v = bitonic_ce<2, 32>(v, lane); v = bitonic_ce<1, 32>(v, lane);
+  v = bitonic_ce<32, 64>(v, lane); v = bitonic_ce<16, 64>(v, lane);
+  v = bitonic_ce<8, 64>(v, lane);  v = bitonic_ce<4, 64>(v, lane);
+  v = bitonic_ce<2, 64>(v, lane);  v = bitonic_ce<1, 64>(v, lane);
+  return v;
+}
+
 __device__ __forceinline__ int wave_min(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
   return v;
@@ -454,7 +501,7 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
     const int ci = cell_i(c), cj = cell_j(c), ck = cell_k(c);
     R f0 = cu.f[0], f1 = cu.f[1], f2 = cu.f[2];
     const R r = P.bp ? (R)1 / (R)cu.bp : (R)1;
-    R w8[8], q[M], vn[3] = {0, 0, 0};
+    R w8[8], q[M], vn[3] = {0, 0, 0}, mr = (R)1;
     weights8(f0, f1, f2, w8);
     int ux = OX ? ci : unwrap(ci, cen[0], g.ncx), uy = unwrap(cj, cen[1], g.ncy),
         uz = unwrap(ck, cen[2], g.ncz);
@@ -468,7 +515,7 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
     } else {
       R Ag[3], Bg[3], E[3], vm[3], vpl[3];
       gather6(g, S, gidx(g, ci, cj, ck), w8, Ag, Bg);
-      const R mr = P.mr ? (R)P.mr[p] : (R)1;
+      if (P.mr) mr = (R)P.mr[p];
       ep_of(Ag, Bg, mr, E);
       rotate(K.dt, v, E, Bg, mr, vm, vpl);
       if constexpr (MODE == 2) {
@@ -482,10 +529,6 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
           const float fv = (float)vn[m];  // plain compares: no NaN quieting
           vlo[m] = valid && fv < vlo[m] ? fv : vlo[m];
           vhi[m] = valid && fv > vhi[m] ? fv : vhi[m];
-        }
-        if (valid) {
-          P.v1x[p] = v[0]; P.v1y[p] = v[1]; P.v1z[p] = v[2];
-          P.vx[p] = vn[0]; P.vy[p] = vn[1]; P.vz[p] = vn[2];
         }
         ux += half_move(f0, vn[0], K.s[0]);
         uy += half_move(f1, vn[1], K.s[1]);
@@ -518,21 +561,17 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
         for (int m = 0; m < M; m++) gacc_add(gacc, node + (int64_t)m * g.nn, w8[cc] * q[m]);
       }
     }
-    R acc[8][M];
 #pragma unroll
     for (int m = 0; m < M; m++) q[m] = inw ? q[m] : (R)0;
-#pragma unroll
-    for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-      for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
     const int cur = inw ? lx + W[0] * (ly + W[1] * lz) : -1;
     float amag = inw ? pmag : 0.f;
+    R d0 = f0, d1 = f1, d2 = f2;  // fractions at the deposit position
     if constexpr (MODE == 3) {
       ux += half_move(f0, vn[0], K.s[0]);
       uy += half_move(f1, vn[1], K.s[1]);
       uz += half_move(f2, vn[2], K.s[2]);
+      uint32_t cnew = 0u;
       if (valid) {
-        uint32_t cnew;
         if (OX && (!alive || ux < 0 || ux >= g.ncx)) {
           cnew = DEAD_CELL;  // the next sort drops it
         } else {
@@ -541,10 +580,53 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
           bmin[1] = min(bmin[1], uy); bmax[1] = max(bmax[1], uy);
           bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
         }
-        P.fx[p] = f0; P.fy[p] = f1; P.fz[p] = f2;
-        P.cell[p] = cnew;
+      }
+      // Periodic runs: the wavefront's particles are stored in the order of
+      // their new cells within the wavefront's own 64 slots, so that the
+      // next step's P1 and P2 find each wavefront's particles in runs of
+      // equal cells (the scan then merges every run into one window add).
+      // Every array of a particle moves with it.  The valid lanes are a
+      // prefix of the wavefront and sort before the others, so the slots
+      // stay inside the work item.
+      int64_t dst = p;
+      if constexpr (!OX) {
+        const uint32_t key = valid ? (((uint32_t)uz & 255u) << 16) |
+                                         (((uint32_t)uy & 255u) << 8) | ((uint32_t)ux & 255u)
+                                   : 0xFFFFFFu;
+        const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+        int rank = lane;
+        if (__ballot(lane > 0 && prev > key) != 0ull) {
+          // sort (key, lane) over the wavefront; lane L then holds the
+          // element of sorted position L and tells that element its rank
+          const uint32_t kv = wave_sort((key << 6) | (uint32_t)lane, lane);
+          rank = __builtin_amdgcn_ds_permute((int)((kv & 63u) << 2), lane);
+        }
+        dst = pb - lane + rank;
+      }
+      if (valid && dst < i1) {
+        P.v1x[dst] = v[0]; P.v1y[dst] = v[1]; P.v1z[dst] = v[2];
+        P.vx[dst] = vn[0]; P.vy[dst] = vn[1]; P.vz[dst] = vn[2];
+        P.fx[dst] = f0; P.fy[dst] = f1; P.fz[dst] = f2;
+        P.cell[dst] = cnew;
+        if (!OX && dst != p) {
+          if (P.bp) {
+            P.bp[dst] = (float)cu.bp;
+            P.mr[dst] = (float)mr;
+          }
+          if (P.id) {
+            const uint32_t pid = P.id[p];
+            P.id[dst] = pid;
+          }
+        }
       }
     }
+    // the deposit: this particle's 8 corners x M moments
+    weights8(d0, d1, d2, w8);
+    R acc[8][M];
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+      for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
     put_runs<M, WS>(acc, amag, cur, lacc, sy, sz, lane);
   }
   if (misses) atomicAdd(dcnt, misses);

@@ -27,8 +27,9 @@ template <typename R>
 struct PDev {
   uint32_t* cell;
   R *fx, *fy, *fz, *vx, *vy, *vz, *v1x, *v1y, *v1z;
-  const float* bp;  // beta_p (null: 1)
-  const float* mr;  // mrat (null: 1)
+  float* bp;     // beta_p (null: 1)
+  float* mr;     // mrat (null: 1)
+  uint32_t* id;  // particle id (null: not tracked)
 };
 
 template <typename R>
@@ -238,6 +239,7 @@ PDev<R> pdev(swap_ctx* c) {
   d.v1x = q[6]; d.v1y = q[7]; d.v1z = q[8];
   d.bp = c->bp;
   d.mr = c->mr;
+  d.id = c->id;
   return d;
 }
 template <typename R>
