@@ -581,34 +581,40 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
           bmin[2] = min(bmin[2], uz); bmax[2] = max(bmax[2], uz);
         }
       }
-      // Periodic runs: the wavefront's particles are stored in the order of
-      // their new cells within the wavefront's own 64 slots, so that the
-      // next step's P1 and P2 find each wavefront's particles in runs of
-      // equal cells (the scan then merges every run into one window add).
-      // Every array of a particle moves with it.  The valid lanes are a
+      // The wavefront's particles are stored in the order of their new
+      // cells within the wavefront's own 64 slots, so that the next step's
+      // P1 and P2 find each wavefront's particles in runs of equal cells
+      // (the scan then merges every run into one window add).  Every array
+      // of a particle moves with it.  The lanes inside the work item are a
       // prefix of the wavefront and sort before the others, so the slots
-      // stay inside the work item.
-      int64_t dst = p;
-      if constexpr (!OX) {
-        const uint32_t key = valid ? (((uint32_t)uz & 255u) << 16) |
-                                         (((uint32_t)uy & 255u) << 8) | ((uint32_t)ux & 255u)
-                                   : 0xFFFFFFu;
-        const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
-        int rank = lane;
-        if (__ballot(lane > 0 && prev > key) != 0ull) {
-          // sort (key, lane) over the wavefront; lane L then holds the
-          // element of sorted position L and tells that element its rank
-          const uint32_t kv = wave_sort((key << 6) | (uint32_t)lane, lane);
-          rank = __builtin_amdgcn_ds_permute((int)((kv & 63u) << 2), lane);
-        }
-        dst = pb - lane + rank;
+      // stay inside the item; on an open x the ions that left (before this
+      // pass or during it) sort behind the live ones and keep the velocity,
+      // weight and id that the escape account reads when a sort drops them.
+      const bool inrange = pb < i1;
+      const bool live = valid && cnew != DEAD_CELL;
+      const uint32_t key = live ? (((uint32_t)uz & 255u) << 16) | (((uint32_t)uy & 255u) << 8) |
+                                      ((uint32_t)ux & 255u)
+                                : (inrange ? 0xFFFFFEu : 0xFFFFFFu);
+      const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+      int rank = lane;
+      if (__ballot(lane > 0 && prev > key) != 0ull) {
+        // sort (key, lane) over the wavefront; lane L then holds the
+        // element of sorted position L and tells that element its rank
+        const uint32_t kv = wave_sort((key << 6) | (uint32_t)lane, lane);
+        rank = __builtin_amdgcn_ds_permute((int)((kv & 63u) << 2), lane);
       }
-      if (valid && dst < i1) {
-        P.v1x[dst] = v[0]; P.v1y[dst] = v[1]; P.v1z[dst] = v[2];
-        P.vx[dst] = vn[0]; P.vy[dst] = vn[1]; P.vz[dst] = vn[2];
-        P.fx[dst] = f0; P.fy[dst] = f1; P.fz[dst] = f2;
-        P.cell[dst] = cnew;
-        if (!OX && dst != p) {
+      const int64_t dst = pb - lane + rank;
+      if (inrange && dst < i1) {
+        if (valid) {
+          P.v1x[dst] = v[0]; P.v1y[dst] = v[1]; P.v1z[dst] = v[2];
+          P.vx[dst] = vn[0]; P.vy[dst] = vn[1]; P.vz[dst] = vn[2];
+          P.fx[dst] = f0; P.fy[dst] = f1; P.fz[dst] = f2;
+          P.cell[dst] = cnew;
+        } else if (dst != p) {  // an ion that left before this pass
+          P.vx[dst] = v[0]; P.vy[dst] = v[1]; P.vz[dst] = v[2];
+          P.cell[dst] = DEAD_CELL;
+        }
+        if (dst != p) {
           if (P.bp) {
             P.bp[dst] = (float)cu.bp;
             P.mr[dst] = (float)mr;
