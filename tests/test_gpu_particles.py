@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import Pair, rel, sort_perm, unpack
+from helpers import TILE, Pair, rel, sort_perm, tiles, unpack
 from swap_amd import api
 
 pytestmark = pytest.mark.gpu
@@ -213,3 +213,51 @@ def test_fixed_point_deposit_accuracy():
     w111 = (f[0] * f[1]) * f[2]
     exact = np.sum(w111.astype(np.float64)) * qn
     assert abs(npg[4, 4, 4] / exact - 1) < 4e-7
+
+
+def _unwrap(c, centre, nc):
+    """the periodic image of cell c nearest to a box centre (k_pass.hip unwrap)"""
+    d = c - centre
+    return np.where(2 * d > nc, c - nc, np.where(2 * d < -nc, c + nc, c))
+
+
+@pytest.mark.gpu
+def test_step_stores_each_wavefront_in_cell_order():
+    """P3 stores the 64 particles of each wavefront in the order of their new
+    cells, inside the wavefront's own slots: after a step from a sorted load,
+    every 64-particle block of every work item (a tile's run) holds the same
+    particles as before, in non-decreasing cell key"""
+    pr = Pair(ncx=32, ncy=16, ncz=16, ppc=16, sort_every=8)
+    s, p = pr.s, pr.p
+    nc = np.array([p["ncx"], p["ncy"], p["ncz"]])
+    s.sort()
+    cell0 = s.particle_array(api.P_CELL)
+    ids0 = s.particle_array(api.P_ID)
+    tile, _, _ = tiles(cell0, nc)
+    assert np.all(np.diff(tile) >= 0)
+    s.step(1)
+    assert s.diagnostics()["sorts"] == 1  # the step itself did not sort
+    cell1 = s.particle_array(api.P_CELL)
+    ids1 = s.particle_array(api.P_ID)
+    i1, j1, k1 = unpack(cell1)
+    ntx, nty = -(-nc[0] // TILE[0]), -(-nc[1] // TILE[1])
+    starts = np.flatnonzero(np.r_[True, tile[1:] != tile[:-1]])
+    ends = np.r_[starts[1:], len(tile)]
+    nblocks = 0
+    for a, b in zip(starts, ends):
+        assert b - a <= 32768  # one work item per tile here
+        t = tile[a]
+        tc = np.array([t % ntx, (t // ntx) % nty, t // (ntx * nty)])
+        lo = tc * TILE
+        hi = np.minimum(lo + TILE, nc) - 1
+        cen = (lo + hi) // 2
+        ux = _unwrap(i1[a:b], cen[0], nc[0])
+        uy = _unwrap(j1[a:b], cen[1], nc[1])
+        uz = _unwrap(k1[a:b], cen[2], nc[2])
+        key = ((uz & 255) << 16) | ((uy & 255) << 8) | (ux & 255)
+        for q in range(0, b - a, 64):
+            blk = slice(q, min(q + 64, b - a))
+            assert np.all(np.diff(key[blk]) >= 0), (a, q)
+            assert np.array_equal(np.sort(ids1[a:b][blk]), np.sort(ids0[a:b][blk])), (a, q)
+            nblocks += 1
+    assert nblocks >= pr.n // 64
