@@ -318,7 +318,7 @@ __device__ __forceinline__ int xnode(const Geo& g, int u) {
 // workgroups (2 waves on each of a CU's 4 SIMDs): two of them (4 waves per
 // SIMD) share a CU when a wave needs at most 128 VGPRs and the LDS holds
 // both windows (2 x 81.6 KB of the CU's 160 KB).  The FP32 passes are held
-// to that (P3 needs 122 VGPRs with no spills); P3 at one workgroup per CU
+// to that (P3 fits 127 VGPRs with no spills); P3 at one workgroup per CU
 // (165 VGPRs when its divergent exits kept the 32 sums live on every path)
 // took 9.8 ms against 7.7.  The FP64 build runs one workgroup per CU with
 // the registers it needs (2 is the compiler's own default for 512 threads).
