@@ -431,9 +431,12 @@ __global__ void k_set_dtsub(double* ts, double dtsub, double ntf) {
 // One subcycle (maind.f:420-476) in two launches instead of nine.
 //   predictor: aj = curlB(b1)/(alpha n), a = aj - up; ct = face_to_center(a)
 //     x edge_to_center(b1); E = edge average of ct + nu aj; b1p2 = b12 -
-//     2 dtsub curlE(E)                                   (gutsf.f:1115-1169)
-//   corrector: the same chain on bh = (b1p2 + b1)/2 with get_Ep1's smoothed
-//     B (edge_to_center, then node->face->centre averages); b = b1 - dtsub
+//     2 dtsub curlE(E)                                   (gutsf.f:1115-1169);
+//     it stores bh = (b1p2 + b1)/2, all the corrector reads of b1p2 (b1 of
+//     its output node comes from its own LDS ring), so that the corrector
+//     stages one FP64 level instead of two
+//   corrector: the same chain on bh with get_Ep1's smoothed B
+//     (edge_to_center, then node->face->centre averages); b = b1 - dtsub
 //     curlE(E)                                 (gutsf.f:1174-1304, 1309-1359)
 // A workgroup owns TX x TY columns of canonical nodes and a run of z planes.
 // It marches up z keeping a ring of planes of every intermediate in LDS, each
@@ -566,6 +569,8 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
   // (staged in LDS), up plane s+2 at the curl positions, nu plane s+1 at the
   // E positions, b12 at the output node of plane s
   double fb[KL][3] = {}, fbase[3] = {};
+  // b1 at the output node, planes s and s+1 (taken from the LDS ring)
+  double b1_prev[3] = {0, 0, 0}, b1_next[3] = {0, 0, 0};
   R fn[KL] = {}, fu[KA][3] = {}, fnu[KE] = {};
   auto fetch = [&](int s) {
     const int64_t pb = wrapc(s + 2, g.nz) * nxy, pn = wrapc(s + 3, g.nz) * nxy,
@@ -688,13 +693,20 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
         se[r][1][p] = (R)0.5 * (sc[r][1][p] + sc[r][1][p + PW]) + nuv * sj[r][1][p];
         se[r][2][p] = (R)0.5 * (sc[r][2][p] + sc[rp][2][p]) + nuv * sj[r][2][p];
       }
+      // b1 of the output node at plane s+1, for the next step's output: its
+      // ring slot takes plane s+3 when that step starts
+      if (oown) {
+        const int p = (olx + 2) + PW * (oly + 2);
+#pragma unroll
+        for (int m = 0; m < 3; m++) b1_next[m] = sb[r][m][p];
+      }
     }
     __syncthreads();
     if (xface && s >= Z0 - 1) {
       ghost_x<PW, PA, NT>(&se[(s + 1) & 1][0][0], 3, tid, X0, g.ncx);
       __syncthreads();
     }
-    if (s >= Z0 && oown) {  // b1p2 = b12 - coef curlE(E) at plane s
+    if (s >= Z0 && oown) {  // b1p2 = b12 - coef curlE(E) at plane s; out = (b1p2 + b1)/2
       const int r = s & 1, rp = r ^ 1;
       const int p = (olx + 2) + PW * (oly + 2);
       const double c0 = ((double)se[r][2][p + PW] - (double)se[r][2][p]) / f.dy +
@@ -703,14 +715,17 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
                         ((double)se[rp][0][p] - (double)se[r][0][p]) / f.dz;
       const double c2 = ((double)se[r][0][p] - (double)se[r][0][p + PW]) / f.dy +
                         ((double)se[r][1][p + 1] - (double)se[r][1][p]) / f.dx;
-      put3<double>(g, out, ox, oy, s, cbase[0] - coef * c0, cbase[1] - coef * c1,
-                   cbase[2] - coef * c2);
+      put3<double>(g, out, ox, oy, s, 0.5 * ((cbase[0] - coef * c0) + b1_prev[0]),
+                   0.5 * ((cbase[1] - coef * c1) + b1_prev[1]),
+                   0.5 * ((cbase[2] - coef * c2) + b1_prev[2]));
     }
+#pragma unroll
+    for (int m = 0; m < 3; m++) b1_prev[m] = b1_next[m];
   }
 }
 
 template <typename R, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* __restrict__ bp,
+__global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* __restrict__ bh,
                                                         const double* __restrict__ b1,
                                                         const R* __restrict__ np,
                                                         const R* __restrict__ up,
@@ -749,10 +764,10 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
   own<KE, NT, PW>(tid, EW, EN, 2, 2, tcx, tcy, g.nx, ep, eq);
   const int olx = tid % TX, oly = tid / TX, ox = X0 + olx, oy = Y0 + oly;
   const bool oown = tid < TX * TY && ox <= g.ncx && oy <= g.ncy;
-  // step s's global data, fetched one step ahead: b1p2 and b1 plane s+3 and
-  // np plane s+4 (staged), up plane s+3 at the curl positions, nu plane s+1
-  // at the E positions, b1 at the output node of plane s
-  double fp[KL][3] = {}, fq[KL][3] = {}, fbase[3] = {};
+  // step s's global data, fetched one step ahead: bh plane s+3 and np
+  // plane s+4 (staged), up plane s+3 at the curl positions, nu plane s+1 at
+  // the E positions, b1 at the output node of plane s
+  double fp[KL][3] = {}, fbase[3] = {};
   R fn[KL] = {}, fu[KA][3] = {}, fnu[KE] = {};
   auto fetch = [&](int s) {
     const int64_t pb = wrapc(s + 3, g.nz) * nxy, pn = wrapc(s + 4, g.nz) * nxy,
@@ -761,10 +776,7 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
     for (int k = 0; k < KL; k++)
       if (lp[k] >= 0) {
 #pragma unroll
-        for (int m = 0; m < 3; m++) {
-          fp[k][m] = bp[pb + lq[k] + m * nn];
-          fq[k][m] = b1[pb + lq[k] + m * nn];
-        }
+        for (int m = 0; m < 3; m++) fp[k][m] = bh[pb + lq[k] + m * nn];
         fn[k] = np[pn + lq[k]];
       }
 #pragma unroll
@@ -792,7 +804,7 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
       for (int k = 0; k < KL; k++)
         if (lp[k] >= 0) {
 #pragma unroll
-          for (int m = 0; m < 3; m++) sb[rb][m][lp[k]] = 0.5 * (fp[k][m] + fq[k][m]);
+          for (int m = 0; m < 3; m++) sb[rb][m][lp[k]] = fp[k][m];
           sn[rn][lp[k]] = fn[k];
         }
 #pragma unroll
