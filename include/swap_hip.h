@@ -65,10 +65,17 @@ enum {
                               the x = 0 face.  Needs ncx >= 4 and
                               sort_every >= 1; no denoiser. */
   SWAP_TRACK_IDS = 4,      /* keep a particle id through sorts (tests) */
-  SWAP_UNFUSED_FIELDS = 8  /* field subcycles of swap_step as one kernel per
+  SWAP_UNFUSED_FIELDS = 8, /* field subcycles of swap_step as one kernel per
                               reference routine (nine launches) instead of the
                               fused LDS-tiled predictor and corrector (two);
                               the same arithmetic, kept for checking */
+  SWAP_PERSISTENT_FIELDS = 16 /* the fused subcycles of a step (or of one
+                              swap_field_subcycles call) as one cooperative
+                              launch with a grid barrier between the
+                              predictor and corrector phases, instead of two
+                              launches per subcycle; bit-identical, measured
+                              slower, kept for comparison.  Ignored with
+                              SWAP_UNFUSED_FIELDS. */
 };
 
 typedef struct {

@@ -27,7 +27,7 @@ P_XP, P_VP, P_VP1, P_EP, P_VPLUS, P_VMINUS, P_IJKP, P_WGHT, P_CELL, P_FRAC, P_BE
     P_ID = range(13)
 V_CUR, V_EXTRAP, V_VP1 = 0, 1, 2
 FP32, FP64 = 32, 64
-SPECIES, OUTFLOW_X, TRACK_IDS, UNFUSED_FIELDS = 1, 2, 4, 8
+SPECIES, OUTFLOW_X, TRACK_IDS, UNFUSED_FIELDS, PERSISTENT_FIELDS = 1, 2, 4, 8, 16
 DN_ALL, DN_DISJOINT, DN_DISJOINT_STEP = 0, 1, 2
 
 

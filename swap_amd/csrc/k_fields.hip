@@ -6,6 +6,8 @@
 // each storage node, so ghost layers come out equal to what boundary.f's
 // periodic() would copy (boundary.f:9-52) and no separate ghost pass runs.
 // B is held and advanced in FP64; everything else in the context precision R.
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 
 #include "swap_internal.h"
@@ -526,26 +528,43 @@ __device__ __forceinline__ void own(int tid, int W, int N, int ox, int oy, const
   }
 }
 
+// the predictor's LDS rings
+template <typename R, int TX, int TY>
+struct PredLds {
+  static constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
+  double sb[2][3][PA];  // b1, planes s+1, s+2
+  R sn[2][PA];          // np, planes s+2, s+3
+  R sa[2][3][PA];       // a = aj - up
+  R sj[2][3][PA];       // aj
+  R sc[2][3][PA];       // ct
+  R se[2][3][PA];       // E
+  int tcx[PW], tcy[PH];
+};
+
+// the predictor over tile (bx, by, bz): TX x TY columns, z planes
+// [1 + bz tz, 1 + (bz + 1) tz); coef = 2 dtsub
 template <typename R, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* __restrict__ b1,
-                                                        const double* __restrict__ b12,
-                                                        const R* __restrict__ np,
-                                                        const R* __restrict__ up,
-                                                        const R* __restrict__ nu,
-                                                        const double* __restrict__ ts, double cmul,
-                                                        int tz, double* __restrict__ out) {
-  const double coef = cmul * ts[0];  // 2 dtsub (predictor) or dtsub (corrector)
+__device__ __forceinline__ void sub_predict_tile(const Geo& g, const FC& f,
+                                                 const double* __restrict__ b1,
+                                                 const double* __restrict__ b12,
+                                                 const R* __restrict__ np,
+                                                 const R* __restrict__ up,
+                                                 const R* __restrict__ nu, double coef, int bx,
+                                                 int by, int bz, int tz,
+                                                 double* __restrict__ out,
+                                                 PredLds<R, TX, TY>& L) {
   constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
   static_assert(NT >= TX * TY, "one thread per output column at least");
-  __shared__ double sb[2][3][PA];  // b1, planes s+1, s+2
-  __shared__ R sn[2][PA];          // np, planes s+2, s+3
-  __shared__ R sa[2][3][PA];       // a = aj - up
-  __shared__ R sj[2][3][PA];       // aj
-  __shared__ R sc[2][3][PA];       // ct
-  __shared__ R se[2][3][PA];       // E
-  __shared__ int tcx[PW], tcy[PH];
+  auto& sb = L.sb;
+  auto& sn = L.sn;
+  auto& sa = L.sa;
+  auto& sj = L.sj;
+  auto& sc = L.sc;
+  auto& se = L.se;
+  auto& tcx = L.tcx;
+  auto& tcy = L.tcy;
   const int tid = threadIdx.x;
-  const int X0 = 1 + blockIdx.x * TX, Y0 = 1 + blockIdx.y * TY, Z0 = 1 + blockIdx.z * tz;
+  const int X0 = 1 + bx * TX, Y0 = 1 + by * TY, Z0 = 1 + bz * tz;
   const int Z1 = min(Z0 + tz, g.ncz + 1);
   if (Z0 > g.ncz) return;  // empty z chunk (whole workgroup)
   for (int t = tid; t < PW; t += NT) tcx[t] = tile_cx(g, X0 - 2 + t);
@@ -724,29 +743,45 @@ __global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* _
   }
 }
 
+// the corrector's LDS rings
+template <typename R, int TX, int TY>
+struct CorrLds {
+  static constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
+  double sb[2][3][PA];  // bh = (b1p2 + b1)/2, planes s+2, s+3
+  R sn[2][PA];          // np, planes s+3, s+4
+  R sa[3][3][PA];       // a = aj - up, planes s+1 .. s+3
+  R sj[3][3][PA];       // aj
+  R sw[3][3][PA];       // btc0 = edge_to_center(bh)
+  R sc[2][3][PA];       // ct
+  R se[2][3][PA];       // E
+  int tcx[PW], tcy[PH];
+};
+
+// the corrector over tile (bx, by, bz); coef = dtsub; E and aj of the tile
+// are stored when Eout is set
 template <typename R, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* __restrict__ bh,
-                                                        const double* __restrict__ b1,
-                                                        const R* __restrict__ np,
-                                                        const R* __restrict__ up,
-                                                        const R* __restrict__ nu,
-                                                        const double* __restrict__ ts, double cmul,
-                                                        int tz, double* __restrict__ out,
-                                                        R* __restrict__ Eout,
-                                                        R* __restrict__ ajout) {
-  const double coef = cmul * ts[0];
+__device__ __forceinline__ void sub_correct_tile(const Geo& g, const FC& f,
+                                                 const double* __restrict__ bh,
+                                                 const double* __restrict__ b1,
+                                                 const R* __restrict__ np,
+                                                 const R* __restrict__ up,
+                                                 const R* __restrict__ nu, double coef, int bx,
+                                                 int by, int bz, int tz,
+                                                 double* __restrict__ out, R* __restrict__ Eout,
+                                                 R* __restrict__ ajout, CorrLds<R, TX, TY>& L) {
   constexpr int PW = FT<TX, TY>::PW, PH = FT<TX, TY>::PH, PA = FT<TX, TY>::PA;
   static_assert(NT >= TX * TY, "one thread per output column at least");
-  __shared__ double sb[2][3][PA];  // bh = (b1p2 + b1)/2, planes s+2, s+3
-  __shared__ R sn[2][PA];          // np, planes s+3, s+4
-  __shared__ R sa[3][3][PA];       // a = aj - up, planes s+1 .. s+3
-  __shared__ R sj[3][3][PA];       // aj
-  __shared__ R sw[3][3][PA];       // btc0 = edge_to_center(bh)
-  __shared__ R sc[2][3][PA];       // ct
-  __shared__ R se[2][3][PA];       // E
-  __shared__ int tcx[PW], tcy[PH];
+  auto& sb = L.sb;
+  auto& sn = L.sn;
+  auto& sa = L.sa;
+  auto& sj = L.sj;
+  auto& sw = L.sw;
+  auto& sc = L.sc;
+  auto& se = L.se;
+  auto& tcx = L.tcx;
+  auto& tcy = L.tcy;
   const int tid = threadIdx.x;
-  const int X0 = 1 + blockIdx.x * TX, Y0 = 1 + blockIdx.y * TY, Z0 = 1 + blockIdx.z * tz;
+  const int X0 = 1 + bx * TX, Y0 = 1 + by * TY, Z0 = 1 + bz * tz;
   const int Z1 = min(Z0 + tz, g.ncz + 1);
   if (Z0 > g.ncz) return;  // empty z chunk (whole workgroup)
   for (int t = tid; t < PW; t += NT) tcx[t] = tile_cx(g, X0 - 2 + t);
@@ -933,6 +968,78 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
       put3<double>(g, out, ox, oy, s, cbase[0] - coef * c0, cbase[1] - coef * c1,
                    cbase[2] - coef * c2);
     }
+  }
+}
+
+// one predictor launch: a workgroup per tile; coef = cmul dtsub
+template <typename R, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_sub_predict(Geo g, FC f, const double* __restrict__ b1,
+                                                        const double* __restrict__ b12,
+                                                        const R* __restrict__ np,
+                                                        const R* __restrict__ up,
+                                                        const R* __restrict__ nu,
+                                                        const double* __restrict__ ts, double cmul,
+                                                        int tz, double* __restrict__ out) {
+  __shared__ PredLds<R, TX, TY> L;
+  sub_predict_tile<R, TX, TY, NT>(g, f, b1, b12, np, up, nu, cmul * ts[0], blockIdx.x,
+                                  blockIdx.y, blockIdx.z, tz, out, L);
+}
+
+// one corrector launch
+template <typename R, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* __restrict__ bh,
+                                                        const double* __restrict__ b1,
+                                                        const R* __restrict__ np,
+                                                        const R* __restrict__ up,
+                                                        const R* __restrict__ nu,
+                                                        const double* __restrict__ ts, double cmul,
+                                                        int tz, double* __restrict__ out,
+                                                        R* __restrict__ Eout,
+                                                        R* __restrict__ ajout) {
+  __shared__ CorrLds<R, TX, TY> L;
+  sub_correct_tile<R, TX, TY, NT>(g, f, bh, b1, np, up, nu, cmul * ts[0], blockIdx.x,
+                                  blockIdx.y, blockIdx.z, tz, out, Eout, ajout, L);
+}
+
+// nsub subcycles in one cooperative launch of as many workgroups as the chip
+// holds at once: the workgroups take the predictor's tiles in turn, meet at a
+// grid barrier (whose fences write back and invalidate every XCD's L2, so
+// each phase reads what the last one wrote on any XCD), take the
+// corrector's tiles, meet again, and so on.  The B buffers rotate as the
+// host rotates them after a two-launch subcycle: b1 and b12 trade places,
+// bh is the predictor's output every time.  Tiles are numbered x fastest,
+// then y, then z, as the two-launch grid dispatches them.
+template <typename R, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_subcycles(Geo g, FC f, double* b1, double* b12,
+                                                      double* bh, const R* np, const R* up,
+                                                      const R* nu, const double* ts, int nsub,
+                                                      int tz, R* Eout, R* ajout) {
+  __shared__ union {
+    PredLds<R, TX, TY> p;
+    CorrLds<R, TX, TY> c;
+  } L;
+  const int gx = (g.ncx + TX - 1) / TX, gy = (g.ncy + TY - 1) / TY;
+  const int nt = gx * gy * ((g.ncz + tz - 1) / tz);
+  const double dts = ts[0];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  for (int q = 0; q < nsub; q++) {
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+      __syncthreads();  // the last tile's reads of the rings are done
+      sub_predict_tile<R, TX, TY, NT>(g, f, b1, b12, np, up, nu, 2.0 * dts, t % gx,
+                                      (t / gx) % gy, t / (gx * gy), tz, bh, L.p);
+    }
+    grid.sync();
+    const bool last = q == nsub - 1;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+      __syncthreads();
+      sub_correct_tile<R, TX, TY, NT>(g, f, bh, b1, np, up, nu, 1.0 * dts, t % gx,
+                                      (t / gx) % gy, t / (gx * gy), tz, b12,
+                                      last ? Eout : nullptr, last ? ajout : nullptr, L.c);
+    }
+    if (!last) grid.sync();
+    double* const t = b1;
+    b1 = b12;
+    b12 = t;
   }
 }
 
@@ -1158,6 +1265,44 @@ int fl_subcycle_fused(swap_ctx* c, bool last) {
                               c->b12,
                               last ? F<R>(c, SWAP_F_E) : (R*)nullptr,
                               last ? F<R>(c, SWAP_F_AJ) : (R*)nullptr));
+  return post(c);
+}
+// nsub fused subcycles in one cooperative launch (k_subcycles): one
+// workgroup per CU at the corrector's LDS, z chunks so that the tiles of a
+// phase match the workgroups; E and aj are stored by the last corrector.
+// The caller rotates the B buffers as after nsub two-launch subcycles.
+int fl_subcycles_persistent(swap_ctx* c, int nsub) {
+  if (nsub < 1) return 0;
+  FC f = fc_of(c);
+  Geo g = c->g;
+  DISPATCH(c, constexpr int TX = FTile<R>::TX, TY = FTile<R>::TY, NT = FTile<R>::NC;
+           auto kern = k_subcycles<R, TX, TY, NT>;
+           const int gx = (g.ncx + TX - 1) / TX, gy = (g.ncy + TY - 1) / TY;
+           if (c->pers_grid <= 0) {
+             int per = 0, cus = 0;
+             HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, NT, 0));
+             HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                             c->device));
+             if (per < 1 || cus < 1)
+               return set_err(c, SWAP_EHIP, "k_subcycles does not fit a CU");
+             const int nz = std::max(1, std::min((cus * per) / (gx * gy), g.ncz));
+             c->pers_tz = (g.ncz + nz - 1) / nz;
+             const int nt = gx * gy * ((g.ncz + c->pers_tz - 1) / c->pers_tz);
+             c->pers_grid = std::min(cus * per, nt);
+           }
+           double* b1 = c->b1;
+           double* b12 = c->b12;
+           double* bh = c->b1p2;
+           const R* np = F<R>(c, SWAP_F_NP);
+           const R* up = F<R>(c, SWAP_F_UP);
+           const R* nu = F<R>(c, SWAP_F_NU);
+           const double* ts = c->ts;
+           int tz = c->pers_tz;
+           R* E = F<R>(c, SWAP_F_E);
+           R* aj = F<R>(c, SWAP_F_AJ);
+           void* args[] = {&g, &f, &b1, &b12, &bh, &np, &up, &nu, &ts, &nsub, &tz, &E, &aj};
+           HIPCHK(c, hipLaunchCooperativeKernel((const void*)kern, dim3(c->pers_grid), dim3(NT),
+                                                args, 0, c->stream)));
   return post(c);
 }
 int fl_zero(swap_ctx* c, void* p, size_t bytes) {

@@ -187,6 +187,21 @@ int subcycle_fused(swap_ctx* c, bool last) {
   return 0;
 }
 
+// nsub fused subcycles, the last storing E and aj: one cooperative launch
+// with SWAP_PERSISTENT_FIELDS, else two launches each; the B buffers end as
+// nsub calls of subcycle_fused leave them
+int subcycles_fused(swap_ctx* c, int nsub) {
+  if (!(c->cfg.flags & SWAP_PERSISTENT_FIELDS)) {
+    for (int q = 0; q < nsub; q++) TRY(subcycle_fused(c, q == nsub - 1));
+    return 0;
+  }
+  if (nsub < 1) return 0;
+  TRY(fl_subcycles_persistent(c, nsub));
+  if (nsub & 1) std::swap(c->b1, c->b12);
+  c->b1p2_alias = true;
+  return 0;
+}
+
 }  // namespace
 
 // An open x keeps the ions that left as dead cells until the next sort;
@@ -865,9 +880,11 @@ int swap_face_to_center(swap_ctx* c, int in, int out) {
 int swap_field_subcycles(swap_ctx* c, int ntf, double dtsub) {
   if (!c || ntf < 0 || !(dtsub > 0)) return SWAP_EINVAL;
   const bool fused = !(c->cfg.flags & SWAP_UNFUSED_FIELDS);
-  if (fused) TRY(fl_set_dtsub(c, dtsub, ntf));
-  for (int q = 0; q < ntf; q++)
-    TRY(fused ? subcycle_fused(c, q == ntf - 1) : subcycle_impl(c, dtsub));
+  if (fused) {
+    TRY(fl_set_dtsub(c, dtsub, ntf));
+    return subcycles_fused(c, ntf);
+  }
+  for (int q = 0; q < ntf; q++) TRY(subcycle_impl(c, dtsub));
   return 0;
 }
 
@@ -978,7 +995,7 @@ int swap_step(swap_ctx* c, int nsteps) {
       HIPCHK(c, hipEventRecord(c->ev_stat, c->stream));
       Stage t(c, SWAP_K_FIELD);
       const int n0 = c->cfg.ntsub;
-      for (int q = 0; q < n0; q++) TRY(subcycle_fused(c, q == n0 - 1));
+      TRY(subcycles_fused(c, n0));
       HIPCHK(c, hipEventSynchronize(c->ev_stat));
       const int ntf = (int)c->hstat->ntf;
       c->box_over = c->hstat->box_over;

@@ -113,6 +113,7 @@ struct swap_ctx {
   uint32_t *icnt = nullptr, *ioff = nullptr, *iblk = nullptr;
   int64_t islots = 0;
   int fz_pred = 0, fz_corr = 0;  // z chunks of the fused field launches (0: not yet chosen)
+  int pers_grid = 0, pers_tz = 0;  // cooperative subcycle launch: workgroups, planes per tile
   double* dred = nullptr;     // FP64 reduction scratch
   double* ts = nullptr;       // device dtsub and ntf of the current step (check_time_step)
   // pinned host copies of ts (dtsub, ntf) and of the box-overflow count, and
@@ -174,6 +175,7 @@ int fl_time_step_dev(swap_ctx* c);
 int fl_set_dtsub(swap_ctx* c, double dtsub, int ntf);
 int fl_copy_b(swap_ctx* c, const double* src, double* dst);
 int fl_subcycle_fused(swap_ctx* c, bool last);  // dtsub from c->ts
+int fl_subcycles_persistent(swap_ctx* c, int nsub);  // nsub of them, one launch
 int fl_zero(swap_ctx* c, void* p, size_t bytes);
 
 // ---- particle launchers (k_particles.hip) ----

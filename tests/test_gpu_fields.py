@@ -160,16 +160,17 @@ def _periodic_np(a, open_x=False):
                                        ((16, 8, 8), True), ((4, 6, 37), True),
                                        ((40, 9, 33), True), ((33, 5, 9), True)])
 def test_fused_subcycles_equal_per_routine_kernels(precision, nc, open_x):
-    """the LDS-tiled predictor/corrector pair (two launches per subcycle) gives
-    bit for bit what the nine per-routine kernels give: B levels, and the E
-    and aj the last corrector leaves.  Shapes off the 32x4 tile and the z
-    chunking, and a 2-cell axis (every halo wraps more than once); with an
-    open x (zero-gradient x ghosts), tiles at either face, a grid narrower
-    than one tile and one a cell past it."""
+    """the LDS-tiled predictor/corrector pair (two launches per subcycle), and
+    the same tiles in one cooperative launch with grid barriers, give bit for
+    bit what the nine per-routine kernels give: B levels, and the E and aj the
+    last corrector leaves.  Shapes off the 32x4 tile and the z chunking, and a
+    2-cell axis (every halo wraps more than once); with an open x
+    (zero-gradient x ghosts), tiles at either face, a grid narrower than one
+    tile and one a cell past it."""
     p = physics.workload(*nc, 2)
     out = []
     ox = api.OUTFLOW_X if open_x else 0
-    for flags in (ox, ox | api.UNFUSED_FIELDS):
+    for flags in (ox, ox | api.UNFUSED_FIELDS, ox | api.PERSISTENT_FIELDS):
         s = physics.make_swap(p, precision=precision, flags=flags, capacity=16)
         rng = np.random.default_rng(5)
         shp = (s.nz, s.ny, s.nx)
@@ -184,7 +185,8 @@ def test_fused_subcycles_equal_per_routine_kernels(precision, nc, open_x):
         s.field_subcycles(3, p["dt"] / 10)
         out.append({f: s.get_field(f) for f in ("b1", "b12", "b1p2", "E", "aj")})
         s.close()
-    for f in out[0]:
-        assert np.array_equal(out[0][f], out[1][f]), f
+    for o, name in zip(out[1:], ("per-routine", "cooperative")):
+        for f in out[0]:
+            assert np.array_equal(out[0][f], o[f]), (name, f)
     # and the subcycles did move B
     assert not np.array_equal(out[0]["b1"], out[0]["b12"])

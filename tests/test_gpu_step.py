@@ -93,6 +93,35 @@ def test_routine_level_step_matches_fused():
     assert rel(s.get_field("b1") - b0, b.s.get_field("b1") - b0) < 1e-4
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_cooperative_field_launch_step_matches_two_launches(precision):
+    """swap_step with the field subcycles in one cooperative launch (grid
+    barriers between the phases) against two launches per subcycle, over
+    steps with and without a sort, on a grid off the field tiles: bit for bit
+    in the FP32 build (fixed-point moment sums); the FP64 build sums its
+    moments with FP64 atomics, whose order varies from run to run, so there
+    to 1e-11 with particles matched by id"""
+    runs = []
+    for extra in (0, api.PERSISTENT_FIELDS):
+        a = Pair(40, 9, 33, ppc=4, precision=precision, sort_every=2,
+                 flags=api.TRACK_IDS | extra)
+        a.s.step(3)
+        r = {f: a.s.get_field(f) for f in ("b1", "b12", "np", "up", "E", "aj")}
+        ids = a.s.particle_array(api.P_ID)
+        r.update(x=_by_id(a.s, a.s.xp()), v=_by_id(a.s, a.s.vp()), ids=np.sort(ids))
+        if precision == 32:
+            r.update(x_raw=a.s.xp(), id_raw=ids)
+        runs.append(r)
+        a.s.close()
+    for k in runs[0]:
+        if precision == 32:
+            assert np.array_equal(runs[0][k], runs[1][k]), k
+        elif k == "ids":
+            assert np.array_equal(runs[0][k], runs[1][k]), k
+        else:
+            assert rel(runs[0][k], runs[1][k]) < 1e-11, k
+
+
 def test_invariants_over_steps():
     pr = Pair(ncx=24, ncy=16, ncz=16, ppc=8)
     s, p = pr.s, pr.p
