@@ -47,13 +47,20 @@ __device__ __forceinline__ int unwrap(int c, int centre, int nc) {
 // fixed-point unit once (exact: a power of two), so its FP32 products and
 // run sums are the unit counts themselves, rounded exactly as the products
 // and sums of the unscaled moments would be.  Adding a sum is then rounding
-// to the nearest integer and converting: three instructions for a sum below
-// 2^31 units; in general the 32-bit form or, for |sum| >= 2^31, an exact
-// split into two 32-bit halves, selected per value without a branch.  The
-// FP64 build adds doubles.
+// to an integer and converting: two instructions for a sum below 2^30
+// units; in general the 32-bit form or, for |sum| >= 2^31, an exact split
+// into two 32-bit halves, selected per value without a branch.  The FP64
+// build adds doubles.
 constexpr float TWO31F = 2147483648.f, TWO32F = 4294967296.f;
-__device__ __forceinline__ unsigned long long fix_small(float x) {  // |rint(x)| < 2^31
-  return (unsigned long long)(long long)(int)rintf(x);
+// |x| < 2^30: floor(x + 1/2) in one instruction (v_cvt_rpi_i32_f32, whose
+// add is exact: it equals floor(x + 1/2) taken in FP64 on every input
+// checked, ties rounding up where rintf goes to even), then the sign
+// extension.  Any rounding that depends on x alone keeps the sums exact and
+// order-free; this one spends two instructions per value instead of three.
+__device__ __forceinline__ unsigned long long fix_small(float x) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return (unsigned long long)(long long)r;
 }
 __device__ __forceinline__ unsigned long long fix_any(float x) {  // |rint(x)| < 2^62
   const float r = rintf(x);
@@ -80,7 +87,7 @@ __device__ __forceinline__ void gacc_add(double* a, int64_t i, double v) {
 // {0, sy} (sy, sz: the window's y and z strides) and moment m lies WS values
 // further on (a compile-time stride), so every add is one DS atomic at one
 // of four base addresses plus an immediate offset.  mag bounds |sum| of
-// every value (FP32 build); below 2^30 the three-instruction conversion
+// every value (FP32 build); below 2^30 the two-instruction conversion
 // holds for all 8 x M of them, with one branch.
 template <int M, int WS>
 __device__ __forceinline__ void win_put(unsigned long long* lacc, int key, int sy, int sz,
@@ -234,12 +241,16 @@ __device__ __forceinline__ void segscan(double (&a)[8][M], float& mag, const Seg
 // runs of consecutive lanes with the same cell are summed across the
 // wavefront first (sorted particles), and only each run's last lane adds
 // into the window; lanes that share no cell with a neighbour add directly.
-// Every lane of the wavefront calls it.
-template <int M, int WS, typename T, typename A>
+// FH: lanes that start a run whatever their neighbour holds (bit L: lane
+// L), so that no run crosses them and the scan steps that reach across
+// them are skipped, at the price of one more tail (a few more active lanes
+// in the same window atomics) per crossing.  Every lane of the wavefront
+// calls it.
+template <int M, int WS, unsigned long long FH, typename T, typename A>
 __device__ __forceinline__ void put_runs(T (&a)[8][M], float& mag, int cur, A* lacc, int sy,
                                          int sz, int lane) {
   const int prevkey = __shfl_up(cur, 1);
-  const unsigned long long hm = __ballot(lane == 0 || prevkey != cur);
+  const unsigned long long hm = __ballot(lane == 0 || prevkey != cur) | FH;
   if (hm != ~0ull) {
     const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
     const SegCond sc = seg_cond(lane, 63 - __clzll(hm & le));
@@ -633,7 +644,13 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
     for (int cc = 0; cc < 8; cc++)
 #pragma unroll
       for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
-    put_runs<M, WS>(acc, amag, cur, lacc, sy, sz, lane);
+    // runs cut at every 16 lanes in P1 (no row_bcast steps), every 8 in P2
+    // (no row_shr:8 either), none in P3, whose deposit atomics already meet
+    // more address conflicts: measured best of 8 / 16 / 32 / none per pass
+    constexpr unsigned long long FH = MODE == 1   ? 0x0001000100010001ull
+                                      : MODE == 2 ? 0x0101010101010101ull
+                                                  : 0ull;
+    put_runs<M, WS, FH>(acc, amag, cur, lacc, sy, sz, lane);
   }
   if (misses) atomicAdd(dcnt, misses);
   if (MODE == 3) {
