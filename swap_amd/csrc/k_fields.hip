@@ -758,8 +758,8 @@ struct CorrLds {
 };
 
 // the corrector over tile (bx, by, bz); coef = dtsub; E and aj of the tile
-// are stored when Eout is set
-template <typename R, int TX, int TY, int NT>
+// are stored when Eout is set.  DEPTH: z steps its loads run ahead.
+template <typename R, int TX, int TY, int NT, int DEPTH>
 __device__ __forceinline__ void sub_correct_tile(const Geo& g, const FC& f,
                                                  const double* __restrict__ bh,
                                                  const double* __restrict__ b1,
@@ -799,37 +799,42 @@ __device__ __forceinline__ void sub_correct_tile(const Geo& g, const FC& f,
   own<KE, NT, PW>(tid, EW, EN, 2, 2, tcx, tcy, g.nx, ep, eq);
   const int olx = tid % TX, oly = tid / TX, ox = X0 + olx, oy = Y0 + oly;
   const bool oown = tid < TX * TY && ox <= g.ncx && oy <= g.ncy;
-  // step s's global data, fetched one step ahead: bh plane s+3 and np
-  // plane s+4 (staged), up plane s+3 at the curl positions, nu plane s+1 at
-  // the E positions, b1 at the output node of plane s
-  double fp[KL][3] = {}, fbase[3] = {};
-  R fn[KL] = {}, fu[KA][3] = {}, fnu[KE] = {};
-  auto fetch = [&](int s) {
+  // step s's global data, fetched DEPTH steps ahead into DEPTH register
+  // sets that the z loop takes in turn: bh plane s+3 and np plane s+4
+  // (staged), up plane s+3 at the curl positions, nu plane s+1 at the E
+  // positions, b1 at the output node of plane s
+  struct Fetched {
+    double fp[KL][3], fbase[3];
+    R fn[KL], fu[KA][3], fnu[KE];
+  } fs[DEPTH]{};
+  auto fetch = [&](Fetched& F, int s) {
     const int64_t pb = wrapc(s + 3, g.nz) * nxy, pn = wrapc(s + 4, g.nz) * nxy,
                   pe = wrapc(s + 1, g.nz) * nxy;
 #pragma unroll
     for (int k = 0; k < KL; k++)
       if (lp[k] >= 0) {
 #pragma unroll
-        for (int m = 0; m < 3; m++) fp[k][m] = bh[pb + lq[k] + m * nn];
-        fn[k] = np[pn + lq[k]];
+        for (int m = 0; m < 3; m++) F.fp[k][m] = bh[pb + lq[k] + m * nn];
+        F.fn[k] = np[pn + lq[k]];
       }
 #pragma unroll
     for (int k = 0; k < KA; k++)
       if (ap[k] >= 0)
 #pragma unroll
-        for (int m = 0; m < 3; m++) fu[k][m] = up[pb + aq[k] + m * nn];
+        for (int m = 0; m < 3; m++) F.fu[k][m] = up[pb + aq[k] + m * nn];
 #pragma unroll
     for (int k = 0; k < KE; k++)
-      if (ep[k] >= 0) fnu[k] = nu[pe + eq[k]];
+      if (ep[k] >= 0) F.fnu[k] = nu[pe + eq[k]];
     if (oown && s >= Z0) {
       const int64_t oc = gidx(g, ox, oy, s);
 #pragma unroll
-      for (int m = 0; m < 3; m++) fbase[m] = b1[oc + m * nn];
+      for (int m = 0; m < 3; m++) F.fbase[m] = b1[oc + m * nn];
     }
   };
-  fetch(Z0 - 5);
-  for (int s = Z0 - 5; s < Z1; s++) {
+#pragma unroll
+  for (int d = 0; d < DEPTH; d++)
+    if (Z0 - 5 + d < Z1) fetch(fs[d], Z0 - 5 + d);
+  auto step = [&](const int s, Fetched& F) {
     R cu[KA][3], cnu[KE];
     double cbase[3];
     {  // stage bh plane s+3 and np plane s+4, keep this step's up, nu, b1,
@@ -839,18 +844,18 @@ __device__ __forceinline__ void sub_correct_tile(const Geo& g, const FC& f,
       for (int k = 0; k < KL; k++)
         if (lp[k] >= 0) {
 #pragma unroll
-          for (int m = 0; m < 3; m++) sb[rb][m][lp[k]] = fp[k][m];
-          sn[rn][lp[k]] = fn[k];
+          for (int m = 0; m < 3; m++) sb[rb][m][lp[k]] = F.fp[k][m];
+          sn[rn][lp[k]] = F.fn[k];
         }
 #pragma unroll
       for (int k = 0; k < KA; k++)
 #pragma unroll
-        for (int m = 0; m < 3; m++) cu[k][m] = fu[k][m];
+        for (int m = 0; m < 3; m++) cu[k][m] = F.fu[k][m];
 #pragma unroll
-      for (int k = 0; k < KE; k++) cnu[k] = fnu[k];
+      for (int k = 0; k < KE; k++) cnu[k] = F.fnu[k];
 #pragma unroll
-      for (int m = 0; m < 3; m++) cbase[m] = fbase[m];
-      if (s + 1 < Z1) fetch(s + 1);
+      for (int m = 0; m < 3; m++) cbase[m] = F.fbase[m];
+      if (s + DEPTH < Z1) fetch(F, s + DEPTH);
     }
     __syncthreads();
     if (s >= Z0 - 4) {  // aj, a over [-1, T+1] and btc0 over [-1, T+2], plane s+3
@@ -968,8 +973,23 @@ __device__ __forceinline__ void sub_correct_tile(const Geo& g, const FC& f,
       put3<double>(g, out, ox, oy, s, cbase[0] - coef * c0, cbase[1] - coef * c1,
                    cbase[2] - coef * c2);
     }
+  };
+  for (int s = Z0 - 5; s < Z1; s += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++)
+      if (s + d < Z1) step(s + d, fs[d]);
   }
 }
+
+// z steps that the corrector's loads run ahead of their use: the corrector
+// holds one workgroup per CU (its LDS), 2 waves per SIMD in FP32, so the
+// registers are there for DEPTH fetched planes, and its z march was bound
+// by the latency of its loads (the field stage at 256x128x128, 10
+// subcycles: one step ahead 3.90 ms, two 3.60, three 3.55, four 3.49)
+template <typename R>
+struct CorrDepth {
+  static constexpr int value = 4;
+};
 
 // one predictor launch: a workgroup per tile; coef = cmul dtsub
 template <typename R, int TX, int TY, int NT>
@@ -997,7 +1017,7 @@ __global__ __launch_bounds__(NT) void k_sub_correct(Geo g, FC f, const double* _
                                                         R* __restrict__ Eout,
                                                         R* __restrict__ ajout) {
   __shared__ CorrLds<R, TX, TY> L;
-  sub_correct_tile<R, TX, TY, NT>(g, f, bh, b1, np, up, nu, cmul * ts[0], blockIdx.x,
+  sub_correct_tile<R, TX, TY, NT, CorrDepth<R>::value>(g, f, bh, b1, np, up, nu, cmul * ts[0], blockIdx.x,
                                   blockIdx.y, blockIdx.z, tz, out, Eout, ajout, L);
 }
 
@@ -1032,7 +1052,7 @@ __global__ __launch_bounds__(NT) void k_subcycles(Geo g, FC f, double* b1, doubl
     const bool last = q == nsub - 1;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
       __syncthreads();
-      sub_correct_tile<R, TX, TY, NT>(g, f, bh, b1, np, up, nu, 1.0 * dts, t % gx,
+      sub_correct_tile<R, TX, TY, NT, CorrDepth<R>::value>(g, f, bh, b1, np, up, nu, 1.0 * dts, t % gx,
                                       (t / gx) % gy, t / (gx * gy), tz, b12,
                                       last ? Eout : nullptr, last ? ajout : nullptr, L.c);
     }
