@@ -577,6 +577,25 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
     const int cur = inw ? lx + W[0] * (ly + W[1] * lz) : -1;
     float amag = inw ? pmag : 0.f;
     R d0 = f0, d1 = f1, d2 = f2;  // fractions at the deposit position
+    {
+      // the deposit: this particle's 8 corners x M moments (in P3 before
+      // the second half move and the stores, which it does not need:
+      // 7.56 -> 7.50 ms with 128 VGPRs and a 16-byte spill, against the
+      // deposit last)
+      weights8(d0, d1, d2, w8);
+      R acc[8][M];
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++)
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
+      // runs cut at every 16 lanes in P1 (no row_bcast steps), every 8 in P2
+      // (no row_shr:8 either), none in P3, whose deposit atomics already meet
+      // more address conflicts: measured best of 8 / 16 / 32 / none per pass
+      constexpr unsigned long long FH = MODE == 1   ? 0x0001000100010001ull
+                                        : MODE == 2 ? 0x0101010101010101ull
+                                                    : 0ull;
+      put_runs<M, WS, FH>(acc, amag, cur, lacc, sy, sz, lane);
+    }
     if constexpr (MODE == 3) {
       ux += half_move(f0, vn[0], K.s[0]);
       uy += half_move(f1, vn[1], K.s[1]);
@@ -637,20 +656,6 @@ __global__ __launch_bounds__(PASS_WG, (PassWaves<R>::value)) void k_pass(
         }
       }
     }
-    // the deposit: this particle's 8 corners x M moments
-    weights8(d0, d1, d2, w8);
-    R acc[8][M];
-#pragma unroll
-    for (int cc = 0; cc < 8; cc++)
-#pragma unroll
-      for (int m = 0; m < M; m++) acc[cc][m] = w8[cc] * q[m];
-    // runs cut at every 16 lanes in P1 (no row_bcast steps), every 8 in P2
-    // (no row_shr:8 either), none in P3, whose deposit atomics already meet
-    // more address conflicts: measured best of 8 / 16 / 32 / none per pass
-    constexpr unsigned long long FH = MODE == 1   ? 0x0001000100010001ull
-                                      : MODE == 2 ? 0x0101010101010101ull
-                                                  : 0ull;
-    put_runs<M, WS, FH>(acc, amag, cur, lacc, sy, sz, lane);
   }
   if (misses) atomicAdd(dcnt, misses);
   if (MODE == 3) {
