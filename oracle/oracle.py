@@ -108,7 +108,7 @@ class OrState(C.Structure):
 
 
 class Oracle:
-    """One oracle grid context.  `width` is 64 (FP64 judge) or 32."""
+    """One oracle grid context.  `width` is 64 (the FP64 reference arithmetic) or 32."""
 
     COORDS = ("qx", "qy", "qz", "dx_grid", "dy_grid", "dz_grid",
               "dx_cell", "dy_cell", "dz_cell", "xrat", "yrat", "zrat")

@@ -7,7 +7,7 @@
  * Fortran indices through the F3/F4 macros so every index can be checked
  * against the Fortran text.
  *
- * Compiled twice: OR_REAL=double (FP64 judge) and OR_REAL=float (the
+ * Compiled twice: OR_REAL=double (FP64 reference arithmetic) and OR_REAL=float (the
  * reference's own "-real-size 32" width, makefile:1).  Every literal goes
  * through R() so the float build really computes in float.
  */

@@ -5,8 +5,8 @@
  * in swap_amd/csrc.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  The product path never calls it.
  *
- * One source, two builds: -DOR_REAL=double (liboracle_f64.so, the FP64
- * judge) and -DOR_REAL=float (liboracle_f32.so, the reference's own
+ * One source, two builds: -DOR_REAL=double (liboracle_f64.so, FP64
+ * reference arithmetic) and -DOR_REAL=float (liboracle_f32.so, the reference's own
  * arithmetic width, makefile:1 "-real-size 32").
  *
  * Array conventions follow the reference exactly (Fortran column-major):

@@ -82,7 +82,7 @@ def test_disjoint_target_and_pair_transfer():
 
 def test_training_beats_plain_low_ppc_and_gate():
     """pairs from the code's own particles: a 4-ppc subsample (stride 16) of
-    a 64-ppc plasma with a pickup cloud; train on 16 pairs, judge on 8"""
+    a 64-ppc plasma with a pickup cloud; train on 16 pairs, evaluate on 8"""
     nc = (16, 16, 16)
     p, s = _ctx(nc, ppc=64, flags=api.TRACK_IDS | api.SPECIES,
                 capacity=16 * 16 * 16 * 64 + 400000)
