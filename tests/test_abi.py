@@ -97,3 +97,39 @@ def test_struct_layouts_match(cname, pyname):
     assert lay["sizeof"] == C.sizeof(cls)
     for f in fields:
         assert lay[f] == getattr(cls, f).offset, f
+
+
+def c_constants(names):
+    prog = ('#include <stdio.h>\n#include "swap_hip.h"\nint main(void){\n%s\n  return 0;\n}\n'
+            % "\n".join('  printf("%%s %%d\\n", "%s", (int)(%s));' % (n, n) for n in names))
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "k.c"), os.path.join(d, "k")
+        open(src, "w").write(prog)
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), src, "-o", exe],
+                       check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    return {k: int(v) for k, v in (ln.split() for ln in out.splitlines())}
+
+
+def test_python_constants_match_the_header():
+    """the ctypes mirror's config flags, field ids, particle-array ids,
+    velocity selectors and denoiser pair modes carry the header's values"""
+    from swap_amd import api
+    pairs = {
+        "SWAP_SPECIES": api.SPECIES, "SWAP_OUTFLOW_X": api.OUTFLOW_X,
+        "SWAP_TRACK_IDS": api.TRACK_IDS, "SWAP_UNFUSED_FIELDS": api.UNFUSED_FIELDS,
+        "SWAP_PERSISTENT_FIELDS": api.PERSISTENT_FIELDS,
+        "SWAP_FP32": api.FP32, "SWAP_FP64": api.FP64,
+        "SWAP_DN_ALL": api.DN_ALL, "SWAP_DN_DISJOINT": api.DN_DISJOINT,
+        "SWAP_DN_DISJOINT_STEP": api.DN_DISJOINT_STEP,
+        "SWAP_V_CUR": api.V_CUR, "SWAP_V_EXTRAP": api.V_EXTRAP, "SWAP_V_VP1": api.V_VP1,
+    }
+    for f in ("B1", "B12", "B1P2", "BTC", "NP", "UP", "AJ", "E", "NU", "AJC", "UPC", "TMP0",
+              "TMP1", "TMP2"):
+        pairs["SWAP_F_" + f] = getattr(api, f)
+    for p in ("XP", "VP", "VP1", "EP", "VPLUS", "VMINUS", "IJKP", "WGHT", "CELL", "FRAC",
+              "BETA", "MRAT", "ID"):
+        pairs["SWAP_P_" + p] = getattr(api, "P_" + p)
+    got = c_constants(sorted(pairs))
+    bad = {k: (got[k], v) for k, v in pairs.items() if got[k] != v}
+    assert not bad, bad
